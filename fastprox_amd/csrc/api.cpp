@@ -1,0 +1,1054 @@
+// The C ABI of libfastprox_rocm (include/fastprox_rocm.h): fastprox's image-TV
+// entry points over the HIP engine.
+//
+// Argument rules follow the reference's boundary (src/api/api.cpp): every
+// argument is checked before any work (a rejected call leaves the workspace as
+// it was, api.cpp:480-511), no exception crosses the boundary (guard,
+// api.cpp:28-38), and a workspace owns every buffer a solve needs
+// (build_workspace, api.cpp:223-291), so fp_prox allocates nothing.
+//
+// SPDX-License-Identifier: MIT
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/fastprox_rocm.h"
+#include "autotune.hpp"
+#include "deriv.hpp"
+#include "engine.hpp"
+
+#ifndef FASTPROX_ROCM_BUILD
+#define FASTPROX_ROCM_BUILD "1.0.6"
+#endif
+
+// ---------------------------------------------------------------------------
+// Operators and workspaces
+// ---------------------------------------------------------------------------
+
+struct fp_op {
+    bool iso = true;
+    int k = 2;
+    double w[3] = {1.0, 1.0, 1.0};
+    std::vector<double> spatial;  // iso: alpha (N); aniso: beta (k N); empty for none
+    int64_t sshape[3] = {1, 1, 1};
+    double spatial_max = 1.0;
+    // device copies of the spatial weights, per (device, dtype)
+    mutable std::mutex mu;
+    mutable std::vector<std::pair<int, void *>> dev_f32, dev_f64;
+    ~fp_op() {
+        for (auto &p : dev_f32) (void)hipFree(p.second);
+        for (auto &p : dev_f64) (void)hipFree(p.second);
+    }
+    int64_t nparams() const { return k + (int64_t)spatial.size(); }
+    double step_scale() const {
+        double m = 0.0;
+        for (int a = 0; a < 3; ++a) m = std::max(m, a < k ? w[a] : 0.0);
+        return m * (spatial.empty() ? 1.0 : spatial_max);
+    }
+    // The spatial weights on device dev in dtype (4 or 8 bytes); null without.
+    void *device_weights(int dev, int tsize, hipError_t &err) const {
+        err = hipSuccess;
+        if (spatial.empty()) return nullptr;
+        std::lock_guard<std::mutex> lock(mu);
+        auto &vec = tsize == 4 ? dev_f32 : dev_f64;
+        for (auto &p : vec)
+            if (p.first == dev) return p.second;
+        void *ptr = nullptr;
+        const size_t n = spatial.size();
+        if ((err = hipMalloc(&ptr, n * (size_t)tsize))) return nullptr;
+        if (tsize == 4) {
+            std::vector<float> f(spatial.begin(), spatial.end());
+            err = hipMemcpy(ptr, f.data(), n * 4, hipMemcpyHostToDevice);
+        } else {
+            err = hipMemcpy(ptr, spatial.data(), n * 8, hipMemcpyHostToDevice);
+        }
+        if (err) {
+            (void)hipFree(ptr);
+            return nullptr;
+        }
+        vec.emplace_back(dev, ptr);
+        return ptr;
+    }
+};
+
+namespace {
+
+using namespace fpr;
+
+constexpr double kMaxScaledStep = 0x1p960;  // the step range (op.hpp:48)
+constexpr int64_t kMaxElems = (int64_t(1) << 60) - 1;
+
+template <class F>
+fp_status guard(F &&f) noexcept {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return FP_ENOMEM;
+    } catch (...) {
+        return FP_EINTERNAL;
+    }
+}
+
+fp_status hip_status(int e) {
+    if (e == 0) return FP_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return FP_ENOMEM;
+    if (e == -1) return FP_EINTERNAL;
+    return FP_EDEVICE;
+}
+
+template <class E>
+int enum_bits(const E &e) noexcept {
+    int v;
+    std::memcpy(&v, &e, sizeof v);
+    return v;
+}
+
+bool mul_bounded(int64_t a, int64_t b, int64_t &out) {
+    if (a != 0 && b > kMaxElems / a) return false;
+    out = a * b;
+    return out <= kMaxElems;
+}
+
+struct Arr {
+    char *data = nullptr;
+    int dtype = FP_F64;
+    int ndim = 0;
+    int device = FP_DEVICE_CPU;
+    int64_t shape[FP_MAX_NDIM] = {};
+    int64_t strides[FP_MAX_NDIM] = {};
+    int64_t numel() const {
+        int64_t n = 1;
+        for (int d = 0; d < ndim; ++d) n *= shape[d];
+        return n;
+    }
+    size_t itemsize() const { return dtype == FP_F32 || dtype == FP_I32 ? 4 : (dtype == FP_I8 ? 1 : 8); }
+    bool contiguous() const {
+        int64_t s = 1;
+        for (int d = ndim - 1; d >= 0; --d) {
+            if (shape[d] != 1 && strides[d] != s) return false;
+            s *= shape[d];
+        }
+        return true;
+    }
+    // [lo, hi] byte range of the entries (numel > 0).
+    void span(intptr_t &lo, intptr_t &hi) const {
+        int64_t a = 0, b = 0;
+        for (int d = 0; d < ndim; ++d) {
+            if (shape[d] < 2) continue;
+            const int64_t e = (shape[d] - 1) * strides[d];
+            if (e < 0) a += e;
+            else b += e;
+        }
+        lo = (intptr_t)data + (intptr_t)(a * (int64_t)itemsize());
+        hi = (intptr_t)data + (intptr_t)((b + 1) * (int64_t)itemsize()) - 1;
+    }
+};
+
+// The reference's to_arr (api.cpp:60-108), with FP_DEVICE_ROCM accepted.
+fp_status to_arr(const fp_array *a, Arr &out, bool floating = true) noexcept {
+    if (!a) return FP_EINVAL;
+    const int dev = enum_bits(a->device);
+    if (dev != FP_DEVICE_CPU && dev != FP_DEVICE_ROCM) return FP_EINVAL;
+    const int dt = enum_bits(a->dtype);
+    if (floating ? !(dt == FP_F32 || dt == FP_F64) : !(dt == FP_I8 || dt == FP_I32 || dt == FP_I64)) return FP_EDTYPE;
+    if (a->ndim < 0 || a->ndim > FP_MAX_NDIM) return FP_ESHAPE;
+    out.data = static_cast<char *>(a->data);
+    out.dtype = dt;
+    out.device = dev;
+    if (a->ndim == 0) {
+        out.ndim = 1;
+        out.shape[0] = 1;
+        out.strides[0] = 1;
+    } else {
+        if (!a->shape) return FP_EINVAL;
+        out.ndim = a->ndim;
+        int64_t count = 1;
+        for (int d = 0; d < a->ndim; ++d) {
+            if (a->shape[d] < 0) return FP_ESHAPE;
+            out.shape[d] = a->shape[d];
+            out.strides[d] = a->strides ? a->strides[d] : 0;
+            if (a->shape[d] > 1 && !mul_bounded(count, a->shape[d], count)) return FP_ESHAPE;
+        }
+        if (!a->strides) {
+            int64_t s = 1;
+            for (int d = a->ndim - 1; d >= 0; --d) {
+                out.strides[d] = s;
+                s *= out.shape[d];
+            }
+        }
+        int64_t span = 0;
+        for (int d = 0; d < a->ndim; ++d) {
+            if (out.shape[d] < 2) continue;
+            const int64_t st = out.strides[d];
+            if (st < -kMaxElems || st > kMaxElems) return FP_ESHAPE;
+            int64_t e;
+            if (!mul_bounded(out.shape[d] - 1, st < 0 ? -st : st, e) || e > kMaxElems - span) return FP_ESHAPE;
+            span += e;
+        }
+    }
+    if (!out.data && out.numel() > 0) return FP_EINVAL;
+    return FP_OK;
+}
+
+// Copies any float array (host or device, any strides) into host doubles in C order.
+fp_status read_doubles(const Arr &x, std::vector<double> &v) {
+    const int64_t n = x.numel();
+    v.resize((size_t)n);
+    if (n == 0) return FP_OK;
+    intptr_t lo, hi;
+    x.span(lo, hi);
+    const size_t bytes = (size_t)(hi - lo + 1);
+    std::vector<char> host;
+    const char *base = nullptr;
+    if (x.device == FP_DEVICE_ROCM) {
+        host.resize(bytes);
+        if (hipMemcpy(host.data(), (const void *)lo, bytes, hipMemcpyDeviceToHost)) return FP_EDEVICE;
+        base = host.data() + ((intptr_t)x.data - lo);
+    } else {
+        base = x.data;
+    }
+    const int64_t item = (int64_t)x.itemsize();
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = i, off = 0;
+        for (int d = x.ndim - 1; d >= 0; --d) {
+            const int64_t c = r % x.shape[d];
+            r /= x.shape[d];
+            off += c * x.strides[d];
+        }
+        const char *p = base + off * item;
+        v[(size_t)i] = x.dtype == FP_F32 ? (double)*(const float *)p : *(const double *)p;
+    }
+    return FP_OK;
+}
+
+fp_status read_vec(const fp_array *a, std::vector<double> &v) {
+    v.clear();
+    if (!a) return FP_OK;
+    Arr x;
+    if (fp_status s = to_arr(a, x)) return s;
+    if (a->ndim > 1) return FP_ESHAPE;
+    if (x.shape[0] == 0) return FP_ESHAPE;
+    return read_doubles(x, v);
+}
+
+bool finite_nonneg(double v) { return std::isfinite(v) && v >= 0.0; }
+
+fp_status make_tv(bool iso, int ndim, const fp_array *axis_weights, const fp_array *spatial, fp_op **out) {
+    if (!out) return FP_EINVAL;
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) return FP_EPARAM;
+    std::vector<double> w;
+    if (fp_status s = read_vec(axis_weights, w)) return s;
+    double ww[3] = {1.0, 1.0, 1.0};
+    if (w.size() == 1) {
+        for (int a = 0; a < ndim; ++a) ww[a] = w[0];
+    } else if (!w.empty()) {
+        if ((int)w.size() != ndim) return FP_ESHAPE;
+        for (int a = 0; a < ndim; ++a) ww[a] = w[(size_t)a];
+    }
+    for (int a = 0; a < ndim; ++a)
+        if (!finite_nonneg(ww[a])) return FP_EPARAM;
+    std::unique_ptr<fp_op> op(new fp_op());
+    op->iso = iso;
+    op->k = ndim;
+    for (int a = 0; a < 3; ++a) op->w[a] = a < ndim ? ww[a] : 0.0;
+    if (spatial) {
+        Arr s;
+        if (fp_status st = to_arr(spatial, s)) return st;
+        const int want = iso ? ndim : ndim + 1;
+        if (spatial->ndim != want) return FP_ESHAPE;
+        if (!iso && s.shape[0] != ndim) return FP_ESHAPE;
+        for (int a = 0; a < ndim; ++a) op->sshape[a] = s.shape[iso ? a : a + 1];
+        double n = 1.0;
+        for (int a = 0; a < ndim; ++a) n *= (double)op->sshape[a];
+        if (!(n < 2147483647.0)) return FP_ESHAPE;
+        if (fp_status st = read_doubles(s, op->spatial)) return st;
+        if (op->spatial.empty()) return FP_ESHAPE;
+        const int64_t N = (int64_t)n;
+        double mx = 0.0;
+        for (size_t i = 0; i < op->spatial.size(); ++i) {
+            double &b = op->spatial[i];
+            if (!finite_nonneg(b)) return FP_EPARAM;
+            if (!iso) {  // the weight at the last entry along an axis belongs to no edge (tv_aniso.cpp:1447-1451)
+                const int a = (int)((int64_t)i / N);
+                const int64_t p = (int64_t)i % N;
+                int64_t stride = 1;
+                for (int c = ndim - 1; c > a; --c) stride *= op->sshape[c];
+                if ((p / stride) % op->sshape[a] == op->sshape[a] - 1) b = 0.0;
+            }
+            mx = std::max(mx, b);
+        }
+        op->spatial_max = mx;
+    }
+    *out = op.release();
+    return FP_OK;
+}
+
+}  // namespace
+
+struct fp_workspace {
+    const fp_op *op = nullptr;
+    int dtype = FP_F64;
+    int ndim = 0;
+    int64_t shape[FP_MAX_NDIM] = {};
+    int64_t numel = 0;
+    int k = 2;
+    int64_t B = 0, N = 0;
+    int64_t img[3] = {1, 1, 1};
+    int device = 0;
+    void *stream = nullptr;
+    bool keep = false;
+    size_t tsize = 8;
+    fpr::KernelChoice kc;
+    std::unique_ptr<fpr::Engine> eng;
+    void *dstage = nullptr, *zstage = nullptr, *q = nullptr;
+    void *xsave = nullptr, *zsave = nullptr;  // keep_structure: the point (x, z) of the last kept call
+    std::vector<char> host_stage;             // host arrays, packed
+    std::vector<double> gammas;
+    // results of the last call
+    bool dual_valid = false, warm = false, seeded = false;
+    bool info_valid = false;
+    fp_status info_status = FP_OK;
+    fp_info info{0.0, 0, 0};
+    std::vector<double> gaps, zerrs, smooth;
+    std::vector<int> iters;
+    std::vector<char> exact;
+    // derivatives
+    bool valid = false;      // a kept point exists
+    bool steps_on = false;
+    double gamma = 0.0;
+    double deriv_tol = 0.0;
+    int deriv_max_iter = 10000;
+    double deriv_res = 0.0;
+    int deriv_its = 0;
+    std::unique_ptr<fpr::Deriv> deriv;
+    ~fp_workspace() {
+        for (void *p : {dstage, zstage, q, xsave, zsave})
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+fp_status check_output(const Arr *in, const Arr &out) {
+    if (out.numel() == 0) return FP_OK;
+    for (int d = 0; d < out.ndim; ++d)
+        if (out.shape[d] > 1 && out.strides[d] == 0) return FP_EINVAL;
+    if (!in || in->numel() == 0 || in->device != out.device) return FP_OK;
+    bool same = in->data == out.data && in->ndim == out.ndim;
+    for (int d = 0; same && d < in->ndim; ++d)
+        same = in->shape[d] == out.shape[d] && (in->shape[d] < 2 || in->strides[d] == out.strides[d]);
+    if (same) return FP_OK;
+    intptr_t a0, a1, b0, b1;
+    in->span(a0, a1);
+    out.span(b0, b1);
+    return (a1 < b0 || b1 < a0) ? FP_OK : FP_EINVAL;
+}
+
+bool same_shape(const Arr &a, const Arr &b) {
+    if (a.ndim != b.ndim) return false;
+    for (int d = 0; d < a.ndim; ++d)
+        if (a.shape[d] != b.shape[d]) return false;
+    return true;
+}
+
+fp_options defaults() {
+    fp_options o;
+    fp_options_init(&o);
+    return o;
+}
+
+fp_status check_image(const fp_op &op, const int64_t *ishape) {
+    double n = 1.0;
+    for (int a = 0; a < op.k; ++a) n *= (double)ishape[a];
+    if (!op.spatial.empty())
+        for (int a = 0; a < op.k; ++a)
+            if (ishape[a] != op.sshape[a]) return FP_ESHAPE;
+    return n < 2147483647.0 ? FP_OK : FP_ESHAPE;
+}
+
+fp_status step_ok(const fp_op &op, double g) {
+    const double s = op.step_scale();
+    if (s > 0.0 && !(g * s <= kMaxScaledStep)) return FP_EPARAM;
+    return FP_OK;
+}
+
+fpr::StridedShape strided_of(const Arr &a) {
+    fpr::StridedShape sh{};
+    for (int d = 0; d < a.ndim; ++d) {
+        sh.shape[d] = a.shape[d];
+        sh.stride[d] = a.strides[d];
+    }
+    return sh;
+}
+
+// Host array -> contiguous host bytes of the array's dtype, C order.
+void pack_host(const Arr &a, char *dst) {
+    const int64_t n = a.numel(), item = (int64_t)a.itemsize();
+    if (a.contiguous()) {
+        std::memcpy(dst, a.data, (size_t)(n * item));
+        return;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = i, off = 0;
+        for (int d = a.ndim - 1; d >= 0; --d) {
+            const int64_t c = r % a.shape[d];
+            r /= a.shape[d];
+            off += c * a.strides[d];
+        }
+        std::memcpy(dst + i * item, a.data + off * item, (size_t)item);
+    }
+}
+
+void unpack_host(const char *src, const Arr &a) {
+    const int64_t n = a.numel(), item = (int64_t)a.itemsize();
+    if (a.contiguous()) {
+        std::memcpy(a.data, src, (size_t)(n * item));
+        return;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = i, off = 0;
+        for (int d = a.ndim - 1; d >= 0; --d) {
+            const int64_t c = r % a.shape[d];
+            r /= a.shape[d];
+            off += c * a.strides[d];
+        }
+        std::memcpy(a.data + off * item, src + i * item, (size_t)item);
+    }
+}
+
+// Copies the array a (host or device, any strides, the workspace's dtype) into
+// the contiguous device buffer dst.
+fp_status stage_in(fp_workspace &ws, const Arr &a, void *dst, hipStream_t s) {
+    const int64_t n = a.numel();
+    const size_t bytes = (size_t)n * ws.tsize;
+    if (a.device == FP_DEVICE_CPU) {
+        ws.host_stage.resize(bytes);
+        pack_host(a, ws.host_stage.data());
+        if (hipMemcpyAsync(dst, ws.host_stage.data(), bytes, hipMemcpyHostToDevice, s)) return FP_EDEVICE;
+        return hipStreamSynchronize(s) ? FP_EDEVICE : FP_OK;
+    }
+    if (a.contiguous()) return hipMemcpyAsync(dst, a.data, bytes, hipMemcpyDeviceToDevice, s) ? FP_EDEVICE : FP_OK;
+    if (a.ndim > fpr::kMaxStrided) return FP_ESHAPE;
+    const fpr::StridedShape sh = strided_of(a);
+    if (ws.tsize == 4) fpr::launch_gather<float>((const float *)a.data, (float *)dst, n, a.ndim, sh, s);
+    else fpr::launch_gather<double>((const double *)a.data, (double *)dst, n, a.ndim, sh, s);
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+fp_status stage_out(fp_workspace &ws, const void *src, const Arr &a, hipStream_t s) {
+    const int64_t n = a.numel();
+    const size_t bytes = (size_t)n * ws.tsize;
+    if (a.device == FP_DEVICE_CPU) {
+        ws.host_stage.resize(bytes);
+        if (hipMemcpyAsync(ws.host_stage.data(), src, bytes, hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        unpack_host(ws.host_stage.data(), a);
+        return FP_OK;
+    }
+    if (a.contiguous()) return hipMemcpyAsync(a.data, src, bytes, hipMemcpyDeviceToDevice, s) ? FP_EDEVICE : FP_OK;
+    if (a.ndim > fpr::kMaxStrided) return FP_ESHAPE;
+    const fpr::StridedShape sh = strided_of(a);
+    if (ws.tsize == 4) fpr::launch_scatter<float>((const float *)src, (float *)a.data, n, a.ndim, sh, s);
+    else fpr::launch_scatter<double>((const double *)src, (double *)a.data, n, a.ndim, sh, s);
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+fp_status build_engine(fp_workspace &w) {
+    const fp_op &op = *w.op;
+    fpr::Problem P{};
+    P.k = op.k;
+    P.iso = op.iso ? 1 : 0;
+    P.sw = op.spatial.empty() ? 0 : 1;
+    P.sw_per_image = 0;
+    for (int a = 0; a < 3; ++a) P.w[a] = op.w[a];
+    P.sched = op.iso ? fpr::kSchedC : fpr::kSchedDevice;
+    P.wmax_norm = op.iso ? 1 : 0;
+    P.alpha_max = op.spatial.empty() ? 1.0 : op.spatial_max;
+    std::vector<fpr::ImageSpec> specs((size_t)w.B);
+    for (int64_t b = 0; b < w.B; ++b) {
+        fpr::ImageSpec &s = specs[(size_t)b];
+        for (int a = 0; a < 3; ++a) s.L[a] = w.img[a];
+        s.d_off = b * w.N;
+        s.z_off = b * w.N;
+        s.q_off = b * w.N;
+        s.q_cstride = w.B * w.N;
+        s.sw_off = P.sw ? 0 : -1;
+        s.sw_cstride = w.N;
+    }
+    w.eng.reset(new fpr::Engine());
+    return hip_status(w.eng->setup((int)w.tsize, P, w.kc, specs));
+}
+
+fp_status alloc(void **p, size_t bytes) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 16))) return FP_ENOMEM;
+    return FP_OK;
+}
+
+fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *opt, fp_workspace **out) {
+    if (!out) return FP_EINVAL;
+    *out = nullptr;
+    if (!op || !like) return FP_EINVAL;
+    Arr x;
+    if (fp_status s = to_arr(like, x)) return s;
+    const fp_options o = opt ? *opt : defaults();
+    if (x.ndim < op->k) return FP_ESHAPE;
+    std::unique_ptr<fp_workspace> w(new fp_workspace());
+    w->op = op;
+    w->dtype = x.dtype;
+    w->ndim = x.ndim;
+    for (int d = 0; d < x.ndim; ++d) w->shape[d] = x.shape[d];
+    w->numel = x.numel();
+    w->k = op->k;
+    w->tsize = x.itemsize();
+    w->keep = o.keep_structure != 0;
+    int64_t N = 1;
+    for (int a = 0; a < op->k; ++a) {
+        w->img[3 - op->k + a] = x.shape[x.ndim - op->k + a];
+        N *= x.shape[x.ndim - op->k + a];
+    }
+    w->N = N;
+    w->B = N > 0 ? w->numel / N : 0;
+    if (hipGetDevice(&w->device)) return FP_EDEVICE;
+    const int64_t B = w->B;
+    w->gaps.assign((size_t)B, 0.0);
+    w->zerrs.assign((size_t)B, 0.0);
+    w->smooth.assign((size_t)B, 0.0);
+    w->iters.assign((size_t)B, 0);
+    w->exact.assign((size_t)B, 1);
+    w->gammas.assign((size_t)B, 0.0);
+    if (w->numel == 0) {
+        *out = w.release();
+        return FP_OK;
+    }
+    if (fp_status s = check_image(*op, x.shape + (x.ndim - op->k))) return s;
+    w->kc = fpr::default_kernel(op->k, (int)w->tsize, w->img, op->iso, !op->spatial.empty(), B);
+    if (fp_status s = build_engine(*w)) return s;
+    const size_t bytes = (size_t)w->numel * w->tsize;
+    if (fp_status s = alloc(&w->dstage, bytes)) return s;
+    if (fp_status s = alloc(&w->zstage, bytes)) return s;
+    if (fp_status s = alloc(&w->q, bytes * (size_t)op->k)) return s;
+    if (w->keep) {
+        if (fp_status s = alloc(&w->xsave, bytes)) return s;
+        if (fp_status s = alloc(&w->zsave, bytes)) return s;
+    }
+    hipError_t err;
+    (void)op->device_weights(w->device, (int)w->tsize, err);
+    if (err) return hip_status(err);
+    *out = w.release();
+    return FP_OK;
+}
+
+fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *oa, const fp_options *opt,
+                    fp_workspace *ws, fp_info *info, const fp_array *sa) {
+    if (!op || !xa || !oa) return FP_EINVAL;
+    if (!sa && (!std::isfinite(step) || step < 0)) return FP_EPARAM;
+    const fp_options o = opt ? *opt : defaults();
+    if (o.tie != 0 && o.tie != 1) return FP_EINVAL;
+    Arr x, out;
+    if (fp_status s = to_arr(xa, x)) return s;
+    if (fp_status s = to_arr(oa, out)) return s;
+    if (x.dtype != out.dtype) return FP_EDTYPE;
+    if (!same_shape(x, out)) return FP_ESHAPE;
+    if (fp_status s = check_output(&x, out)) return s;
+    const int k = op->k;
+    const bool keep = o.keep_structure != 0;
+    if (keep && !ws) return FP_EINVAL;
+    if (x.ndim < k) return FP_ESHAPE;
+    std::vector<double> steps;
+    Arr st;
+    if (sa) {
+        if (fp_status s = to_arr(sa, st)) return s;
+        const int nb = x.ndim - k;
+        if (sa->ndim != nb) return FP_ESHAPE;
+        for (int d = 0; d < nb; ++d)
+            if (st.shape[d] != x.shape[d]) return FP_ESHAPE;
+        if (fp_status s = check_output(&st, out)) return s;
+    }
+    if (ws) {
+        if (ws->op != op || ws->dtype != x.dtype || ws->ndim != x.ndim) return FP_EINVAL;
+        for (int d = 0; d < x.ndim; ++d)
+            if (ws->shape[d] != x.shape[d]) return FP_ESHAPE;
+        if (keep && !ws->keep) return FP_EINVAL;
+    }
+    if (info) *info = fp_info{0.0, 0, 0};
+    if (!(o.tol >= 0.0) || !std::isfinite(o.tol) || o.max_iter < 1) return FP_EPARAM;
+    if (sa) {
+        int64_t count = 1;
+        for (int d = 0; d < sa->ndim; ++d) count *= st.shape[d];
+        if (count > 0) {
+            if (fp_status s = read_doubles(st, steps)) return s;
+            for (double v : steps) {
+                if (!std::isfinite(v) || v < 0) return FP_EPARAM;
+                if (fp_status s = step_ok(*op, v)) return s;
+            }
+        }
+    } else if (fp_status s = step_ok(*op, step)) {
+        return s;
+    }
+    if (x.numel() != 0)
+        if (fp_status s = check_image(*op, x.shape + (x.ndim - k))) return s;
+    // Every argument is checked: from here the call replaces the saved point.
+    std::unique_ptr<fp_workspace> local;
+    if (!ws) {
+        fp_workspace *w = nullptr;
+        fp_options lo = o;
+        lo.keep_structure = 0;
+        if (fp_status s = create_impl(op, xa, &lo, &w)) return s;
+        local.reset(w);
+        ws = w;
+    }
+    ws->valid = false;
+    ws->steps_on = false;
+    ws->info_valid = false;
+    ws->deriv.reset();
+    if (x.numel() == 0) {
+        ws->dual_valid = true;
+        ws->info = fp_info{0.0, 0, 0};
+        ws->info_status = FP_OK;
+        ws->info_valid = true;
+        if (keep) {
+            ws->valid = true;
+            ws->gamma = step;
+        }
+        return FP_OK;
+    }
+    int cur_dev = 0;
+    (void)hipGetDevice(&cur_dev);
+    if (cur_dev != ws->device && hipSetDevice(ws->device)) return FP_EDEVICE;
+    struct Restore {
+        int dev, was;
+        ~Restore() {
+            if (dev != was) (void)hipSetDevice(was);
+        }
+    } restore{ws->device, cur_dev};
+    hipStream_t s = (hipStream_t)ws->stream;
+    const int64_t B = ws->B;
+    for (int64_t b = 0; b < B; ++b) ws->gammas[(size_t)b] = sa ? steps[(size_t)b] : step;
+    // Inputs: the caller's device array itself when contiguous and not
+    // written by this call, else staged.
+    const bool x_direct = x.device == FP_DEVICE_ROCM && x.contiguous() && x.data != out.data;
+    const void *d = x.data;
+    if (!x_direct) {
+        if (fp_status st2 = stage_in(*ws, x, ws->dstage, s)) return st2;
+        d = ws->dstage;
+    }
+    const bool z_direct = out.device == FP_DEVICE_ROCM && out.contiguous();
+    void *z = z_direct ? (void *)out.data : ws->zstage;
+    hipError_t err;
+    const void *sw = op->device_weights(ws->device, (int)ws->tsize, err);
+    if (err) return hip_status(err);
+    const bool warm = ws->dual_valid && (ws->warm || ws->seeded);
+    int r = ws->eng->solve(s, d, z, ws->q, sw, ws->gammas.data(), warm, o.tol, o.max_iter);
+    ws->seeded = false;
+    if (r) {
+        ws->dual_valid = false;
+        return hip_status(r);
+    }
+    if (!z_direct)
+        if (fp_status st2 = stage_out(*ws, z, out, s)) return st2;
+    // Per-image results (driver.hpp:1216-1236, 1331-1341).
+    const fpr::ImgState *S = ws->eng->state();
+    bool bad = false, notconv = false;
+    double gsum = 0.0;
+    int itmax = 0;
+    for (int64_t b = 0; b < B; ++b) {
+        const fpr::ImgState &t = S[b];
+        if (t.status) bad = true;
+        double g = t.gp;
+        if (op->iso && g == g) g = g < 0.0 ? 0.0 : g;
+        if (op->iso && t.e) g = std::ldexp(g, 2 * t.e);
+        ws->gaps[(size_t)b] = g;
+        ws->iters[(size_t)b] = t.it;
+        ws->zerrs[(size_t)b] = t.zerr;
+        if (!t.converged) notconv = true;
+        gsum += g;
+        itmax = std::max(itmax, t.it);
+    }
+    if (bad) {
+        ws->dual_valid = false;
+        return FP_EINVAL;
+    }
+    ws->dual_valid = true;
+    const fp_status status = notconv ? FP_ENOTCONV : FP_OK;
+    ws->info = fp_info{gsum, itmax, 0};
+    ws->info_status = status;
+    ws->info_valid = true;
+    if (info) *info = ws->info;
+    if (keep) {
+        const size_t bytes = (size_t)ws->numel * ws->tsize;
+        if (hipMemcpyAsync(ws->xsave, d, bytes, hipMemcpyDeviceToDevice, s)) return FP_EDEVICE;
+        if (hipMemcpyAsync(ws->zsave, z, bytes, hipMemcpyDeviceToDevice, s)) return FP_EDEVICE;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        ws->valid = true;
+        ws->steps_on = sa != nullptr;
+        ws->gamma = step;
+    }
+    if (!z_direct || !x_direct)
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    return status;
+}
+
+// The dual between the workspace (k planes of B N, x's dtype) and an array
+// of shape (k, *x.shape) (api.cpp:1151-1180, 1235-1264).
+fp_status dual_impl(fp_workspace *ws, const fp_array *aa, bool to_ws) {
+    if (!ws || !aa || !ws->op) return FP_EINVAL;
+    Arr a;
+    if (fp_status s = to_arr(aa, a)) return s;
+    if (aa->ndim != ws->ndim + 1 || a.shape[0] != ws->k) return FP_ESHAPE;
+    for (int d = 0; d < ws->ndim; ++d)
+        if (a.shape[d + 1] != ws->shape[d]) return FP_ESHAPE;
+    if (!to_ws && !ws->dual_valid) return FP_EINVAL;
+    if (!to_ws)
+        if (fp_status s = check_output(nullptr, a)) return s;
+    if (ws->numel == 0) {
+        if (to_ws) ws->dual_valid = ws->seeded = true;
+        return FP_OK;
+    }
+    hipStream_t s = (hipStream_t)ws->stream;
+    const int64_t n = ws->numel * ws->k;
+    // Through doubles on the host when dtypes differ or the array is on the
+    // host; device to device otherwise.
+    if (a.device == FP_DEVICE_ROCM && a.dtype == ws->dtype) {
+        fp_status st = to_ws ? stage_in(*ws, a, ws->q, s) : stage_out(*ws, ws->q, a, s);
+        if (st) return st;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    } else {
+        std::vector<double> v;
+        if (to_ws) {
+            if (fp_status st = read_doubles(a, v)) return st;
+        } else {
+            std::vector<char> raw((size_t)n * ws->tsize);
+            if (hipMemcpy(raw.data(), ws->q, raw.size(), hipMemcpyDeviceToHost)) return FP_EDEVICE;
+            v.resize((size_t)n);
+            for (int64_t i = 0; i < n; ++i)
+                v[(size_t)i] = ws->tsize == 4 ? (double)((const float *)raw.data())[i] : ((const double *)raw.data())[i];
+            // write v into a (host or device) in a's dtype and strides
+            std::vector<char> packed((size_t)n * a.itemsize());
+            for (int64_t i = 0; i < n; ++i) {
+                if (a.dtype == FP_F32) ((float *)packed.data())[i] = (float)v[(size_t)i];
+                else ((double *)packed.data())[i] = v[(size_t)i];
+            }
+            if (a.device == FP_DEVICE_CPU) {
+                unpack_host(packed.data(), a);
+            } else {
+                void *tmp = nullptr;
+                if (hipMalloc(&tmp, packed.size())) return FP_ENOMEM;
+                fp_status st = FP_OK;
+                if (hipMemcpy(tmp, packed.data(), packed.size(), hipMemcpyHostToDevice)) st = FP_EDEVICE;
+                const size_t saved = ws->tsize;
+                ws->tsize = a.itemsize();
+                if (!st) st = stage_out(*ws, tmp, a, s);
+                ws->tsize = saved;
+                if (!st && hipStreamSynchronize(s)) st = FP_EDEVICE;
+                (void)hipFree(tmp);
+                if (st) return st;
+            }
+            return FP_OK;
+        }
+        std::vector<char> raw((size_t)n * ws->tsize);
+        for (int64_t i = 0; i < n; ++i) {
+            const double val = v[(size_t)i];
+            if (ws->tsize == 4) ((float *)raw.data())[i] = (float)val;
+            else ((double *)raw.data())[i] = val;
+        }
+        if (hipMemcpy(ws->q, raw.data(), raw.size(), hipMemcpyHostToDevice)) return FP_EDEVICE;
+    }
+    if (to_ws) {
+        // A seed must not be infinite; NaN entries are stored as 0 (api.cpp:1241-1256).
+        std::vector<char> raw((size_t)n * ws->tsize);
+        if (hipMemcpy(raw.data(), ws->q, raw.size(), hipMemcpyDeviceToHost)) return FP_EDEVICE;
+        bool ok = true, fix = false;
+        for (int64_t i = 0; i < n; ++i) {
+            double v = ws->tsize == 4 ? (double)((float *)raw.data())[i] : ((double *)raw.data())[i];
+            if (std::isinf(v)) ok = false;
+            if (v != v) {
+                fix = true;
+                if (ws->tsize == 4) ((float *)raw.data())[i] = 0.0f;
+                else ((double *)raw.data())[i] = 0.0;
+            }
+        }
+        if (!ok) {
+            ws->dual_valid = false;
+            return FP_EINVAL;
+        }
+        if (fix && hipMemcpy(ws->q, raw.data(), raw.size(), hipMemcpyHostToDevice)) return FP_EDEVICE;
+        ws->dual_valid = true;
+        ws->seeded = true;
+    }
+    return FP_OK;
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+fp_status op_check(const fp_op *op, int *row_parallel_only) {
+    if (!op) return FP_EINVAL;
+    if (row_parallel_only) *row_parallel_only = 0;
+    return FP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exported entry points
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+FP_API void fp_options_init(fp_options *opt) {
+    if (!opt) return;
+    opt->axis = -1;
+    opt->threads = 0;
+    opt->tol = 1e-6;
+    opt->max_iter = 1000;
+    opt->tie = 0;
+    opt->keep_structure = 0;
+}
+
+FP_API fp_status fp_op_tv_aniso(int ndim, const fp_array *axis_weights, fp_op **out) {
+    return guard([&] { return make_tv(false, ndim, axis_weights, nullptr, out); });
+}
+
+FP_API fp_status fp_op_tv_aniso_weighted(int ndim, const fp_array *axis_weights, const fp_array *edge_weights,
+                                         fp_op **out) {
+    return guard([&] {
+        if (!out) return FP_EINVAL;
+        *out = nullptr;
+        if (!edge_weights) return FP_EINVAL;
+        return make_tv(false, ndim, axis_weights, edge_weights, out);
+    });
+}
+
+FP_API fp_status fp_op_tv_iso(int ndim, const fp_array *axis_weights, fp_op **out) {
+    return guard([&] { return make_tv(true, ndim, axis_weights, nullptr, out); });
+}
+
+FP_API fp_status fp_op_tv_iso_weighted(int ndim, const fp_array *axis_weights, const fp_array *pixel_weights,
+                                       fp_op **out) {
+    return guard([&] {
+        if (!out) return FP_EINVAL;
+        *out = nullptr;
+        if (!pixel_weights) return FP_EINVAL;
+        return make_tv(true, ndim, axis_weights, pixel_weights, out);
+    });
+}
+
+FP_API void fp_op_free(fp_op *op) { delete op; }
+FP_API int64_t fp_op_nparams(const fp_op *op) { return op ? op->nparams() : 0; }
+FP_API const char *fp_op_name(const fp_op *op) { return op ? (op->iso ? "tv_iso" : "tv_aniso") : nullptr; }
+
+FP_API fp_status fp_workspace_create(const fp_op *op, const fp_array *like, const fp_options *opt,
+                                     fp_workspace **out) {
+    return guard([&] { return create_impl(op, like, opt, out); });
+}
+
+FP_API void fp_workspace_free(fp_workspace *ws) { delete ws; }
+
+FP_API fp_status fp_workspace_get_dual(const fp_workspace *ws, fp_array *out) {
+    return guard([&] { return dual_impl(const_cast<fp_workspace *>(ws), out, false); });
+}
+
+FP_API fp_status fp_workspace_set_dual(fp_workspace *ws, const fp_array *dual) {
+    return guard([&] { return dual_impl(ws, dual, true); });
+}
+
+FP_API fp_status fp_workspace_set_warm(fp_workspace *ws, int on) {
+    if (!ws) return FP_EINVAL;
+    ws->warm = on != 0;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_set_deriv_tol(fp_workspace *ws, double tol, int max_iter) {
+    if (!ws) return FP_EINVAL;
+    if (!(tol >= 0.0) || !std::isfinite(tol) || max_iter < 1) return FP_EPARAM;
+    ws->deriv_tol = tol;
+    ws->deriv_max_iter = max_iter;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_deriv_info(const fp_workspace *ws, double *residual, int *iters) {
+    if (!ws) return FP_EINVAL;
+    if (residual) *residual = ws->deriv_res;
+    if (iters) *iters = ws->deriv_its;
+    return FP_OK;
+}
+
+FP_API fp_status fp_prox(const fp_op *op, double gamma, const fp_array *x, fp_array *out, const fp_options *opt,
+                         fp_workspace *ws, fp_info *info) {
+    return guard([&] { return prox_impl(op, gamma, x, out, opt, ws, info, nullptr); });
+}
+
+FP_API fp_status fp_prox_steps(const fp_op *op, const fp_array *gamma, const fp_array *x, fp_array *out,
+                               const fp_options *opt, fp_workspace *ws) {
+    return guard([&] {
+        if (!gamma) return FP_EINVAL;
+        return prox_impl(op, 0.0, x, out, opt, ws, nullptr, gamma);
+    });
+}
+
+FP_API fp_status fp_jvp(const fp_op *op, const fp_workspace *ws, const fp_array *dx, double dgamma,
+                        const fp_array *dparams, fp_array *dout) {
+    return guard([&] { return fpr::jvp_impl(op, const_cast<fp_workspace *>(ws), dx, dgamma, dparams, dout); });
+}
+
+FP_API fp_status fp_vjp(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx, double *ggamma,
+                        fp_array *gparams) {
+    return guard([&] {
+        return fpr::vjp_impl(op, const_cast<fp_workspace *>(ws), g, gx, ggamma, nullptr, gparams);
+    });
+}
+
+FP_API fp_status fp_vjp_device(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx,
+                               fp_array *ggamma, fp_array *gparams) {
+    return guard([&] {
+        return fpr::vjp_impl(op, const_cast<fp_workspace *>(ws), g, gx, nullptr, ggamma, gparams);
+    });
+}
+
+FP_API int fp_device_available(fp_device dev) {
+    const int d = enum_bits(dev);
+    if (d == FP_DEVICE_CPU) return 1;
+    return d == FP_DEVICE_ROCM && device_count() > 0 ? 1 : 0;
+}
+
+FP_API fp_status fp_op_device_check(const fp_op *op, fp_device dev, int *row_parallel_only) {
+    if (!op) return FP_EINVAL;
+    const int d = enum_bits(dev);
+    if (d == FP_DEVICE_CPU) {
+        if (row_parallel_only) *row_parallel_only = 0;
+        return FP_OK;
+    }
+    if (d != FP_DEVICE_ROCM || device_count() == 0) return FP_EINVAL;
+    return op_check(op, row_parallel_only);
+}
+
+FP_API fp_status fp_workspace_set_stream(fp_workspace *ws, void *stream) {
+    if (!ws) return FP_EINVAL;
+    ws->stream = stream;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_get_stream(const fp_workspace *ws, void **stream) {
+    if (!ws || !stream) return FP_EINVAL;
+    *stream = ws->stream;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_info(const fp_workspace *ws, fp_info *info) {
+    if (!ws || !info || !ws->info_valid) return FP_EINVAL;
+    *info = ws->info;
+    return ws->info_status;
+}
+
+FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, double *smoothing) {
+    if (!ws || !ws->valid) return FP_EINVAL;
+    int all = 1;
+    double mu = 0.0;
+    for (size_t i = 0; i < ws->exact.size(); ++i) {
+        all = all && ws->exact[i];
+        mu = std::max(mu, ws->smooth[i]);
+    }
+    if (exact) *exact = all;
+    if (smoothing) *smoothing = mu;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_image_info(const fp_workspace *ws, double *gaps, int *iters, double *zerrs,
+                                         int64_t count) {
+    if (!ws || !ws->info_valid) return FP_EINVAL;
+    int64_t images = 1;
+    for (int d = 0; d < ws->ndim - ws->k; ++d) images *= ws->shape[d];
+    if (count != images) return FP_ESHAPE;
+    const bool held = ws->numel != 0;
+    for (int64_t r = 0; r < count; ++r) {
+        if (gaps) gaps[r] = held ? ws->gaps[(size_t)r] : 0.0;
+        if (iters) iters[r] = held ? ws->iters[(size_t)r] : 0;
+        if (zerrs) zerrs[r] = held ? ws->zerrs[(size_t)r] : 0.0;
+    }
+    return FP_OK;
+}
+
+FP_API const char *fp_strerror(fp_status s) {
+    switch (s) {
+        case FP_OK: return "success";
+        case FP_EINVAL: return "invalid argument";
+        case FP_EDTYPE: return "unsupported or mismatched dtype";
+        case FP_ESHAPE: return "mismatched shape or parameter length";
+        case FP_ENOMEM: return "out of memory";
+        case FP_ENOTCONV: return "iteration stopped before reaching the tolerance";
+        case FP_EPARAM: return "invalid operator parameter or step";
+        case FP_EINTERNAL: return "internal error";
+        case FP_EDEVICE: return "device error";
+    }
+    return "unknown status";
+}
+
+FP_API int fp_abi_version(void) { return FP_ABI_VERSION; }
+FP_API const char *fp_version_string(void) { return FASTPROX_ROCM_BUILD; }
+FP_API int fp_default_threads(void) { return 1; }
+
+static int plugin_device_count(void) { return device_count(); }
+static fp_status plugin_op_check(const fp_op *op, int *rpo) { return op_check(op, rpo); }
+
+FP_API const fpr_plugin_table *fastprox_rocm_plugin(void) {
+    static const fpr_plugin_table table = {FPR_PLUGIN_ABI, FASTPROX_ROCM_BUILD, plugin_device_count,
+                                           plugin_op_check};
+    return &table;
+}
+
+FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt, int ry) {
+    return guard([&] {
+        if (!ws) return FP_EINVAL;
+        fpr::KernelChoice kc;
+        kc.generic = generic != 0;
+        kc.TT = tt;
+        kc.RY = ry;
+        if (!kc.generic && (ws->k != 2 || !fpr::kernel_available((int)ws->tsize, tt, ry))) return FP_EINVAL;
+        if (kc.generic) {
+            kc.TT = 1;
+            kc.RY = 4;
+        }
+        ws->kc = kc;
+        if (ws->numel == 0) return FP_OK;
+        ws->dual_valid = false;
+        return build_engine(*ws);
+    });
+}
+
+FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, int *tt, int *ry) {
+    if (!ws) return FP_EINVAL;
+    if (generic) *generic = ws->kc.generic ? 1 : 0;
+    if (tt) *tt = ws->kc.TT;
+    if (ry) *ry = ws->kc.RY;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs) {
+    if (!ws) return FP_EINVAL;
+    if (launches) *launches = ws->eng ? ws->eng->stats.launches : 0;
+    if (host_syncs) *host_syncs = ws->eng ? ws->eng->stats.host_syncs : 0;
+    return FP_OK;
+}
+
+}  // extern "C"
+
+// Derivatives: not yet on the device.
+namespace fpr {
+struct Deriv {};
+fp_status jvp_impl(const fp_op *, fp_workspace *, const fp_array *, double, const fp_array *, fp_array *) {
+    return FP_EINVAL;
+}
+fp_status vjp_impl(const fp_op *, fp_workspace *, const fp_array *, fp_array *, double *, fp_array *, fp_array *) {
+    return FP_EINVAL;
+}
+}  // namespace fpr

@@ -1,0 +1,17 @@
+// The step-kernel choice per workspace shape (autotune.cpp).
+//
+// SPDX-License-Identifier: MIT
+#pragma once
+
+#include <cstdint>
+
+#include "tv_kernels.hpp"
+
+namespace fpr {
+
+// The kernel for images of extents img (padded to 3) in a batch of B: the
+// entry of the tuning table for (k, dtype, spatial weights) nearest to the
+// image size, the built-in default without one.
+KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B);
+
+}  // namespace fpr

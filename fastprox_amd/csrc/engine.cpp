@@ -1,0 +1,180 @@
+// The solve engine (engine.hpp).
+//
+// SPDX-License-Identifier: MIT
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+namespace fpr {
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+    auto f = [](void *p) {
+        if (p) (void)hipFree(p);
+    };
+    f(arena_);
+    f(desc_);
+    f(state_);
+    f(part_);
+    f(list_[0]);
+    f(list_[1]);
+    f(counts_);
+    f(gam_);
+    if (hstate_) (void)hipHostFree(hstate_);
+    if (hcount_) (void)hipHostFree(hcount_);
+    if (hgam_) (void)hipHostFree(hgam_);
+    arena_ = nullptr;
+    desc_ = nullptr;
+    state_ = nullptr;
+    part_ = nullptr;
+    list_[0] = list_[1] = nullptr;
+    counts_ = nullptr;
+    gam_ = nullptr;
+    hstate_ = nullptr;
+    hcount_ = nullptr;
+    hgam_ = nullptr;
+}
+
+int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std::vector<ImageSpec> &imgs) {
+    release();
+    tsize_ = tsize;
+    P_ = P;
+    kc_ = kc;
+    P_.TT = kc.iters();
+    specs_ = imgs;
+    const int B = (int)imgs.size();
+    hdesc_.assign((size_t)B, ImgDesc{});
+    int64_t arena = 0;
+    int tiles = 0;
+    maxN_ = 0;
+    tv1d_possible_ = false;
+    const int k = P.k;
+    for (int b = 0; b < B; ++b) {
+        const ImageSpec &s = imgs[(size_t)b];
+        ImgDesc &D = hdesc_[(size_t)b];
+        for (int a = 0; a < 3; ++a) D.L[a] = (int32_t)s.L[a];
+        const int64_t N = s.L[0] * s.L[1] * s.L[2];
+        maxN_ = std::max(maxN_, N);
+        D.d_off = s.d_off;
+        D.z_off = s.z_off;
+        D.q_off = s.q_off;
+        D.q_cstride = s.q_cstride;
+        D.sw_off = s.sw_off;
+        D.sw_cstride = s.sw_cstride;
+        D.st_off = arena;
+        arena += 4 * (int64_t)k * N;
+        D.ntx = (int32_t)((s.L[2] + kc.tile_x() - 1) / kc.tile_x());
+        D.nty = (int32_t)((s.L[1] + kc.tile_y() - 1) / kc.tile_y());
+        D.ntz = (int32_t)(kc.generic ? s.L[0] : 1);
+        D.ntiles = D.ntx * D.nty * D.ntz;
+        D.tile_base = tiles;
+        tiles += D.ntiles;
+        if (P.iso && !P.sw) {
+            int active = 0;
+            for (int a = 0; a < k; ++a)
+                if (s.L[3 - k + a] > 1 && P.w[a] > 0.0) ++active;
+            if (active == 1) tv1d_possible_ = true;
+        }
+    }
+    total_tiles_ = tiles;
+    hipError_t e;
+    if ((e = hipMalloc(&arena_, (size_t)std::max<int64_t>(arena, 1) * (size_t)tsize))) return e;
+    if ((e = hipMalloc((void **)&desc_, sizeof(ImgDesc) * (size_t)std::max(B, 1)))) return e;
+    if ((e = hipMalloc((void **)&state_, sizeof(ImgState) * (size_t)std::max(B, 1)))) return e;
+    if ((e = hipMalloc((void **)&part_, sizeof(double) * kSlots * (size_t)std::max(tiles, 1)))) return e;
+    for (int i = 0; i < 2; ++i)
+        if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)std::max(tiles, 1)))) return e;
+    if ((e = hipMalloc((void **)&counts_, 64))) return e;
+    if ((e = hipMalloc((void **)&gam_, sizeof(double) * (size_t)std::max(B, 1)))) return e;
+    if ((e = hipHostMalloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(B, 1), 0))) return e;
+    if ((e = hipHostMalloc((void **)&hcount_, 64, 0))) return e;
+    if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)std::max(B, 1), 0))) return e;
+    if (B > 0 && (e = hipMemcpy(desc_, hdesc_.data(), sizeof(ImgDesc) * (size_t)B, hipMemcpyHostToDevice)))
+        return e;
+    return hipSuccess;
+}
+
+template <class T>
+static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, const T *d, T *z, T *q,
+                     const T *sw, void *arena, ImgDesc *desc, ImgState *state, double *part, Tile **list,
+                     int *counts, double *gam, int B, int total_tiles, int64_t maxN, bool tv1d, int *hcount,
+                     int steps_per_sync, int max_iter) {
+    const double u_round = 0.5 * (sizeof(T) == 4 ? (double)FLT_EPSILON : DBL_EPSILON);
+    const double eta = 0.5 * (sizeof(T) == 4 ? (double)FLT_TRUE_MIN : DBL_TRUE_MIN);
+    launch_prologue<T>(P, d, desc, state, gam, B, s);
+    launch_reset_counts(counts, s);
+    launch_initial_list(desc, state, list[0], counts, B, s);
+    if (P.warm) launch_warm<T>(P, q, sw, (T *)arena, desc, state, B, maxN, s);
+    if (tv1d) launch_tv1d<T>(P, d, z, q, desc, state, B, maxN, u_round, eta, s);
+    StepArgs<T> A;
+    A.d = d;
+    A.arena = (T *)arena;
+    A.z = z;
+    A.q = q;
+    A.sw = sw;
+    A.desc = desc;
+    A.state = state;
+    A.part = part;
+    A.k = P.k;
+    int grid;
+    if (kc.generic) grid = std::max(1, std::min(total_tiles, 8192));
+    else grid = std::max(8, std::min(((total_tiles + 3) / 4 + 7) / 8 * 8, 2048));
+    const int TT = kc.iters();
+    // A bound on the launches any image can need: every accepted launch runs
+    // at least one iteration or ends the image, and each restart or stop
+    // costs at most two more.
+    const int64_t cap = 8LL * ((int64_t)max_iter / std::max(1, TT) + 8) + 64;
+    int p = 0;
+    int64_t launched = 0;
+    hipError_t err = hipGetLastError();
+    if (err) return err;
+    for (;;) {
+        for (int i = 0; i < steps_per_sync; ++i) {
+            A.list_in = list[p];
+            A.count_in = counts + p;
+            A.count_out = counts + (p ^ 1);
+            launch_step<T>(kc, P.iso != 0, P.sw != 0, A, grid, s);
+            launch_epilogue(desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s);
+            p ^= 1;
+            ++launched;
+        }
+        E.stats.launches += steps_per_sync;
+        if ((err = hipMemcpyAsync(hcount, counts + p, sizeof(int), hipMemcpyDeviceToHost, s))) return err;
+        if ((err = hipStreamSynchronize(s))) return err;
+        ++E.stats.host_syncs;
+        if (*hcount == 0) break;
+        if (launched > cap) return -1;  // the schedule did not finish: an internal error
+    }
+    return hipSuccess;
+}
+
+int Engine::solve(hipStream_t s, const void *d, void *z, void *q, const void *sw, const double *gammas, bool warm,
+                  double tol, int max_iter) {
+    const int B = nimg();
+    stats = SolveStats{};
+    if (B == 0) return hipSuccess;
+    P_.warm = warm ? 1 : 0;
+    P_.tol = tol;
+    P_.max_iter = max_iter;
+    std::memcpy(hgam_, gammas, sizeof(double) * (size_t)B);
+    hipError_t e = hipMemcpyAsync(gam_, hgam_, sizeof(double) * (size_t)B, hipMemcpyHostToDevice, s);
+    if (e) return e;
+    int r;
+    if (tsize_ == 4)
+        r = run_solve<float>(*this, P_, kc_, s, (const float *)d, (float *)z, (float *)q, (const float *)sw, arena_,
+                             desc_, state_, part_, list_, counts_, gam_, B, total_tiles_, maxN_, tv1d_possible_,
+                             hcount_, steps_per_sync, max_iter);
+    else
+        r = run_solve<double>(*this, P_, kc_, s, (const double *)d, (double *)z, (double *)q, (const double *)sw,
+                              arena_, desc_, state_, part_, list_, counts_, gam_, B, total_tiles_, maxN_,
+                              tv1d_possible_, hcount_, steps_per_sync, max_iter);
+    if (r) return r;
+    if ((e = hipMemcpyAsync(hstate_, state_, sizeof(ImgState) * (size_t)B, hipMemcpyDeviceToHost, s))) return e;
+    return hipStreamSynchronize(s);
+}
+
+}  // namespace fpr

@@ -1,0 +1,78 @@
+// The solve engine: device buffers for a batch of images and the host loop
+// that drives the prologue, the step/epilogue launch pairs and the readback.
+//
+// SPDX-License-Identifier: MIT
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "tv_common.hpp"
+#include "tv_kernels.hpp"
+
+namespace fpr {
+
+// One image of a batch: its extents (k axes), and where its data lives
+// relative to the base pointers of a solve (offsets in elements).
+struct ImageSpec {
+    int64_t L[3];       // extents, padded in front with 1
+    int64_t d_off, z_off, q_off, q_cstride, sw_off, sw_cstride;
+};
+
+struct SolveStats {
+    int launches = 0;   // step launches
+    int host_syncs = 0;
+};
+
+class Engine {
+public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // Allocates every device buffer for the batch: the states, the per-image
+    // descriptors and results, the tile lists and the partial sums. kc picks
+    // the step kernel. Returns a hipError_t.
+    int setup(int tsize, const Problem &P, const KernelChoice &kc, const std::vector<ImageSpec> &imgs);
+
+    // Solves every image: d, z, q and sw are the base pointers of the inputs,
+    // outputs, duals (read first when P.warm) and spatial weights. gammas:
+    // one step per image (host). Blocks until done; results in state().
+    int solve(hipStream_t s, const void *d, void *z, void *q, const void *sw, const double *gammas, bool warm,
+              double tol, int max_iter);
+
+    const ImgState *state() const { return hstate_; }
+    ImgState *device_state() const { return state_; }
+    int nimg() const { return (int)specs_.size(); }
+    int total_tiles() const { return total_tiles_; }
+    const KernelChoice &kernel() const { return kc_; }
+    const Problem &problem() const { return P_; }
+    SolveStats stats;
+    int steps_per_sync = 8;
+
+private:
+    void release();
+    int tsize_ = 4;
+    Problem P_{};
+    KernelChoice kc_{};
+    std::vector<ImageSpec> specs_;
+    std::vector<ImgDesc> hdesc_;
+    int total_tiles_ = 0;
+    int64_t maxN_ = 0;
+    bool tv1d_possible_ = false;
+    void *arena_ = nullptr;
+    ImgDesc *desc_ = nullptr;
+    ImgState *state_ = nullptr;
+    double *part_ = nullptr;
+    Tile *list_[2] = {nullptr, nullptr};
+    int *counts_ = nullptr;
+    double *gam_ = nullptr;
+    ImgState *hstate_ = nullptr;  // pinned
+    int *hcount_ = nullptr;       // pinned
+    double *hgam_ = nullptr;      // pinned
+};
+
+}  // namespace fpr

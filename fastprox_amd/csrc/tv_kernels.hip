@@ -1,0 +1,1288 @@
+// HIP kernels of the image-TV engine for gfx950 (CDNA4, wave64).
+//
+// The solve of a batch is a sequence of launch pairs:
+//   step      every tile of every live image runs up to TT dual-FISTA
+//             iterations (tv_iso.cpp:491-537 / _tv_torch.py:259-281) from the
+//             image's current state buffer into the other one, and writes the
+//             restart inner products and duality gaps its levels are asked for
+//             as FP64 partial sums (one slot per tile and level);
+//   epilogue  one wavefront per image sums those partials over the image's
+//             tiles in a fixed order, replays the schedule of the reference on
+//             them (restart test, interval rule, gap check), accepts the launch
+//             or rolls the image back to the first event inside it, and plans
+//             the image's next launch, appending its tiles to the next list.
+// A launch is speculative: it assumes no restart and no stop inside it. An
+// event at a level before the last makes the image re-run just that prefix
+// from its untouched input buffer, so every image follows the reference's
+// iteration sequence exactly; events are rare (a stop once per image,
+// restarts a few times per solve).
+//
+// The 2D step kernel keeps a tile in registers: a wavefront covers 64
+// columns (one per lane) and RY rows (registers), with a halo of TT on each
+// side, so TT iterations need one HBM round trip of the state; neighbours
+// along the row come from the adjacent lane (cross-lane shifts), neighbours
+// along the column from the lane's own registers. The generic one-iteration
+// kernel (step1) serves 3D images and reads its neighbours through the cache.
+//
+// Per pixel the arithmetic follows the reference's expressions in order
+// (tv_iso.cpp:251-288, 499-524), so in float64 with contraction off every
+// accepted iterate has the reference's bits; the reductions differ only in
+// their summation order.
+//
+// SPDX-License-Identifier: MIT
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "tv_common.hpp"
+#include "tv_kernels.hpp"
+
+namespace fpr {
+
+// ---------------------------------------------------------------------------
+// Small device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double fista_t(double t) { return 0.5 * (1.0 + sqrt(1.0 + 4.0 * t * t)); }
+
+__device__ __forceinline__ bool is_test(int sched, int n, int max_iter) {
+    return sched == kSchedC ? (n - 1) % 4 == 0 : (n % 4 == 0 || n == max_iter);
+}
+__device__ __forceinline__ bool is_gap(int n, int max_iter) { return n % 4 == 0 || n == max_iter; }
+
+template <class T>
+__device__ __forceinline__ T lane_prev(T v) {  // the value of lane - 1 (lane 0: its own)
+    return __shfl_up(v, 1, 64);
+}
+template <class T>
+__device__ __forceinline__ T lane_next(T v) {  // the value of lane + 1 (lane 63: its own)
+    return __shfl_down(v, 1, 64);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {  // fixed butterfly: the same tree on every call
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ T scale_in(T v, int e) {
+    return e ? (T)ldexp((double)v, -e) : v;
+}
+
+// The radius of pixel p with pixel weights: tv_iso.cpp:180-185.
+__device__ __forceinline__ double iso_radius(double gamma, double al, double bound) {
+    if (al == 0.0) return 0.0;
+    const double r = gamma * al;
+    return r <= bound ? r : bound;
+}
+
+// ---------------------------------------------------------------------------
+// Plan and walk (the schedule, per image, one thread)
+// ---------------------------------------------------------------------------
+
+__device__ void plan_next(ImgState &S, int TT) {
+    if (S.mode != kIterate) return;
+    const int rem = S.max_iter - S.it;
+    int cnt = S.sched == kSchedC ? ((1 - S.it) % TT + TT) % TT : ((-S.it) % TT + TT) % TT;
+    if (cnt == 0) cnt = TT;
+    if (cnt > rem) cnt = rem;
+    S.cnt = cnt;
+    double t = S.t;
+    uint32_t tm = 0, gm = S.pending ? 1u : 0u;
+    for (int j = 1; j <= cnt; ++j) {
+        const double tn = fista_t(t);
+        S.beta[j] = (t - 1.0) / tn;
+        t = tn;
+        const int n = S.it + j;
+        if (is_test(S.sched, n, S.max_iter)) tm |= 1u << j;
+        if (j < cnt && is_gap(n, S.max_iter)) gm |= 1u << j;
+    }
+    S.test_mask = tm;
+    S.gap_mask = gm;
+}
+
+__device__ void finish(ImgState &S, int64_t N, double u_round, double eta) {
+    S.mode = kDone;
+    const double gp = S.gp;
+    if (gp != gp) {
+        S.zerr = gp;
+        return;
+    }
+    // driver.hpp:1186-1212 on the solved scale.
+    const double e_ = sqrt(2.0 * (gp < 0.0 ? 0.0 : gp));
+    const double a = ldexp(eta, -S.e) * sqrt((double)N);
+    const double s = S.sumsq;
+    if (s >= 1e-280 && s <= 1e280) {
+        const double n = sqrt(s);
+        S.zerr = e_ / n + (u_round * (1.0 + e_ / n) + a / n);
+    } else {
+        const double m = S.maxabs;  // max |x_s|; sumsq_m = sum (x_s / m)^2
+        if (!(m > 0.0)) {
+            S.zerr = 0.0;
+            return;
+        }
+        const double rt = sqrt(S.sumsq_m);
+        const double r = e_ / m / rt;
+        S.zerr = r + (u_round * (1.0 + r) + a / m / rt);
+    }
+}
+
+// The walk over the levels 0..cnt of the launch just run, with gaps[j] and
+// dots[j] summed over the image's tiles; then accept or roll back, and plan.
+__device__ void walk(ImgState &S, const double *gaps, const double *dots, int TT, int64_t N, double u_round,
+                     double eta) {
+    if (S.mode == kOutput || S.mode == kOutputNaN) {
+        S.it += S.cnt;
+        S.converged = S.mode == kOutputNaN ? 1 : (S.gp <= S.target);
+        if (S.mode == kOutputNaN) S.it = 0;
+        finish(S, N, u_round, eta);
+        return;
+    }
+    if (S.mode != kIterate) return;
+    const int cnt = S.cnt;
+    double t = S.t, gp = S.gp;
+    int since = S.since;
+    int ev = 0, evj = -1;  // 1 stop, 2 restart
+    if (S.gap_mask & 1u) {
+        gp = gaps[0];
+        if (gp <= S.target || S.it == S.max_iter) {
+            ev = 1;
+            evj = 0;
+        }
+    }
+    for (int j = 1; j <= cnt && !ev; ++j) {
+        t = fista_t(t);
+        const int n = S.it + j;
+        if (S.sched == kSchedC) ++since;
+        if ((S.test_mask >> j) & 1u) {
+            const double s = dots[j];
+            const bool rs = S.sched == kSchedC ? (s > 0.0 && since >= S.last) : (s > 0.0);
+            if (rs) {
+                ev = 2;
+                evj = j;
+            }
+        }
+        if ((S.gap_mask >> j) & 1u) {
+            gp = gaps[j];
+            if (gp <= S.target || n == S.max_iter) {
+                ev = 1;
+                evj = j;
+            }
+        }
+    }
+    S.gp = gp;
+    if (ev == 1) {  // stop at level evj: an output launch re-runs that prefix
+        S.mode = kOutput;
+        S.cnt = evj;
+        S.test_mask = 0;
+        S.gap_mask = 0;
+        S.pending = 0;
+        return;
+    }
+    if (ev == 2 && evj < cnt) {  // restart inside: re-run the prefix, then restart
+        S.cnt = evj;
+        S.test_mask = 0;
+        S.gap_mask = 0;
+        S.pending = 0;
+        S.after = kAfterRestart;
+        return;
+    }
+    // Accept every level.
+    S.it += cnt;
+    if (cnt > 0) {
+        S.cur ^= 1;
+        S.fresh = 0;
+    }
+    S.t = t;
+    S.since = since;
+    S.vsrc_u = 0;
+    S.pending = (cnt > 0 && is_gap(S.it, S.max_iter)) ? 1 : 0;
+    if (ev == 2 || S.after == kAfterRestart) {
+        S.t = 1.0;
+        S.vsrc_u = 1;
+        if (S.sched == kSchedC) {
+            S.last = S.since;
+            S.since = 0;
+        }
+    }
+    S.after = kAfterNone;
+    plan_next(S, TT);
+}
+
+// ---------------------------------------------------------------------------
+// Epilogue: one wavefront per image
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void epilogue_kernel(const ImgDesc *desc, ImgState *state, const double *part,
+                                                      Tile *list_out, int *count_out, int TT, double u_round,
+                                                      double eta, int nimg) {
+    const int img = blockIdx.x;
+    if (img >= nimg) return;
+    const int lane = threadIdx.x;
+    ImgState &S = state[img];
+    const int mode = S.mode;
+    if (mode == kDone || mode == kTV1D) return;
+    const ImgDesc &D = desc[img];
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    __shared__ double gaps[kMaxTT + 1], dots[kMaxTT + 1];
+    const int cnt = S.cnt;
+    const uint32_t gm = S.gap_mask, tm = S.test_mask;
+    if (mode == kIterate) {
+        for (int j = 0; j <= cnt; ++j) {
+            const bool g = (gm >> j) & 1u, d = (tm >> j) & 1u;
+            if (!g && !d) continue;
+            double sg = 0.0, sd = 0.0;
+            for (int i = lane; i < D.ntiles; i += 64) {
+                const double *p = part + (int64_t)(D.tile_base + i) * kSlots + 2 * j;
+                if (g) sg += p[0];
+                if (d) sd += p[1];
+            }
+            sg = wave_sum(sg);
+            sd = wave_sum(sd);
+            if (lane == 0) {
+                gaps[j] = sg;
+                dots[j] = sd;
+            }
+        }
+    }
+    __syncthreads();
+    __shared__ int base;
+    if (lane == 0) {
+        walk(S, gaps, dots, TT, N, u_round, eta);
+        const int m = S.mode;
+        base = (m == kIterate || m == kOutput || m == kOutputNaN) ? atomicAdd(count_out, D.ntiles) : -1;
+    }
+    __syncthreads();
+    const int b = base;
+    if (b < 0) return;
+    const int nxy = D.ntx * D.nty;
+    for (int i = lane; i < D.ntiles; i += 64) {
+        Tile t;
+        t.img = img;
+        t.tz = i / nxy;
+        const int r = i - t.tz * nxy;
+        t.ty = (int16_t)(r / D.ntx);
+        t.tx = (int16_t)(r - t.ty * D.ntx);
+        t.pad_ = i;
+        list_out[b + i] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Prologue: one workgroup of 256 per image
+// ---------------------------------------------------------------------------
+
+__device__ double block_sum(double v, double *sh) {
+    const int tid = threadIdx.x;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double *sh) {
+    const int tid = threadIdx.x;
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if ((tid & 63) == 0) sh[tid >> 6] = v;
+    __syncthreads();
+    const double r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    __syncthreads();
+    return r;
+}
+
+__device__ int rescale_exponent(double m, double gamma, double step) {  // image_util.hpp:56-65
+    if (!(m > 0.0)) return 0;
+    int e = ilogb(m);
+    if (e >= -64 && e <= 64) return 0;
+    const double top = gamma < step ? step : gamma;
+    if (e < 0 && top > 0.0) {
+        const int lo = ilogb(top) + 1 - 960;
+        const int t = e > lo ? e : lo;
+        e = t < 0 ? t : 0;
+    }
+    return e;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void prologue_kernel(Problem P, const T *d, const ImgDesc *desc,
+                                                       ImgState *state, const double *gammas, int TT) {
+    const int img = blockIdx.x;
+    const int tid = threadIdx.x;
+    const ImgDesc &D = desc[img];
+    ImgState &S = state[img];
+    const int k = P.k;
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const T *x = d + D.d_off;
+    __shared__ double sh[4];
+    __shared__ double shared_e, shared_mean;
+    __shared__ int flags;
+    if (tid == 0) flags = 0;
+    __syncthreads();
+    // Pass 1: NaN, Inf, max |x|.
+    double m = 0.0;
+    int nan = 0, inf = 0;
+    for (int64_t i = tid; i < N; i += 256) {
+        const double v = (double)x[i];
+        nan |= v != v;
+        inf |= isinf(v);
+        m = fmax(m, fabs(v));
+    }
+    if (nan) atomicOr(&flags, 1);
+    if (inf) atomicOr(&flags, 2);
+    m = block_max(m, sh);
+    __syncthreads();
+    const int fl = flags;
+    // The image's axes (extents L[3-k..2]) and parameters.
+    const double gamma0 = gammas[img];
+    double wa[3] = {0.0, 0.0, 0.0}, L[3];
+    for (int a = 0; a < k; ++a) L[a] = (double)D.L[3 - k + a];
+    int e = 0, na = 0, act = -1;
+    if (tid == 0) {
+        if (P.iso) {
+            double wmaxL = 0.0;
+            for (int a = 0; a < k; ++a)
+                if (L[a] > 1) wmaxL = fmax(wmaxL, P.w[a]);
+            for (int a = 0; a < k; ++a)
+                if (gamma0 > 0.0 && P.w[a] > 0.0 && L[a] > 1 && P.w[a] >= 1e-100 * wmaxL) {
+                    wa[a] = P.w[a];
+                    act = a;
+                    ++na;
+                }
+            double ss = 0.0;
+            for (int a = 0; a < k; ++a) ss = fmax(ss, P.w[a]);
+            e = na ? rescale_exponent(m, gamma0, gamma0 * ss * P.alpha_max) : 0;
+            for (int a = 0; a < k && e; ++a)
+                if (wa[a] > 0.0 && !(ldexp(gamma0 * wa[a], -e) > 0.0)) e = 0;
+        }
+        if (fl) e = 0;
+        shared_e = (double)e;
+    }
+    __syncthreads();
+    e = (int)shared_e;
+    // Pass 2: sum and sum of squares of x_s = 2^-e x.
+    double s1 = 0.0, s2 = 0.0;
+    if (!fl)
+        for (int64_t i = tid; i < N; i += 256) {
+            const double v = ldexp((double)x[i], -e);
+            s1 += v;
+            s2 += v * v;
+        }
+    s1 = block_sum(s1, sh);
+    s2 = block_sum(s2, sh);
+    if (tid == 0) shared_mean = s1 / (double)N;
+    __syncthreads();
+    const double mean = shared_mean, ms = ldexp(m, -e);
+    // Pass 3: the imbalance sum |x_s - mean| and sum (x_s / max)^2.
+    double s3 = 0.0, s4 = 0.0;
+    if (!fl)
+        for (int64_t i = tid; i < N; i += 256) {
+            const double v = ldexp((double)x[i], -e);
+            s3 += fabs(v - mean);
+            const double r = ms > 0.0 ? v / ms : 0.0;
+            s4 += r * r;
+        }
+    s3 = block_sum(s3, sh);
+    s4 = block_sum(s4, sh);
+    if (tid != 0) return;
+    // Set the image up (tv_iso.cpp:333-415, 424-483; _tv_torch.py:229-257).
+    S.e = e;
+    S.it = 0;
+    S.since = 0;
+    S.last = 0;
+    S.cur = 0;
+    S.vsrc_u = 0;
+    S.fresh = 1;
+    S.cnt = 0;
+    S.test_mask = 0;
+    S.gap_mask = 0;
+    S.pending = 0;
+    S.after = kAfterNone;
+    S.sched = P.sched;
+    S.max_iter = P.max_iter;
+    S.converged = 1;
+    S.status = 0;
+    S.t = 1.0;
+    S.gp = INFINITY;
+    S.maxabs = ms;
+    S.sumsq = s2;
+    S.sumsq_m = s4;
+    S.zerr = 0.0;
+    S.bound = INFINITY;
+    S.wmax = 1.0;
+    S.gamma_raw = ldexp(gamma0, -e);
+    S.target = P.tol * 0.5 * s2;
+    S.active_axis = -1;
+    S.tv1d_lam = 0.0;
+    for (int a = 0; a < 3; ++a) S.wa[a] = 0.0;
+    if (fl & 2) {  // an infinite entry: the call fails (FP_EINVAL)
+        S.status = 1;
+        S.mode = kDone;
+        return;
+    }
+    if (fl & 1) {  // not solved: NaN outputs, gap NaN, 0 iterations
+        S.mode = kOutputNaN;
+        S.gp = NAN;
+        return;
+    }
+    const double gamma = S.gamma_raw;
+    if (P.iso) {
+        if (na == 0) {  // the identity
+            S.mode = kOutput;
+            S.gp = 0.0;
+            return;
+        }
+        if (na == 1 && !P.sw) {  // exactly 1D TV along the active axis
+            S.mode = kTV1D;
+            S.active_axis = act;
+            S.tv1d_lam = gamma * P.w[act];
+            S.gp = 0.0;
+            return;
+        }
+        double wmax = 0.0, whmin = 1.0;
+        for (int a = 0; a < k; ++a) wmax = wmax < wa[a] ? wa[a] : wmax;
+        for (int a = 0; a < k; ++a)
+            if (wa[a] != 0.0) {
+                wa[a] = wa[a] / wmax;
+                whmin = wa[a] < whmin ? wa[a] : whmin;
+            }
+        double g = gamma * wmax;
+        const double bound = 2.0 * s3 / whmin + DBL_MIN;
+        if (!(g <= bound) && !P.sw) g = bound;
+        double L2 = 0.0;
+        for (int a = 0; a < k; ++a) L2 += 4.0 * wa[a] * wa[a];
+        S.tau = 1.0 / L2;
+        S.gamma = g;
+        S.bound = bound;
+        S.wmax = wmax;
+        for (int a = 0; a < k; ++a) S.wa[a] = wa[a];
+        S.pending = P.warm ? 1 : 0;
+    } else {
+        double L2 = 0.0;
+        for (int a = 0; a < k; ++a) {
+            S.wa[a] = L[a] > 1 ? P.w[a] : 0.0;
+            L2 += 4.0 * P.w[a] * P.w[a];
+        }
+        S.target = P.tol * s2 / 2.0;
+        if (L2 == 0.0 || !(gamma > 0.0)) {  // _tv_torch.py:247-250: the zero dual, z = x
+            S.mode = kOutput;
+            S.gp = 0.0;
+            S.gamma = 0.0;
+            S.tau = 0.0;
+            return;
+        }
+        S.tau = 1.0 / L2;
+        S.gamma = gamma;
+    }
+    S.mode = kIterate;
+    S.fresh = P.warm ? 0 : 1;
+    plan_next(S, TT);
+}
+
+// ---------------------------------------------------------------------------
+// Warm starts: the dual buffer to the initial state (buffer 0, u and v)
+// ---------------------------------------------------------------------------
+
+// Iso: u = dual * wmax * 2^-e, zero on inactive axes, past the last entry and
+// where not finite, then projected onto the pixel's ball (tv_iso.cpp:456-477).
+template <class T>
+__global__ void warm_iso_kernel(Problem P, const T *q, const T *sw, T *arena, const ImgDesc *desc,
+                                const ImgState *state, int nimg) {
+    const int img = blockIdx.y;
+    if (img >= nimg) return;
+    const ImgState &S = state[img];
+    if (S.mode != kIterate) return;
+    const ImgDesc &D = desc[img];
+    const int k = P.k;
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < N; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c2 = p % D.L[2], c1 = (p / D.L[2]) % D.L[1], c0 = p / ((int64_t)D.L[2] * D.L[1]);
+        const int64_t cc[3] = {c0, c1, c2};
+        double u[3];
+        double nn = 0.0;
+        for (int a = 0; a < k; ++a) {
+            const int ax = 3 - k + a;
+            double v = ldexp((double)q[D.q_off + a * D.q_cstride + p] * S.wmax, -S.e);
+            const bool has = cc[ax] + 1 < D.L[ax];
+            if (S.wa[a] == 0.0 || !has || !isfinite(v)) v = 0.0;
+            u[a] = v;
+            nn += v * v;
+        }
+        double rp = S.gamma;
+        if (P.sw) rp = iso_radius(S.gamma, (double)sw[D.sw_off + p], S.bound);
+        if (sqrt(nn) > rp) {
+            const double sc = rp / sqrt(nn);
+            for (int a = 0; a < k; ++a) u[a] *= sc;
+        }
+        T *U = arena + D.st_off;
+        for (int a = 0; a < k; ++a) {
+            U[a * N + p] = (T)u[a];
+            U[(k + a) * N + p] = (T)u[a];
+        }
+    }
+}
+
+// Aniso: u_a = -cumsum_a(q_a) / w_a along every fibre, clamped to the box,
+// zero at the last entry (_tv_torch.py:596-606, 242-243). One thread per fibre.
+template <class T>
+__global__ void warm_aniso_kernel(Problem P, const T *q, const T *sw, T *arena, const ImgDesc *desc,
+                                  const ImgState *state, int nimg) {
+    const int img = blockIdx.y;
+    if (img >= nimg) return;
+    const ImgState &S = state[img];
+    if (S.mode != kIterate) return;
+    const ImgDesc &D = desc[img];
+    const int k = P.k;
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const int64_t Ls[3] = {D.L[0], D.L[1], D.L[2]};
+    const int64_t St[3] = {(int64_t)D.L[1] * D.L[2], D.L[2], 1};
+    T *U = arena + D.st_off;
+    for (int a = 0; a < k; ++a) {
+        const int ax = 3 - k + a;
+        const int64_t L = Ls[ax], s = St[ax], nf = N / L;
+        for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
+            // fibre f: its start has coordinate 0 along ax
+            const int64_t outer = f / s, inner = f % s;
+            const int64_t st = outer * L * s + inner;
+            double cs = 0.0;
+            for (int64_t i = 0; i < L; ++i) {
+                const int64_t o = st + i * s;
+                cs += (double)q[D.q_off + a * D.q_cstride + o];
+                double v = P.w[a] > 0.0 ? -cs / P.w[a] : 0.0;
+                double r = S.gamma;
+                if (P.sw) r = S.gamma * (double)sw[D.sw_off + a * D.sw_cstride + o];
+                v = r > 0.0 ? fmin(fmax(v, -r), r) : 0.0;
+                if (i + 1 == L || !isfinite(v)) v = 0.0;
+                U[a * N + o] = (T)v;
+                U[(k + a) * N + o] = (T)v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact 1D TV (the single-active-axis case): Condat's direct algorithm per
+// fibre, one thread per fibre; the dual is the running sum of x - z
+// (tv_iso.cpp:381-398).
+// ---------------------------------------------------------------------------
+
+template <class T>
+__device__ void condat(const T *y, int64_t ys, T *x, int64_t xs, int64_t n, double lam, int e) {
+    auto Y = [&](int64_t i) { return ldexp((double)y[i * ys], -e); };
+    auto X = [&](int64_t i, double v) { x[i * xs] = (T)ldexp(v, e); };
+    if (n == 1 || !(lam > 0.0)) {
+        for (int64_t i = 0; i < n; ++i) X(i, Y(i));
+        return;
+    }
+    int64_t k = 0, k0 = 0, kmin = 0, kplus = 0;
+    double vmin = Y(0) - lam, vmax = Y(0) + lam, umin = lam, umax = -lam;
+    const double twol = 2.0 * lam;
+    for (;;) {
+        if (k == n - 1) {
+            if (umin < 0.0) {
+                for (int64_t i = k0; i <= kmin; ++i) X(i, vmin);
+                k = k0 = kmin = kmin + 1;
+                vmin = Y(k);
+                umin = lam;
+                umax = Y(k) + lam - vmax;
+                continue;
+            } else if (umax > 0.0) {
+                for (int64_t i = k0; i <= kplus; ++i) X(i, vmax);
+                k = k0 = kplus = kplus + 1;
+                vmax = Y(k);
+                umax = -lam;
+                umin = Y(k) - lam - vmin;
+                continue;
+            } else {
+                const double v = vmin + umin / (double)(k - k0 + 1);
+                for (int64_t i = k0; i <= k; ++i) X(i, v);
+                return;
+            }
+        }
+        const double yn = Y(k + 1);
+        if (yn + umin < vmin - lam) {
+            for (int64_t i = k0; i <= kmin; ++i) X(i, vmin);
+            k = k0 = kmin = kplus = kmin + 1;
+            vmin = Y(k);
+            vmax = Y(k) + twol;
+            umin = lam;
+            umax = -lam;
+        } else if (yn + umax > vmax + lam) {
+            for (int64_t i = k0; i <= kplus; ++i) X(i, vmax);
+            k = k0 = kmin = kplus = kplus + 1;
+            vmin = Y(k) - twol;
+            vmax = Y(k);
+            umin = lam;
+            umax = -lam;
+        } else {
+            k = k + 1;
+            umin = umin + Y(k) - vmin;
+            umax = umax + Y(k) - vmax;
+            if (umin >= lam) {
+                vmin = vmin + (umin - lam) / (double)(k - k0 + 1);
+                umin = lam;
+                kmin = k;
+            }
+            if (umax <= -lam) {
+                vmax = vmax + (umax + lam) / (double)(k - k0 + 1);
+                umax = -lam;
+                kplus = k;
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ void tv1d_kernel(Problem P, const T *d, T *z, T *q, const ImgDesc *desc, ImgState *state, int nimg) {
+    const int img = blockIdx.y;
+    if (img >= nimg) return;
+    ImgState &S = state[img];
+    if (S.mode != kTV1D) return;
+    const ImgDesc &D = desc[img];
+    const int k = P.k;
+    const int a = S.active_axis, ax = 3 - k + a;
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const int64_t Ls[3] = {D.L[0], D.L[1], D.L[2]};
+    const int64_t St[3] = {(int64_t)D.L[1] * D.L[2], D.L[2], 1};
+    const int64_t L = Ls[ax], s = St[ax], nf = N / L;
+    const int e = S.e;
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t outer = f / s, inner = f % s;
+        const int64_t st = outer * L * s + inner;
+        const T *y = d + D.d_off + st;
+        T *x = z + D.z_off + st;
+        condat(y, s, x, s, L, S.tv1d_lam, e);
+        double cs = 0.0;
+        for (int64_t i = 0; i < L; ++i) {
+            const int64_t o = st + i * s;
+            cs += ldexp((double)d[D.d_off + o], -e) - ldexp((double)z[D.z_off + o], -e);
+            for (int b = 0; b < k; ++b)
+                q[D.q_off + b * D.q_cstride + o] = b == a ? (T)(i + 1 < L ? ldexp(-cs / P.w[a], e) : 0.0) : (T)0;
+        }
+    }
+}
+
+__global__ void tv1d_finish_kernel(const ImgDesc *desc, ImgState *state, int nimg, double u_round, double eta) {
+    const int img = blockIdx.x * blockDim.x + threadIdx.x;
+    if (img >= nimg) return;
+    ImgState &S = state[img];
+    if (S.mode != kTV1D) return;
+    const ImgDesc &D = desc[img];
+    S.gp = 0.0;
+    S.it = 0;
+    S.converged = 1;
+    finish(S, (int64_t)D.L[0] * D.L[1] * D.L[2], u_round, eta);
+}
+
+// ---------------------------------------------------------------------------
+// The 2D step kernel: a wavefront per tile, TT iterations in registers.
+// ---------------------------------------------------------------------------
+
+template <class T, bool ISO, bool SW, int TT, int RY>
+__device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, const int lane) {
+    constexpr int TXI = 64 - 2 * TT;
+    constexpr int TYI = RY - 2 * TT;
+    static_assert(RY <= 32 && TYI > 0, "tile rows");
+    const int img = __builtin_amdgcn_readfirstlane(tl.img);
+    const int tx = __builtin_amdgcn_readfirstlane(tl.tx);
+    const int ty = __builtin_amdgcn_readfirstlane(tl.ty);
+    const int tloc = __builtin_amdgcn_readfirstlane(tl.pad_);
+    const ImgDesc &D = A.desc[img];
+    ImgState &S = A.state[img];
+    const int mode = S.mode;
+    const int H = D.L[1], W = D.L[2];
+    const int64_t N = (int64_t)H * W;
+    const int x = tx * TXI - TT + lane;
+    const int y0 = ty * TYI - TT;
+    const bool xin = x >= 0 && x < W;
+    const bool lane_in = lane >= TT && lane < 64 - TT;
+    const double w0d = S.wa[0], w1d = S.wa[1];
+    const T w0 = (T)w0d, w1 = (T)w1d;
+    uint32_t rowin = 0, hn0 = 0;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = y0 + r;
+        if (y >= 0 && y < H) rowin |= 1u << r;
+        if (y >= 0 && y + 1 < H) hn0 |= 1u << r;
+    }
+    if (w0d == 0.0 || !xin) hn0 = 0;
+    const bool hn1 = xin && x + 1 < W && w1d != 0.0;
+    constexpr uint32_t kInt = ((RY - TT >= 32) ? 0xffffffffu : ((1u << (RY - TT)) - 1u)) & ~((1u << TT) - 1u);
+    const uint32_t intr = lane_in && xin ? (rowin & kInt) : 0u;  // interior rows of this lane
+    double *part = A.part + (int64_t)(D.tile_base + tloc) * kSlots;
+
+    if (mode == kOutputNaN) {
+        const T nanv = (T)NAN;
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+            if ((intr >> r) & 1u) {
+                const int64_t o = (int64_t)(y0 + r) * W + x;
+                A.z[D.z_off + o] = nanv;
+                A.q[D.q_off + o] = nanv;
+                A.q[D.q_off + D.q_cstride + o] = nanv;
+            }
+        return;
+    }
+
+    const int e = S.e;
+    const int cnt = S.cnt;
+    const uint32_t tm = S.test_mask, gm = S.gap_mask;
+    const bool fresh = S.fresh;
+    const T tau = (T)S.tau;
+    const double gamd = S.gamma, boundd = S.bound;
+    const T gam = (T)gamd;
+    const T *dp = A.d + D.d_off;
+    const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
+    const T *V = S.vsrc_u ? U : U + 2 * N;
+
+    T d[RY], u0[RY], u1[RY], v0[RY], v1[RY];
+    T ra[SW ? RY : 1], rb[(SW && !ISO) ? RY : 1];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const bool in = xin && ((rowin >> r) & 1u);
+        const int64_t o = in ? (int64_t)(y0 + r) * W + x : 0;
+        d[r] = in ? scale_in(dp[o], e) : (T)0;
+        if (fresh) {
+            u0[r] = u1[r] = v0[r] = v1[r] = (T)0;
+        } else {
+            u0[r] = in ? U[o] : (T)0;
+            u1[r] = in ? U[N + o] : (T)0;
+            v0[r] = in ? V[o] : (T)0;
+            v1[r] = in ? V[N + o] : (T)0;
+        }
+        if constexpr (SW) {
+            const T *sw = A.sw + D.sw_off;
+            if constexpr (ISO) {
+                ra[r] = in ? (T)iso_radius(gamd, (double)sw[o], boundd) : (T)0;
+            } else {
+                ra[r] = in ? (T)(gamd * (double)sw[o]) : (T)0;
+                rb[r] = in ? (T)(gamd * (double)sw[D.sw_cstride + o]) : (T)0;
+            }
+        }
+    }
+
+    // The gap of the state (u) on the interior, FP64 terms (tv_iso.cpp:290-306).
+    auto gap_level = [&](int j) {
+        double acc = 0.0;
+        auto zu = [&](int r) -> double {
+            const double up = r > 0 ? (double)u0[r - 1] : 0.0;
+            const double ul = lane_prev((double)u1[r]);
+            return ((double)d[r] + w0d * ((double)u0[r] - up)) + w1d * ((double)u1[r] - ul);
+        };
+        double zc = zu(TT - 1);
+#pragma unroll
+        for (int r = TT - 1; r < RY - TT; ++r) {
+            const double zn = zu(r + 1);
+            const double zr = lane_next(zc);
+            if ((intr >> r) & 1u) {
+                const double g0 = ((hn0 >> r) & 1u) ? w0d * (zn - zc) : 0.0;
+                const double g1 = hn1 ? w1d * (zr - zc) : 0.0;
+                if constexpr (ISO) {
+                    const double nn = g0 * g0 + g1 * g1;
+                    const double dot = g0 * (double)u0[r] + g1 * (double)u1[r];
+                    const double rp = SW ? (double)ra[r] : gamd;
+                    acc += rp * sqrt(nn) - dot;
+                } else {
+                    const double r0 = SW ? (double)ra[r] : gamd, r1 = SW ? (double)rb[r] : gamd;
+                    acc += r0 * fabs(g0) - g0 * (double)u0[r];
+                    acc += r1 * fabs(g1) - g1 * (double)u1[r];
+                }
+            }
+            zc = zn;
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) part[2 * j] = acc;
+    };
+
+    if (gm & 1u) gap_level(0);
+
+#pragma unroll
+    for (int j = 1; j <= TT; ++j) {
+        if (j > cnt) break;
+        const T beta = (T)S.beta[j];
+        const bool test = (tm >> j) & 1u;
+        double dot = 0.0;
+        // z(v) on row r + 1 before row r is updated (z[r + 1] reads the old v0[r]).
+        auto zv = [&](int r) -> T {
+            const T vu = r > 0 ? v0[r - 1] : (T)0;
+            const T vl = lane_prev(v1[r]);
+            return (d[r] + w0 * (v0[r] - vu)) + w1 * (v1[r] - vl);
+        };
+        T zc = zv(0);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const T zn = r + 1 < RY ? zv(r + 1) : (T)0;
+            const T zr = lane_next(zc);
+            const T g0 = ((hn0 >> r) & 1u) ? w0 * (zn - zc) : (T)0;
+            const T g1 = (hn1 && ((rowin >> r) & 1u)) ? w1 * (zr - zc) : (T)0;
+            const T c0 = w0d != 0.0 ? v0[r] + tau * g0 : (T)0;
+            const T c1 = w1d != 0.0 ? v1[r] + tau * g1 : (T)0;
+            T n0, n1;
+            if constexpr (ISO) {
+                const T nn = c0 * c0 + c1 * c1;
+                const T nr = sqrt(nn);
+                const T rp = SW ? ra[r] : gam;
+                const T sc = nr > rp ? rp / nr : (T)1;
+                n0 = c0 * sc;
+                n1 = c1 * sc;
+            } else {
+                const T r0 = SW ? ra[r] : gam, r1 = SW ? rb[r] : gam;
+                n0 = c0 > r0 ? r0 : (c0 < -r0 ? -r0 : c0);
+                n1 = c1 > r1 ? r1 : (c1 < -r1 ? -r1 : c1);
+            }
+            if (test && ((intr >> r) & 1u)) {
+                dot += ((double)v0[r] - (double)n0) * ((double)n0 - (double)u0[r]);
+                dot += ((double)v1[r] - (double)n1) * ((double)n1 - (double)u1[r]);
+            }
+            v0[r] = n0 + beta * (n0 - u0[r]);
+            v1[r] = n1 + beta * (n1 - u1[r]);
+            u0[r] = n0;
+            u1[r] = n1;
+            zc = zn;
+        }
+        if (test) {
+            dot = wave_sum(dot);
+            if (lane == 0) part[2 * j + 1] = dot;
+        }
+        if (j < TT && ((gm >> j) & 1u)) gap_level(j);
+    }
+
+    if (mode == kOutput) {
+        // z(u) and the dual on the interior (tv_iso.cpp:539, 403-413; _tv_torch.py:588-593).
+        const double wmax = S.wmax;
+#pragma unroll
+        for (int r = TT; r < RY - TT; ++r) {
+            const double up = (double)u0[r - 1];
+            const double ul = lane_prev((double)u1[r]);
+            const double zz = ((double)d[r] + w0d * ((double)u0[r] - up)) + w1d * ((double)u1[r] - ul);
+            if ((intr >> r) & 1u) {
+                const int64_t o = (int64_t)(y0 + r) * W + x;
+                A.z[D.z_off + o] = (T)ldexp(zz, e);
+                if constexpr (ISO) {
+                    A.q[D.q_off + o] = (T)ldexp((double)u0[r] / wmax, e);
+                    A.q[D.q_off + D.q_cstride + o] = (T)ldexp((double)u1[r] / wmax, e);
+                } else {
+                    A.q[D.q_off + o] = (T)(w0d * (up - (double)u0[r]));
+                    A.q[D.q_off + D.q_cstride + o] = (T)(w1d * (ul - (double)u1[r]));
+                }
+            }
+        }
+        return;
+    }
+    if (cnt == 0) return;
+    T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N;
+#pragma unroll
+    for (int r = TT; r < RY - TT; ++r)
+        if ((intr >> r) & 1u) {
+            const int64_t o = (int64_t)(y0 + r) * W + x;
+            Uo[o] = u0[r];
+            Uo[N + o] = u1[r];
+            Uo[2 * N + o] = v0[r];
+            Uo[3 * N + o] = v1[r];
+        }
+}
+
+// Tiles of one launch in chunks of 4 (one per wavefront); each XCD (workgroup
+// ids congruent mod 8 share one) takes a contiguous range of the list, so an
+// image's neighbouring tiles share an L2.
+template <class T, bool ISO, bool SW, int TT, int RY>
+__global__ __launch_bounds__(256) void step2d_kernel(StepArgs<T> A) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;
+    const int ntiles = *A.count_in;
+    const int nchunks = (ntiles + 3) >> 2;
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, j8 = blockIdx.x >> 3;
+    const int cpx = (nchunks + 7) >> 3;
+    const int cend = min((xcd + 1) * cpx, nchunks);
+    for (int c = xcd * cpx + j8; c < cend; c += per) {
+        const int ti = c * 4 + wv;
+        if (ti < ntiles) tile2d<T, ISO, SW, TT, RY>(A, A.list_in[ti], lane);
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// The generic step kernel: one iteration per launch, one thread per pixel,
+// neighbours read through the cache; 2D and 3D. Tiles of 64 x 4 pixels of
+// one slice (a workgroup each).
+// ---------------------------------------------------------------------------
+
+template <class T>
+struct Pix {
+    int64_t L[3], S[3];
+    int k;
+    // z = d_s + sum_a w_a (f_a[p] - f_a[p - e_a]) at p (in the image), f = u or v planes.
+    __device__ __forceinline__ T zval(const T *dp, int e, const T *F, int64_t N, const T *w, int64_t p,
+                                      const int64_t *c) const {
+        T z = scale_in(dp[p], e);
+        for (int a = 0; a < k; ++a) {
+            const int ax = 3 - k + a;
+            const T fp = c[ax] > 0 ? F[a * N + p - S[ax]] : (T)0;
+            z = z + w[a] * (F[a * N + p] - fp);
+        }
+        return z;
+    }
+};
+
+template <class T, bool ISO, bool SW>
+__global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;
+    const int ntiles = *A.count_in;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    __shared__ double red[4][2 * 2];
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const Tile tl = A.list_in[ti];
+        const int img = tl.img;
+        const ImgDesc &D = A.desc[img];
+        ImgState &S = A.state[img];
+        const int mode = S.mode;
+        const int k = A.k;
+        Pix<T> G;
+        G.k = k;
+        for (int a = 0; a < 3; ++a) G.L[a] = D.L[a];
+        G.S[2] = 1;
+        G.S[1] = D.L[2];
+        G.S[0] = (int64_t)D.L[1] * D.L[2];
+        const int64_t N = G.S[0] * D.L[0];
+        const int64_t c[3] = {tl.tz, (int64_t)tl.ty * 4 + wv, (int64_t)tl.tx * 64 + lane};
+        const bool in = c[0] < D.L[0] && c[1] < D.L[1] && c[2] < D.L[2];
+        const int64_t p = in ? c[0] * G.S[0] + c[1] * G.S[1] + c[2] : 0;
+        double *part = A.part + (int64_t)(D.tile_base + tl.pad_) * kSlots;
+        if (mode == kOutputNaN) {
+            if (in) {
+                A.z[D.z_off + p] = (T)NAN;
+                for (int a = 0; a < k; ++a) A.q[D.q_off + a * D.q_cstride + p] = (T)NAN;
+            }
+            continue;
+        }
+        const int e = S.e, cnt = S.cnt;
+        const uint32_t tm = S.test_mask, gm = S.gap_mask;
+        T w[3];
+        double wd[3];
+        bool has[3];
+        for (int a = 0; a < 3; ++a) {
+            wd[a] = a < k ? S.wa[a] : 0.0;
+            w[a] = (T)wd[a];
+        }
+        for (int a = 0; a < k; ++a) has[a] = in && c[3 - k + a] + 1 < D.L[3 - k + a] && wd[a] != 0.0;
+        const T *dp = A.d + D.d_off;
+        const bool fresh = S.fresh;
+        const T *U = A.arena + D.st_off + (int64_t)S.cur * 2 * k * N;
+        const T *V = S.vsrc_u ? U : U + k * N;
+        const T *sw = SW ? A.sw + D.sw_off : nullptr;
+        const double gamd = S.gamma;
+        double rad[3] = {gamd, gamd, gamd};
+        if (SW && in) {
+            if (ISO) rad[0] = iso_radius(gamd, (double)sw[p], S.bound);
+            else
+                for (int a = 0; a < k; ++a) rad[a] = gamd * (double)sw[a * D.sw_cstride + p];
+        }
+        // The gap of the u planes at p, FP64 (tv_iso.cpp:290-306).
+        auto gap_at = [&]() -> double {
+            if (!in || fresh) {
+                if (!in) return 0.0;
+            }
+            double zc = 0.0, g[3] = {0.0, 0.0, 0.0};
+            auto zu = [&](int64_t q, const int64_t *cq) -> double {
+                double z = ldexp((double)dp[q], -e);
+                for (int a = 0; a < k; ++a) {
+                    const int ax = 3 - k + a;
+                    const double fp = (fresh || cq[ax] == 0) ? 0.0 : (double)U[a * N + q - G.S[ax]];
+                    const double fc = fresh ? 0.0 : (double)U[a * N + q];
+                    z = z + wd[a] * (fc - fp);
+                }
+                return z;
+            };
+            zc = zu(p, c);
+            for (int a = 0; a < k; ++a) {
+                if (!has[a]) continue;
+                const int ax = 3 - k + a;
+                int64_t cn[3] = {c[0], c[1], c[2]};
+                cn[ax] += 1;
+                g[a] = wd[a] * (zu(p + G.S[ax], cn) - zc);
+            }
+            double acc = 0.0;
+            if (ISO) {
+                double nn = 0.0, dot = 0.0;
+                for (int a = 0; a < k; ++a) {
+                    nn += g[a] * g[a];
+                    dot += g[a] * (fresh ? 0.0 : (double)U[a * N + p]);
+                }
+                acc = rad[0] * sqrt(nn) - dot;
+            } else {
+                for (int a = 0; a < k; ++a) acc += rad[a] * fabs(g[a]) - g[a] * (fresh ? 0.0 : (double)U[a * N + p]);
+            }
+            return acc;
+        };
+        double gsum = 0.0, dsum = 0.0;
+        const bool g0 = gm & 1u;
+        if (g0) gsum = gap_at();
+        T un[3] = {0, 0, 0}, vn[3] = {0, 0, 0};
+        const bool step = cnt >= 1 && mode == kIterate;
+        if (step && in) {
+            // z(v) at p and its forward neighbours, then the update (tv_iso.cpp:499-524).
+            T vv[3], uu[3];
+            for (int a = 0; a < k; ++a) {
+                vv[a] = fresh ? (T)0 : V[a * N + p];
+                uu[a] = fresh ? (T)0 : U[a * N + p];
+            }
+            auto zv = [&](int64_t q, const int64_t *cq) -> T {
+                if (fresh) return scale_in(dp[q], e);
+                return G.zval(dp, e, V, N, w, q, cq);
+            };
+            const T zc = zv(p, c);
+            T g[3] = {0, 0, 0};
+            for (int a = 0; a < k; ++a) {
+                if (!has[a]) continue;
+                const int ax = 3 - k + a;
+                int64_t cn[3] = {c[0], c[1], c[2]};
+                cn[ax] += 1;
+                g[a] = w[a] * (zv(p + G.S[ax], cn) - zc);
+            }
+            const T tau = (T)S.tau, beta = (T)S.beta[1];
+            T cv[3];
+            T nn = 0;
+            for (int a = 0; a < k; ++a) {
+                cv[a] = wd[a] != 0.0 ? vv[a] + tau * g[a] : (T)0;
+                nn += cv[a] * cv[a];
+            }
+            if (ISO) {
+                const T nr = sqrt(nn), rp = (T)rad[0];
+                const T sc = nr > rp ? rp / nr : (T)1;
+                for (int a = 0; a < k; ++a) un[a] = cv[a] * sc;
+            } else {
+                for (int a = 0; a < k; ++a) {
+                    const T r = (T)rad[a];
+                    un[a] = cv[a] > r ? r : (cv[a] < -r ? -r : cv[a]);
+                }
+            }
+            for (int a = 0; a < k; ++a) {
+                if ((tm >> 1) & 1u) dsum += ((double)vv[a] - (double)un[a]) * ((double)un[a] - (double)uu[a]);
+                vn[a] = un[a] + beta * (un[a] - uu[a]);
+            }
+        }
+        // Reductions: a fixed tree over the workgroup.
+        const bool dt = step && ((tm >> 1) & 1u);
+        if (g0 || dt) {
+            const double a0 = wave_sum(gsum), a1 = wave_sum(dsum);
+            if (lane == 0) {
+                red[wv][0] = a0;
+                red[wv][1] = a1;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (g0) part[0] = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+                if (dt) part[3] = (red[0][1] + red[1][1]) + (red[2][1] + red[3][1]);
+            }
+            __syncthreads();
+        }
+        if (!in) continue;
+        if (mode == kOutput) {
+            // cnt is 0 here: z(u) and the dual at p.
+            double zz = ldexp((double)dp[p], -e);
+            for (int a = 0; a < k; ++a) {
+                const int ax = 3 - k + a;
+                const double fp = (fresh || c[ax] == 0) ? 0.0 : (double)U[a * N + p - G.S[ax]];
+                const double fc = fresh ? 0.0 : (double)U[a * N + p];
+                zz = zz + wd[a] * (fc - fp);
+                double qv;
+                if (ISO) qv = ldexp(fc / S.wmax, e);
+                else qv = wd[a] * (fp - fc);
+                A.q[D.q_off + a * D.q_cstride + p] = (T)qv;
+            }
+            A.z[D.z_off + p] = (T)ldexp(zz, e);
+            continue;
+        }
+        if (!step) continue;
+        T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 2 * k * N;
+        for (int a = 0; a < k; ++a) {
+            Uo[a * N + p] = un[a];
+            Uo[(k + a) * N + p] = vn[a];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bookkeeping kernels
+// ---------------------------------------------------------------------------
+
+__global__ void reset_counts_kernel(int *counts) {
+    if (threadIdx.x < 2) counts[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(64) void initial_list_kernel(const ImgDesc *desc, const ImgState *state, Tile *list,
+                                                          int *count, int nimg) {
+    const int img = blockIdx.x;
+    if (img >= nimg) return;
+    const ImgState &S = state[img];
+    const int m = S.mode;
+    if (!(m == kIterate || m == kOutput || m == kOutputNaN)) return;
+    const ImgDesc &D = desc[img];
+    __shared__ int base;
+    if (threadIdx.x == 0) base = atomicAdd(count, D.ntiles);
+    __syncthreads();
+    const int nxy = D.ntx * D.nty;
+    for (int i = threadIdx.x; i < D.ntiles; i += 64) {
+        Tile t;
+        t.img = img;
+        t.tz = i / nxy;
+        const int r = i - t.tz * nxy;
+        t.ty = (int16_t)(r / D.ntx);
+        t.tx = (int16_t)(r - t.ty * D.ntx);
+        t.pad_ = i;
+        list[base + i] = t;
+    }
+}
+
+template <class T>
+__global__ void strided_copy_kernel(const T *src, T *dst, int64_t n, int nd, StridedShape sh, int src_strided) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = i, off = 0;
+        for (int d = nd - 1; d >= 0; --d) {
+            const int64_t c = r % sh.shape[d];
+            r /= sh.shape[d];
+            off += c * sh.stride[d];
+        }
+        if (src_strided) dst[i] = src[off];
+        else dst[off] = src[i];
+    }
+}
+
+template <class S, class D>
+__global__ void convert_kernel(const S *src, D *dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (D)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static int grid_for(int64_t n, int block, int cap = 65535) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)(g > cap ? cap : g);
+}
+
+template <class T>
+void launch_prologue(const Problem &P, const T *d, const ImgDesc *desc, ImgState *state, const double *gammas,
+                     int nimg, hipStream_t s) {
+    hipLaunchKernelGGL(prologue_kernel<T>, dim3(nimg), dim3(256), 0, s, P, d, desc, state, gammas, P.TT);
+}
+
+void launch_reset_counts(int *counts, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counts_kernel, dim3(1), dim3(64), 0, s, counts);
+}
+
+void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list, int *count, int nimg,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(initial_list_kernel, dim3(nimg), dim3(64), 0, s, desc, state, list, count, nimg);
+}
+
+template <class T>
+void launch_warm(const Problem &P, const T *q, const T *sw, T *arena, const ImgDesc *desc, const ImgState *state,
+                 int nimg, int64_t maxN, hipStream_t s) {
+    if (P.iso)
+        hipLaunchKernelGGL(warm_iso_kernel<T>, dim3(grid_for(maxN, 256, 4096), nimg), dim3(256), 0, s, P, q, sw,
+                           arena, desc, state, nimg);
+    else
+        hipLaunchKernelGGL(warm_aniso_kernel<T>, dim3(grid_for(maxN, 256, 1024), nimg), dim3(256), 0, s, P, q, sw,
+                           arena, desc, state, nimg);
+}
+
+template <class T>
+void launch_tv1d(const Problem &P, const T *d, T *z, T *q, const ImgDesc *desc, ImgState *state, int nimg,
+                 int64_t maxN, double u_round, double eta, hipStream_t s) {
+    hipLaunchKernelGGL(tv1d_kernel<T>, dim3(grid_for(maxN, 64, 4096), nimg), dim3(64), 0, s, P, d, z, q, desc,
+                       state, nimg);
+    hipLaunchKernelGGL(tv1d_finish_kernel, dim3(grid_for(nimg, 64)), dim3(64), 0, s, desc, state, nimg, u_round,
+                       eta);
+}
+
+void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out, int *count_out,
+                     int TT, double u_round, double eta, int nimg, hipStream_t s) {
+    hipLaunchKernelGGL(epilogue_kernel, dim3(nimg), dim3(64), 0, s, desc, state, part, list_out, count_out, TT,
+                       u_round, eta, nimg);
+}
+
+template <class T, bool ISO, bool SW>
+static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid, hipStream_t s) {
+    if (kc.generic) {
+        hipLaunchKernelGGL((step1_kernel<T, ISO, SW>), dim3(grid), dim3(256), 0, s, A);
+        return;
+    }
+#define FPR_CASE(TTv, RYv)                                                                              \
+    if (kc.TT == TTv && kc.RY == RYv) {                                                                 \
+        hipLaunchKernelGGL((step2d_kernel<T, ISO, SW, TTv, RYv>), dim3(grid), dim3(256), 0, s, A);      \
+        return;                                                                                         \
+    }
+    if constexpr (sizeof(T) == 4) {
+        FPR_CASE(1, 32)
+        FPR_CASE(2, 32)
+        FPR_CASE(4, 32)
+        FPR_CASE(4, 24)
+        FPR_CASE(8, 32)
+    } else {
+        FPR_CASE(1, 16)
+        FPR_CASE(2, 16)
+        FPR_CASE(4, 16)
+        FPR_CASE(2, 24)
+    }
+#undef FPR_CASE
+}
+
+template <class T>
+void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s) {
+    if (iso) {
+        if (sw) step_dispatch<T, true, true>(kc, A, grid, s);
+        else step_dispatch<T, true, false>(kc, A, grid, s);
+    } else {
+        if (sw) step_dispatch<T, false, true>(kc, A, grid, s);
+        else step_dispatch<T, false, false>(kc, A, grid, s);
+    }
+}
+
+bool kernel_available(int dtype_bytes, int TT, int RY) {
+    if (dtype_bytes == 4)
+        return (TT == 1 && RY == 32) || (TT == 2 && RY == 32) || (TT == 4 && RY == 32) || (TT == 4 && RY == 24) ||
+               (TT == 8 && RY == 32);
+    return (TT == 1 && RY == 16) || (TT == 2 && RY == 16) || (TT == 4 && RY == 16) || (TT == 2 && RY == 24);
+}
+
+template <class T>
+void launch_gather(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s) {
+    hipLaunchKernelGGL(strided_copy_kernel<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n, nd, sh, 1);
+}
+template <class T>
+void launch_scatter(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s) {
+    hipLaunchKernelGGL(strided_copy_kernel<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n, nd, sh, 0);
+}
+template <class S, class D>
+void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((convert_kernel<S, D>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n);
+}
+
+#define FPR_INST(T)                                                                                              \
+    template void launch_prologue<T>(const Problem &, const T *, const ImgDesc *, ImgState *, const double *, int, \
+                                     hipStream_t);                                                              \
+    template void launch_warm<T>(const Problem &, const T *, const T *, T *, const ImgDesc *, const ImgState *, int, \
+                                 int64_t, hipStream_t);                                                         \
+    template void launch_tv1d<T>(const Problem &, const T *, T *, T *, const ImgDesc *, ImgState *, int, int64_t,  \
+                                 double, double, hipStream_t);                                                  \
+    template void launch_step<T>(const KernelChoice &, bool, bool, const StepArgs<T> &, int, hipStream_t);       \
+    template void launch_gather<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);            \
+    template void launch_scatter<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);
+FPR_INST(float)
+FPR_INST(double)
+#undef FPR_INST
+template void launch_convert<float, double>(const float *, double *, int64_t, hipStream_t);
+template void launch_convert<double, float>(const double *, float *, int64_t, hipStream_t);
+template void launch_convert<float, float>(const float *, float *, int64_t, hipStream_t);
+template void launch_convert<double, double>(const double *, double *, int64_t, hipStream_t);
+
+}  // namespace fpr

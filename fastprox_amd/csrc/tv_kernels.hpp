@@ -1,0 +1,56 @@
+// Host launchers of the image-TV kernels (tv_kernels.hip).
+//
+// SPDX-License-Identifier: MIT
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tv_common.hpp"
+
+namespace fpr {
+
+constexpr int kMaxStrided = 32;
+
+struct StridedShape {
+    int64_t shape[kMaxStrided];
+    int64_t stride[kMaxStrided];
+};
+
+// The step kernel of a solve: the register-tile 2D kernel with TT iterations
+// per launch on tiles of (64 - 2 TT) x (RY - 2 TT) pixels, or the generic
+// one-iteration kernel (2D and 3D).
+struct KernelChoice {
+    bool generic = false;
+    int TT = 4;
+    int RY = 32;
+    int tile_x() const { return generic ? 64 : 64 - 2 * TT; }
+    int tile_y() const { return generic ? 4 : RY - 2 * TT; }
+    int iters() const { return generic ? 1 : TT; }
+};
+
+bool kernel_available(int dtype_bytes, int TT, int RY);
+
+template <class T>
+void launch_prologue(const Problem &P, const T *d, const ImgDesc *desc, ImgState *state, const double *gammas,
+                     int nimg, hipStream_t s);
+void launch_reset_counts(int *counts, hipStream_t s);
+void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list, int *count, int nimg,
+                         hipStream_t s);
+template <class T>
+void launch_warm(const Problem &P, const T *q, const T *sw, T *arena, const ImgDesc *desc, const ImgState *state,
+                 int nimg, int64_t maxN, hipStream_t s);
+template <class T>
+void launch_tv1d(const Problem &P, const T *d, T *z, T *q, const ImgDesc *desc, ImgState *state, int nimg,
+                 int64_t maxN, double u_round, double eta, hipStream_t s);
+template <class T>
+void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s);
+void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out, int *count_out,
+                     int TT, double u_round, double eta, int nimg, hipStream_t s);
+template <class T>
+void launch_gather(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s);
+template <class T>
+void launch_scatter(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s);
+template <class S, class D>
+void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s);
+
+}  // namespace fpr

@@ -1,0 +1,189 @@
+/*
+ * fastprox_rocm.h -- fastprox's plain-C ABI for image total variation, on
+ * AMD Instinct GPUs (gfx950) through HIP.
+ *
+ * This header declares the subset of fastprox's public C interface that the
+ * image-TV path binds (the reference's include/fastprox.h, cited per entry
+ * below), with the same names, types, status codes and semantics, so that a
+ * caller of the reference's TV operators links against libfastprox_rocm.so
+ * unchanged. Arrays may live on the host (FP_DEVICE_CPU: staged through
+ * device memory) or on the GPU (FP_DEVICE_ROCM, DLPack's kDLROCM, the value
+ * the reference reserves at include/fastprox.h:53). Every solve runs on the
+ * HIP device current when the workspace was created, on the workspace's
+ * stream (fp_workspace_set_stream; NULL: the null stream).
+ *
+ * Extensions with no counterpart in the reference are prefixed fpr_.
+ *
+ * SPDX-License-Identifier: MIT
+ */
+#ifndef FASTPROX_ROCM_H
+#define FASTPROX_ROCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#define FP_VERSION_MAJOR 1
+#define FP_VERSION_MINOR 0
+#define FP_VERSION_PATCH 6
+#define FP_ABI_VERSION 1
+
+#define FP_MAX_NDIM 32
+
+#define FP_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/fastprox.h:52 */
+typedef enum { FP_F32 = 1, FP_F64 = 2, FP_I8 = 3, FP_I32 = 4, FP_I64 = 5 } fp_dtype;
+/* include/fastprox.h:54, with the reserved kDLROCM value enabled. */
+typedef enum { FP_DEVICE_CPU = 0, FP_DEVICE_CUDA = 2, FP_DEVICE_ROCM = 10 } fp_device;
+
+/* include/fastprox.h:61-68 */
+typedef struct {
+    void          *data;
+    fp_dtype       dtype;
+    int            ndim;
+    const int64_t *shape;
+    const int64_t *strides; /* in elements; NULL: C-contiguous */
+    fp_device      device;
+} fp_array;
+
+/* include/fastprox.h:70-80 */
+typedef enum {
+    FP_OK = 0,
+    FP_EINVAL,
+    FP_EDTYPE,
+    FP_ESHAPE,
+    FP_ENOMEM,
+    FP_ENOTCONV,
+    FP_EPARAM,
+    FP_EINTERNAL,
+    FP_EDEVICE
+} fp_status;
+
+typedef struct fp_op fp_op;
+typedef struct fp_workspace fp_workspace;
+
+/* include/fastprox.h:93-103 (axis, tie: accepted and ignored by image operators;
+ * threads: ignored, the GPU runs the batch) */
+typedef struct {
+    int    axis;
+    int    threads;
+    double tol;
+    int    max_iter;
+    int    tie;
+    int    keep_structure;
+} fp_options;
+
+/* include/fastprox.h:105-109 */
+typedef struct {
+    double  gap;
+    int     iters;
+    int64_t ties;
+} fp_info;
+
+/* include/fastprox.h:113 -- defaults: axis -1, threads 0, tol 1e-6, max_iter 1000. */
+FP_API void fp_options_init(fp_options *opt);
+
+/* include/fastprox.h:322 -- anisotropic TV of the last ndim (2 or 3) axes. On
+ * the GPU it is the device method of fastprox.torch.tv_aniso
+ * (bindings/python/fastprox/_tv_torch.py:229-283): dual FISTA on the per-edge
+ * duals with restarts and the gap check every 4 iterations. */
+FP_API fp_status fp_op_tv_aniso(int ndim, const fp_array *axis_weights, fp_op **out);
+/* include/fastprox.h:332 -- with edge weights of shape (ndim, image shape). */
+FP_API fp_status fp_op_tv_aniso_weighted(int ndim, const fp_array *axis_weights, const fp_array *edge_weights,
+                                         fp_op **out);
+/* include/fastprox.h:343 -- isotropic TV, solved by the reference's dual FISTA
+ * (src/ops/tv_iso.cpp:424-543) with its schedule. */
+FP_API fp_status fp_op_tv_iso(int ndim, const fp_array *axis_weights, fp_op **out);
+/* include/fastprox.h:352 -- with pixel weights of the image shape. */
+FP_API fp_status fp_op_tv_iso_weighted(int ndim, const fp_array *axis_weights, const fp_array *pixel_weights,
+                                       fp_op **out);
+
+/* include/fastprox.h:392-394 */
+FP_API void        fp_op_free(fp_op *op);
+FP_API int64_t     fp_op_nparams(const fp_op *op);
+FP_API const char *fp_op_name(const fp_op *op);
+
+/* include/fastprox.h:411-413 -- all device memory of a solve is allocated
+ * here: fp_prox and fp_prox_steps on a workspace allocate nothing. */
+FP_API fp_status fp_workspace_create(const fp_op *op, const fp_array *like, const fp_options *opt,
+                                     fp_workspace **out);
+FP_API void      fp_workspace_free(fp_workspace *ws);
+
+/* include/fastprox.h:426-428 -- the dual of the last call, shape (ndim, *x.shape). */
+FP_API fp_status fp_workspace_get_dual(const fp_workspace *ws, fp_array *out);
+FP_API fp_status fp_workspace_set_dual(fp_workspace *ws, const fp_array *dual);
+FP_API fp_status fp_workspace_set_warm(fp_workspace *ws, int on);
+
+/* include/fastprox.h:437-438 */
+FP_API fp_status fp_workspace_set_deriv_tol(fp_workspace *ws, double tol, int max_iter);
+FP_API fp_status fp_workspace_deriv_info(const fp_workspace *ws, double *residual, int *iters);
+
+/* include/fastprox.h:599-601 */
+FP_API fp_status fp_prox(const fp_op *op, double gamma, const fp_array *x, fp_array *out, const fp_options *opt,
+                         fp_workspace *ws, fp_info *info);
+/* include/fastprox.h:613-614 -- one step per image. */
+FP_API fp_status fp_prox_steps(const fp_op *op, const fp_array *gamma, const fp_array *x, fp_array *out,
+                               const fp_options *opt, fp_workspace *ws);
+
+/* include/fastprox.h:646-654 */
+FP_API fp_status fp_jvp(const fp_op *op, const fp_workspace *ws, const fp_array *dx, double dgamma,
+                        const fp_array *dparams, fp_array *dout);
+FP_API fp_status fp_vjp(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx, double *ggamma,
+                        fp_array *gparams);
+
+/* include/fastprox.h:712-725 */
+FP_API int       fp_device_available(fp_device dev);
+FP_API fp_status fp_op_device_check(const fp_op *op, fp_device dev, int *row_parallel_only);
+FP_API fp_status fp_workspace_set_stream(fp_workspace *ws, void *stream);
+FP_API fp_status fp_workspace_get_stream(const fp_workspace *ws, void **stream);
+
+/* include/fastprox.h:732, 769, 773-774, 779-780 */
+FP_API fp_status fp_workspace_info(const fp_workspace *ws, fp_info *info);
+FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, double *smoothing);
+FP_API fp_status fp_workspace_image_info(const fp_workspace *ws, double *gaps, int *iters, double *zerrs,
+                                         int64_t count);
+FP_API fp_status fp_vjp_device(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx,
+                               fp_array *ggamma, fp_array *gparams);
+
+/* include/fastprox.h:782-789 */
+FP_API const char *fp_strerror(fp_status s);
+FP_API int         fp_abi_version(void);
+FP_API const char *fp_version_string(void);
+FP_API int         fp_default_threads(void);
+
+/* ------------------------------------------------------------------------
+ * The plugin table (src/device/plugin.hpp:16-23 in the reference): the one
+ * symbol a loader resolves, with the ABI number, the build string it must
+ * match, the device count and the operator check.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int abi;
+    const char *build;
+    int (*device_count)(void);
+    fp_status (*op_check)(const fp_op *op, int *row_parallel_only);
+} fpr_plugin_table;
+#define FPR_PLUGIN_ABI 1
+FP_API const fpr_plugin_table *fastprox_rocm_plugin(void);
+
+/* ------------------------------------------------------------------------
+ * Extensions (no counterpart in the reference).
+ * ---------------------------------------------------------------------- */
+
+/* The step kernel of a workspace: generic = 1 selects the one-iteration
+ * kernel; otherwise the register-tile kernel with tt iterations per launch
+ * and ry rows per tile (FP_EINVAL for a combination not built). The default
+ * is the autotuned choice for the workspace's shape and dtype. */
+FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt, int ry);
+FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, int *tt, int *ry);
+/* Launch statistics of the last solve on ws: step launches, host syncs. */
+FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTPROX_ROCM_H */
