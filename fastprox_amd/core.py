@@ -30,7 +30,14 @@ def lib():
 class FastproxError(RuntimeError):
     def __init__(self, status, what=""):
         self.status = status
-        msg = lib().fp_strerror(status).decode()
+        L = lib()
+        msg = L.fp_strerror(status).decode()
+        f = getattr(L, "fpr_last_error", None)
+        if f is not None:
+            f.restype = C.c_char_p
+            detail = f().decode()
+            if detail:
+                msg += f": {detail}"
         super().__init__(f"{what}: {A.STATUS.get(status, status)} ({msg})" if what else msg)
 
 

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -80,6 +81,14 @@ struct fp_op {
 namespace {
 
 using namespace fpr;
+
+thread_local char g_last_error[256] = "";
+
+// Records why the current call fails (fpr_last_error) and returns s.
+fp_status fail(fp_status s, const char *why) {
+    std::snprintf(g_last_error, sizeof g_last_error, "%s", why);
+    return s;
+}
 
 constexpr double kMaxScaledStep = 0x1p960;  // the step range (op.hpp:48)
 constexpr int64_t kMaxElems = (int64_t(1) << 60) - 1;
@@ -311,6 +320,7 @@ struct fp_workspace {
     void *xsave = nullptr, *zsave = nullptr;  // keep_structure: the point (x, z) of the last kept call
     std::vector<char> host_stage;             // host arrays, packed
     std::vector<double> gammas;
+    std::vector<fpr::ImagePtrs> ptrs;
     // results of the last call
     bool dual_valid = false, warm = false, seeded = false;
     bool info_valid = false;
@@ -475,15 +485,32 @@ fp_status build_engine(fp_workspace &w) {
     for (int64_t b = 0; b < w.B; ++b) {
         fpr::ImageSpec &s = specs[(size_t)b];
         for (int a = 0; a < 3; ++a) s.L[a] = w.img[a];
-        s.d_off = b * w.N;
-        s.z_off = b * w.N;
-        s.q_off = b * w.N;
         s.q_cstride = w.B * w.N;
-        s.sw_off = P.sw ? 0 : -1;
         s.sw_cstride = w.N;
     }
+    w.ptrs.assign((size_t)w.B, fpr::ImagePtrs{});
     w.eng.reset(new fpr::Engine());
     return hip_status(w.eng->setup((int)w.tsize, P, w.kc, specs));
+}
+
+// kind 0: the streaming kernel (tt iterations, segments of ry rows); 1: the
+// generic one-iteration kernel; 2: the block kernel (tt, ry).
+fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, fpr::KernelChoice &kc) {
+    kc = fpr::KernelChoice{};
+    if (kind == 1) {
+        kc.generic = true;
+        kc.stream = false;
+        kc.TT = 1;
+        kc.RY = 4;
+        return FP_OK;
+    }
+    if (kind != 0 && kind != 2) return fail(FP_EINVAL, "kernel kind must be 0 (stream), 1 (generic) or 2 (block)");
+    kc.stream = kind == 0;
+    kc.TT = tt;
+    kc.RY = ry;
+    if (k != 2) return fail(FP_EINVAL, "the tiled kernels are 2D only");
+    if (!fpr::kernel_available(tsize, kc.stream, tt, ry)) return fail(FP_EINVAL, "no such kernel built");
+    return FP_OK;
 }
 
 fp_status alloc(void **p, size_t bytes) {
@@ -645,7 +672,14 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     const void *sw = op->device_weights(ws->device, (int)ws->tsize, err);
     if (err) return hip_status(err);
     const bool warm = ws->dual_valid && (ws->warm || ws->seeded);
-    int r = ws->eng->solve(s, d, z, ws->q, sw, ws->gammas.data(), warm, o.tol, o.max_iter);
+    for (int64_t b = 0; b < B; ++b) {
+        fpr::ImagePtrs &p = ws->ptrs[(size_t)b];
+        p.d = (const char *)d + (size_t)(b * ws->N) * ws->tsize;
+        p.z = (char *)z + (size_t)(b * ws->N) * ws->tsize;
+        p.q = (char *)ws->q + (size_t)(b * ws->N) * ws->tsize;
+        p.sw = sw;
+    }
+    int r = ws->eng->solve(s, ws->ptrs.data(), ws->gammas.data(), warm, o.tol, o.max_iter);
     ws->seeded = false;
     if (r) {
         ws->dual_valid = false;
@@ -795,6 +829,44 @@ fp_status op_check(const fp_op *op, int *row_parallel_only) {
     if (!op) return FP_EINVAL;
     if (row_parallel_only) *row_parallel_only = 0;
     return FP_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batches of uneven images
+// ---------------------------------------------------------------------------
+
+struct fpr_batch {
+    const fp_op *op = nullptr;
+    int dtype = FP_F32;
+    size_t tsize = 4;
+    int weighted = 0;
+    int device = 0;
+    std::vector<fpr::ImageSpec> specs;
+    fpr::KernelChoice kc;
+    std::unique_ptr<fpr::Engine> eng;
+    std::vector<fpr::ImagePtrs> ptrs;
+    std::vector<double> gammas, gaps, zerrs;
+    std::vector<int> iters, conv;
+    bool valid = false;
+};
+
+namespace {
+
+fp_status batch_build(fpr_batch &b) {
+    const fp_op &op = *b.op;
+    fpr::Problem P{};
+    P.k = op.k;
+    P.iso = op.iso ? 1 : 0;
+    P.sw = b.weighted ? 1 : 0;
+    P.sw_per_image = 1;
+    for (int a = 0; a < 3; ++a) P.w[a] = op.w[a];
+    P.sched = op.iso ? fpr::kSchedC : fpr::kSchedDevice;
+    P.wmax_norm = op.iso ? 1 : 0;
+    P.alpha_max = 1.0;  // per image, from its weights (the prologue)
+    b.eng.reset(new fpr::Engine());
+    return hip_status(b.eng->setup((int)b.tsize, P, b.kc, b.specs));
 }
 
 }  // namespace
@@ -1006,18 +1078,14 @@ FP_API const fpr_plugin_table *fastprox_rocm_plugin(void) {
     return &table;
 }
 
+FP_API const char *fpr_last_error(void) { return g_last_error; }
+
 FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt, int ry) {
     return guard([&] {
-        if (!ws) return FP_EINVAL;
+        g_last_error[0] = 0;
+        if (!ws) return fail(FP_EINVAL, "null workspace");
         fpr::KernelChoice kc;
-        kc.generic = generic != 0;
-        kc.TT = tt;
-        kc.RY = ry;
-        if (!kc.generic && (ws->k != 2 || !fpr::kernel_available((int)ws->tsize, tt, ry))) return FP_EINVAL;
-        if (kc.generic) {
-            kc.TT = 1;
-            kc.RY = 4;
-        }
+        if (fp_status st = kernel_choice(generic, tt, ry, ws->k, (int)ws->tsize, kc)) return st;
         ws->kc = kc;
         if (ws->numel == 0) return FP_OK;
         ws->dual_valid = false;
@@ -1027,9 +1095,171 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
 
 FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, int *tt, int *ry) {
     if (!ws) return FP_EINVAL;
-    if (generic) *generic = ws->kc.generic ? 1 : 0;
+    if (generic) *generic = ws->kc.generic ? 1 : (ws->kc.stream ? 0 : 2);
     if (tt) *tt = ws->kc.TT;
     if (ry) *ry = ws->kc.RY;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count, const int64_t *shapes,
+                                  int weighted, fpr_batch **out) {
+    return guard([&] {
+        if (!out) return FP_EINVAL;
+        *out = nullptr;
+        if (!op || count < 0 || (count > 0 && !shapes)) return FP_EINVAL;
+        const int dt = enum_bits(dtype);
+        if (dt != FP_F32 && dt != FP_F64) return FP_EDTYPE;
+        if (count > (int64_t(1) << 24)) return FP_ESHAPE;
+        std::unique_ptr<fpr_batch> b(new fpr_batch());
+        b->op = op;
+        b->dtype = dt;
+        b->tsize = dt == FP_F32 ? 4 : 8;
+        b->weighted = weighted ? 1 : 0;
+        if (hipGetDevice(&b->device)) return FP_EDEVICE;
+        const int k = op->k;
+        b->specs.resize((size_t)count);
+        int64_t total = 0;
+        for (int64_t i = 0; i < count; ++i) {
+            fpr::ImageSpec &sp = b->specs[(size_t)i];
+            sp.L[0] = sp.L[1] = sp.L[2] = 1;
+            double n = 1.0;
+            for (int a = 0; a < k; ++a) {
+                const int64_t e = shapes[i * k + a];
+                if (e < 1) return FP_ESHAPE;
+                sp.L[3 - k + a] = e;
+                n *= (double)e;
+            }
+            if (!(n < 2147483647.0)) return FP_ESHAPE;
+            const int64_t N = (int64_t)n;
+            sp.q_cstride = N;
+            sp.sw_cstride = N;
+            total += N;
+            if (!b->weighted && !op->spatial.empty())
+                for (int a = 0; a < k; ++a)
+                    if (sp.L[3 - k + a] != op->sshape[a]) return FP_ESHAPE;
+        }
+        (void)total;
+        b->kc = fpr::default_kernel(k, (int)b->tsize, count ? b->specs[0].L : nullptr, op->iso,
+                                    b->weighted != 0, count);
+        b->ptrs.assign((size_t)count, fpr::ImagePtrs{});
+        b->gammas.assign((size_t)count, 0.0);
+        b->gaps.assign((size_t)count, 0.0);
+        b->zerrs.assign((size_t)count, 0.0);
+        b->iters.assign((size_t)count, 0);
+        b->conv.assign((size_t)count, 1);
+        if (fp_status s = batch_build(*b)) return s;
+        *out = b.release();
+        return FP_OK;
+    });
+}
+
+FP_API void fpr_batch_free(fpr_batch *b) { delete b; }
+
+FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp_options *opt, int warm,
+                                 void *stream) {
+    return guard([&] {
+        if (!b || (!images && !b->specs.empty())) return FP_EINVAL;
+        const fp_options o = opt ? *opt : defaults();
+        if (!(o.tol >= 0.0) || !std::isfinite(o.tol) || o.max_iter < 1) return FP_EPARAM;
+        const fp_op &op = *b->op;
+        const int64_t B = (int64_t)b->specs.size();
+        hipError_t err;
+        const void *opw = b->weighted ? nullptr : op.device_weights(b->device, (int)b->tsize, err);
+        for (int64_t i = 0; i < B; ++i) {
+            const fpr_image &im = images[i];
+            if (!im.x || !im.z || !im.dual) return FP_EINVAL;
+            if (b->weighted && !im.weights) return FP_EINVAL;
+            if (!std::isfinite(im.gamma) || im.gamma < 0.0) return FP_EPARAM;
+            if (!b->weighted)
+                if (fp_status s = step_ok(op, im.gamma)) return s;
+            fpr::ImagePtrs &p = b->ptrs[(size_t)i];
+            p.d = im.x;
+            p.z = im.z;
+            p.q = im.dual;
+            p.sw = b->weighted ? im.weights : opw;
+            b->gammas[(size_t)i] = im.gamma;
+        }
+        b->valid = false;
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (cur != b->device && hipSetDevice(b->device)) return FP_EDEVICE;
+        const int r = b->eng->solve((hipStream_t)stream, b->ptrs.data(), b->gammas.data(), warm != 0, o.tol,
+                                    o.max_iter);
+        if (cur != b->device) (void)hipSetDevice(cur);
+        if (r) return hip_status(r);
+        const fpr::ImgState *S = b->eng->state();
+        bool bad = false, notconv = false;
+        for (int64_t i = 0; i < B; ++i) {
+            const fpr::ImgState &t = S[i];
+            if (t.status) bad = true;
+            double g = t.gp;
+            if (op.iso && g == g) g = g < 0.0 ? 0.0 : g;
+            if (op.iso && t.e) g = std::ldexp(g, 2 * t.e);
+            b->gaps[(size_t)i] = g;
+            b->iters[(size_t)i] = t.it;
+            b->zerrs[(size_t)i] = t.zerr;
+            b->conv[(size_t)i] = t.converged;
+            if (!t.converged) notconv = true;
+        }
+        if (bad) return FP_EINVAL;
+        b->valid = true;
+        return notconv ? FP_ENOTCONV : FP_OK;
+    });
+}
+
+FP_API fp_status fpr_batch_image_info(const fpr_batch *b, double *gaps, int *iters, double *zerrs, int *converged,
+                                      int64_t count) {
+    if (!b || !b->valid) return FP_EINVAL;
+    if (count != (int64_t)b->specs.size()) return FP_ESHAPE;
+    for (int64_t i = 0; i < count; ++i) {
+        if (gaps) gaps[i] = b->gaps[(size_t)i];
+        if (iters) iters[i] = b->iters[(size_t)i];
+        if (zerrs) zerrs[i] = b->zerrs[(size_t)i];
+        if (converged) converged[i] = b->conv[(size_t)i];
+    }
+    return FP_OK;
+}
+
+FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry) {
+    return guard([&] {
+        if (!b) return FP_EINVAL;
+        fpr::KernelChoice kc;
+        if (fp_status st = kernel_choice(generic, tt, ry, b->op->k, (int)b->tsize, kc)) return st;
+        b->kc = kc;
+        b->valid = false;
+        const bool timing = b->eng && b->eng->timing;
+        fp_status st = batch_build(*b);
+        if (!st) b->eng->timing = timing;
+        return st;
+    });
+}
+
+FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry) {
+    if (!b) return FP_EINVAL;
+    if (generic) *generic = b->kc.generic ? 1 : (b->kc.stream ? 0 : 2);
+    if (tt) *tt = b->kc.TT;
+    if (ry) *ry = b->kc.RY;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs) {
+    if (!b) return FP_EINVAL;
+    if (launches) *launches = b->eng ? b->eng->stats.launches : 0;
+    if (host_syncs) *host_syncs = b->eng ? b->eng->stats.host_syncs : 0;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on) {
+    if (!b || !b->eng) return FP_EINVAL;
+    b->eng->timing = on != 0;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches) {
+    if (!b || !b->eng) return FP_EINVAL;
+    if (step_ms) *step_ms = b->eng->stats.step_ms;
+    if (epilogue_ms) *epilogue_ms = b->eng->stats.epi_ms;
+    if (launches) *launches = b->eng->stats.timed;
     return FP_OK;
 }
 

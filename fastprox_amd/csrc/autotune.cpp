@@ -17,33 +17,38 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
     KernelChoice kc;
     if (k != 2) {
         kc.generic = true;
+        kc.stream = false;
         kc.TT = 1;
         kc.RY = 4;
         return kc;
     }
-    if (const char *e = std::getenv("FASTPROX_ROCM_KERNEL")) {  // "generic" or "TT,RY"
+    // FASTPROX_ROCM_KERNEL: "generic", "stream:TT,SEG" or "block:TT,RY".
+    if (const char *e = std::getenv("FASTPROX_ROCM_KERNEL")) {
         if (e[0] == 'g') {
             kc.generic = true;
+            kc.stream = false;
             kc.TT = 1;
             kc.RY = 4;
             return kc;
         }
-        int tt = std::atoi(e), ry = 0;
-        for (const char *p = e; *p; ++p)
-            if (*p == ',') ry = std::atoi(p + 1);
-        if (kernel_available(tsize, tt, ry)) {
-            kc.TT = tt;
-            kc.RY = ry;
-            return kc;
+        const bool stream = e[0] == 's';
+        const char *c = e;
+        while (*c && *c != ':') ++c;
+        if (*c == ':') {
+            int tt = std::atoi(c + 1), ry = 0;
+            for (const char *p = c; *p; ++p)
+                if (*p == ',') ry = std::atoi(p + 1);
+            if (kernel_available(tsize, stream, tt, ry)) {
+                kc.stream = stream;
+                kc.TT = tt;
+                kc.RY = ry;
+                return kc;
+            }
         }
     }
-    if (tsize == 4) {
-        kc.TT = 4;
-        kc.RY = 32;
-    } else {
-        kc.TT = 2;
-        kc.RY = 16;
-    }
+    kc.stream = true;
+    kc.TT = tsize == 4 ? 8 : 4;
+    kc.RY = 64;
     return kc;
 }
 

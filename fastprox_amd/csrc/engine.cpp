@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace fpr {
 
@@ -25,6 +26,7 @@ void Engine::release() {
     f(counts_);
     f(gam_);
     if (hstate_) (void)hipHostFree(hstate_);
+    if (hdesc_pinned_) (void)hipHostFree(hdesc_pinned_);
     if (hcount_) (void)hipHostFree(hcount_);
     if (hgam_) (void)hipHostFree(hgam_);
     arena_ = nullptr;
@@ -35,6 +37,7 @@ void Engine::release() {
     counts_ = nullptr;
     gam_ = nullptr;
     hstate_ = nullptr;
+    hdesc_pinned_ = nullptr;
     hcount_ = nullptr;
     hgam_ = nullptr;
 }
@@ -59,11 +62,11 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
         for (int a = 0; a < 3; ++a) D.L[a] = (int32_t)s.L[a];
         const int64_t N = s.L[0] * s.L[1] * s.L[2];
         maxN_ = std::max(maxN_, N);
-        D.d_off = s.d_off;
-        D.z_off = s.z_off;
-        D.q_off = s.q_off;
+        D.d = nullptr;
+        D.z = nullptr;
+        D.q = nullptr;
+        D.sw = nullptr;
         D.q_cstride = s.q_cstride;
-        D.sw_off = s.sw_off;
         D.sw_cstride = s.sw_cstride;
         D.st_off = arena;
         arena += 4 * (int64_t)k * N;
@@ -91,31 +94,26 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
     if ((e = hipMalloc((void **)&counts_, 64))) return e;
     if ((e = hipMalloc((void **)&gam_, sizeof(double) * (size_t)std::max(B, 1)))) return e;
     if ((e = hipHostMalloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(B, 1), 0))) return e;
+    if ((e = hipHostMalloc((void **)&hdesc_pinned_, sizeof(ImgDesc) * (size_t)std::max(B, 1), 0))) return e;
     if ((e = hipHostMalloc((void **)&hcount_, 64, 0))) return e;
     if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)std::max(B, 1), 0))) return e;
-    if (B > 0 && (e = hipMemcpy(desc_, hdesc_.data(), sizeof(ImgDesc) * (size_t)B, hipMemcpyHostToDevice)))
-        return e;
     return hipSuccess;
 }
 
 template <class T>
-static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, const T *d, T *z, T *q,
-                     const T *sw, void *arena, ImgDesc *desc, ImgState *state, double *part, Tile **list,
+static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, void *arena,
+                     ImgDesc *desc, ImgState *state, double *part, Tile **list,
                      int *counts, double *gam, int B, int total_tiles, int64_t maxN, bool tv1d, int *hcount,
                      int steps_per_sync, int max_iter) {
     const double u_round = 0.5 * (sizeof(T) == 4 ? (double)FLT_EPSILON : DBL_EPSILON);
     const double eta = 0.5 * (sizeof(T) == 4 ? (double)FLT_TRUE_MIN : DBL_TRUE_MIN);
-    launch_prologue<T>(P, d, desc, state, gam, B, s);
+    launch_prologue<T>(P, desc, state, gam, B, s);
     launch_reset_counts(counts, s);
     launch_initial_list(desc, state, list[0], counts, B, s);
-    if (P.warm) launch_warm<T>(P, q, sw, (T *)arena, desc, state, B, maxN, s);
-    if (tv1d) launch_tv1d<T>(P, d, z, q, desc, state, B, maxN, u_round, eta, s);
+    if (P.warm) launch_warm<T>(P, (T *)arena, desc, state, B, maxN, s);
+    if (tv1d) launch_tv1d<T>(P, desc, state, B, maxN, u_round, eta, s);
     StepArgs<T> A;
-    A.d = d;
     A.arena = (T *)arena;
-    A.z = z;
-    A.q = q;
-    A.sw = sw;
     A.desc = desc;
     A.state = state;
     A.part = part;
@@ -132,28 +130,62 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     int64_t launched = 0;
     hipError_t err = hipGetLastError();
     if (err) return err;
+    // Optional timing: an event before and after every step and epilogue,
+    // read back after each chunk of launches.
+    const bool timing = E.timing;
+    std::vector<hipEvent_t> ev;
+    if (timing) {
+        ev.resize((size_t)(3 * steps_per_sync));
+        for (auto &x : ev)
+            if ((err = hipEventCreate(&x))) return err;
+    }
+    auto destroy = [&]() {
+        for (auto &x : ev) (void)hipEventDestroy(x);
+    };
     for (;;) {
         for (int i = 0; i < steps_per_sync; ++i) {
             A.list_in = list[p];
             A.count_in = counts + p;
             A.count_out = counts + (p ^ 1);
+            if (timing) (void)hipEventRecord(ev[(size_t)(3 * i)], s);
             launch_step<T>(kc, P.iso != 0, P.sw != 0, A, grid, s);
+            if (timing) (void)hipEventRecord(ev[(size_t)(3 * i + 1)], s);
             launch_epilogue(desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s);
+            if (timing) (void)hipEventRecord(ev[(size_t)(3 * i + 2)], s);
             p ^= 1;
             ++launched;
         }
         E.stats.launches += steps_per_sync;
-        if ((err = hipMemcpyAsync(hcount, counts + p, sizeof(int), hipMemcpyDeviceToHost, s))) return err;
-        if ((err = hipStreamSynchronize(s))) return err;
+        if ((err = hipMemcpyAsync(hcount, counts + p, sizeof(int), hipMemcpyDeviceToHost, s))) {
+            destroy();
+            return err;
+        }
+        if ((err = hipStreamSynchronize(s))) {
+            destroy();
+            return err;
+        }
         ++E.stats.host_syncs;
+        if (timing)
+            for (int i = 0; i < steps_per_sync; ++i) {
+                float a = 0.f, b = 0.f;
+                (void)hipEventElapsedTime(&a, ev[(size_t)(3 * i)], ev[(size_t)(3 * i + 1)]);
+                (void)hipEventElapsedTime(&b, ev[(size_t)(3 * i + 1)], ev[(size_t)(3 * i + 2)]);
+                E.stats.step_ms += a;
+                E.stats.epi_ms += b;
+                ++E.stats.timed;
+            }
         if (*hcount == 0) break;
-        if (launched > cap) return -1;  // the schedule did not finish: an internal error
+        if (launched > cap) {
+            destroy();
+            return -1;  // the schedule did not finish: an internal error
+        }
     }
+    destroy();
     return hipSuccess;
 }
 
-int Engine::solve(hipStream_t s, const void *d, void *z, void *q, const void *sw, const double *gammas, bool warm,
-                  double tol, int max_iter) {
+int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol,
+                  int max_iter) {
     const int B = nimg();
     stats = SolveStats{};
     if (B == 0) return hipSuccess;
@@ -161,17 +193,24 @@ int Engine::solve(hipStream_t s, const void *d, void *z, void *q, const void *sw
     P_.tol = tol;
     P_.max_iter = max_iter;
     std::memcpy(hgam_, gammas, sizeof(double) * (size_t)B);
+    for (int b = 0; b < B; ++b) {
+        ImgDesc D = hdesc_[(size_t)b];
+        D.d = ptrs[b].d;
+        D.z = ptrs[b].z;
+        D.q = ptrs[b].q;
+        D.sw = ptrs[b].sw;
+        hdesc_pinned_[b] = D;
+    }
     hipError_t e = hipMemcpyAsync(gam_, hgam_, sizeof(double) * (size_t)B, hipMemcpyHostToDevice, s);
     if (e) return e;
+    if ((e = hipMemcpyAsync(desc_, hdesc_pinned_, sizeof(ImgDesc) * (size_t)B, hipMemcpyHostToDevice, s))) return e;
     int r;
     if (tsize_ == 4)
-        r = run_solve<float>(*this, P_, kc_, s, (const float *)d, (float *)z, (float *)q, (const float *)sw, arena_,
-                             desc_, state_, part_, list_, counts_, gam_, B, total_tiles_, maxN_, tv1d_possible_,
-                             hcount_, steps_per_sync, max_iter);
+        r = run_solve<float>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, B, total_tiles_,
+                             maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
     else
-        r = run_solve<double>(*this, P_, kc_, s, (const double *)d, (double *)z, (double *)q, (const double *)sw,
-                              arena_, desc_, state_, part_, list_, counts_, gam_, B, total_tiles_, maxN_,
-                              tv1d_possible_, hcount_, steps_per_sync, max_iter);
+        r = run_solve<double>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, B, total_tiles_,
+                              maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
     if (r) return r;
     if ((e = hipMemcpyAsync(hstate_, state_, sizeof(ImgState) * (size_t)B, hipMemcpyDeviceToHost, s))) return e;
     return hipStreamSynchronize(s);

@@ -14,16 +14,27 @@
 
 namespace fpr {
 
-// One image of a batch: its extents (k axes), and where its data lives
-// relative to the base pointers of a solve (offsets in elements).
+// One image of a batch: its extents and the layout of its dual and weights.
 struct ImageSpec {
-    int64_t L[3];       // extents, padded in front with 1
-    int64_t d_off, z_off, q_off, q_cstride, sw_off, sw_cstride;
+    int64_t L[3];        // extents, padded in front with 1
+    int64_t q_cstride;   // distance between the components of its dual
+    int64_t sw_cstride;  // distance between its edge-weight planes
+};
+
+// Where one image's data lives for a solve (device pointers of the dtype).
+struct ImagePtrs {
+    const void *d;   // input, contiguous
+    void *z;         // output, contiguous
+    void *q;         // dual (component 0), read first on a warm start
+    const void *sw;  // spatial weights or null
 };
 
 struct SolveStats {
-    int launches = 0;   // step launches
+    int launches = 0;      // step launches
     int host_syncs = 0;
+    double step_ms = 0.0;  // summed step-kernel time (HIP events), when timing is on
+    double epi_ms = 0.0;   // summed epilogue time
+    int timed = 0;         // step launches timed
 };
 
 class Engine {
@@ -38,11 +49,9 @@ public:
     // the step kernel. Returns a hipError_t.
     int setup(int tsize, const Problem &P, const KernelChoice &kc, const std::vector<ImageSpec> &imgs);
 
-    // Solves every image: d, z, q and sw are the base pointers of the inputs,
-    // outputs, duals (read first when P.warm) and spatial weights. gammas:
-    // one step per image (host). Blocks until done; results in state().
-    int solve(hipStream_t s, const void *d, void *z, void *q, const void *sw, const double *gammas, bool warm,
-              double tol, int max_iter);
+    // Solves every image, ptrs[i] its data and gammas[i] its step (host
+    // arrays of nimg()). Blocks until done; results in state().
+    int solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
 
     const ImgState *state() const { return hstate_; }
     ImgState *device_state() const { return state_; }
@@ -52,6 +61,7 @@ public:
     const Problem &problem() const { return P_; }
     SolveStats stats;
     int steps_per_sync = 8;
+    bool timing = false;   // time every step and epilogue launch with HIP events
 
 private:
     void release();
@@ -70,6 +80,7 @@ private:
     Tile *list_[2] = {nullptr, nullptr};
     int *counts_ = nullptr;
     double *gam_ = nullptr;
+    ImgDesc *hdesc_pinned_ = nullptr;  // pinned
     ImgState *hstate_ = nullptr;  // pinned
     int *hcount_ = nullptr;       // pinned
     double *hgam_ = nullptr;      // pinned

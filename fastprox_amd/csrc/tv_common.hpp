@@ -40,12 +40,12 @@ enum : int { kSchedC = 0, kSchedDevice = 1 };
 enum : int { kAfterNone = 0, kAfterRestart = 1 };
 
 struct ImgDesc {
-    int64_t d_off;      // the image's input (contiguous, C order) in the d buffer
+    const void *d;      // the image's input (contiguous, C order)
+    void *z;            // its output z
+    void *q;            // component 0 of its dual
+    const void *sw;     // its spatial weights (alpha: N; beta: k planes), null for none
     int64_t st_off;     // its state (2 buffers x 2k planes x N) in the state arena
-    int64_t z_off;      // its output z in the z buffer
-    int64_t q_off;      // component 0 of its dual in the dual buffer
     int64_t q_cstride;  // distance between dual components
-    int64_t sw_off;     // its spatial weights (alpha: N; beta: k planes of N), -1 for none
     int64_t sw_cstride; // distance between edge-weight planes
     int32_t L[3];       // extents, padded in front with 1 (L[2] the contiguous axis)
     int32_t tile_base;  // index of the image's first tile in the partials
@@ -115,11 +115,7 @@ struct Problem {
 
 template <class T>
 struct StepArgs {
-    const T *d;          // inputs
     T *arena;            // states
-    T *z;                // outputs
-    T *q;                // duals
-    const T *sw;         // spatial weights or null
     const ImgDesc *desc;
     ImgState *state;
     double *part;        // kSlots per tile

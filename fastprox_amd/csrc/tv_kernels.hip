@@ -310,7 +310,7 @@ __device__ int rescale_exponent(double m, double gamma, double step) {  // image
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void prologue_kernel(Problem P, const T *d, const ImgDesc *desc,
+__global__ __launch_bounds__(256) void prologue_kernel(Problem P, const ImgDesc *desc,
                                                        ImgState *state, const double *gammas, int TT) {
     const int img = blockIdx.x;
     const int tid = threadIdx.x;
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const T *d, co
     ImgState &S = state[img];
     const int k = P.k;
     const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
-    const T *x = d + D.d_off;
+    const T *x = (const T *)D.d;
     __shared__ double sh[4];
     __shared__ double shared_e, shared_mean;
     __shared__ int flags;
@@ -336,6 +336,13 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const T *d, co
     if (nan) atomicOr(&flags, 1);
     if (inf) atomicOr(&flags, 2);
     m = block_max(m, sh);
+    double amax = P.alpha_max;
+    if (P.sw && P.sw_per_image && P.iso) {  // the image's own largest pixel weight (the step scale)
+        double am = 0.0;
+        const T *al = (const T *)D.sw;
+        for (int64_t i = tid; i < N; i += 256) am = fmax(am, (double)al[i]);
+        amax = block_max(am, sh);
+    }
     __syncthreads();
     const int fl = flags;
     // The image's axes (extents L[3-k..2]) and parameters.
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const T *d, co
                 }
             double ss = 0.0;
             for (int a = 0; a < k; ++a) ss = fmax(ss, P.w[a]);
-            e = na ? rescale_exponent(m, gamma0, gamma0 * ss * P.alpha_max) : 0;
+            e = na ? rescale_exponent(m, gamma0, gamma0 * ss * amax) : 0;
             for (int a = 0; a < k && e; ++a)
                 if (wa[a] > 0.0 && !(ldexp(gamma0 * wa[a], -e) > 0.0)) e = 0;
         }
@@ -491,7 +498,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const T *d, co
 // Iso: u = dual * wmax * 2^-e, zero on inactive axes, past the last entry and
 // where not finite, then projected onto the pixel's ball (tv_iso.cpp:456-477).
 template <class T>
-__global__ void warm_iso_kernel(Problem P, const T *q, const T *sw, T *arena, const ImgDesc *desc,
+__global__ void warm_iso_kernel(Problem P, T *arena, const ImgDesc *desc,
                                 const ImgState *state, int nimg) {
     const int img = blockIdx.y;
     if (img >= nimg) return;
@@ -507,14 +514,14 @@ __global__ void warm_iso_kernel(Problem P, const T *q, const T *sw, T *arena, co
         double nn = 0.0;
         for (int a = 0; a < k; ++a) {
             const int ax = 3 - k + a;
-            double v = ldexp((double)q[D.q_off + a * D.q_cstride + p] * S.wmax, -S.e);
+            double v = ldexp((double)((const T *)D.q)[a * D.q_cstride + p] * S.wmax, -S.e);
             const bool has = cc[ax] + 1 < D.L[ax];
             if (S.wa[a] == 0.0 || !has || !isfinite(v)) v = 0.0;
             u[a] = v;
             nn += v * v;
         }
         double rp = S.gamma;
-        if (P.sw) rp = iso_radius(S.gamma, (double)sw[D.sw_off + p], S.bound);
+        if (P.sw) rp = iso_radius(S.gamma, (double)((const T *)D.sw)[p], S.bound);
         if (sqrt(nn) > rp) {
             const double sc = rp / sqrt(nn);
             for (int a = 0; a < k; ++a) u[a] *= sc;
@@ -530,7 +537,7 @@ __global__ void warm_iso_kernel(Problem P, const T *q, const T *sw, T *arena, co
 // Aniso: u_a = -cumsum_a(q_a) / w_a along every fibre, clamped to the box,
 // zero at the last entry (_tv_torch.py:596-606, 242-243). One thread per fibre.
 template <class T>
-__global__ void warm_aniso_kernel(Problem P, const T *q, const T *sw, T *arena, const ImgDesc *desc,
+__global__ void warm_aniso_kernel(Problem P, T *arena, const ImgDesc *desc,
                                   const ImgState *state, int nimg) {
     const int img = blockIdx.y;
     if (img >= nimg) return;
@@ -552,10 +559,10 @@ __global__ void warm_aniso_kernel(Problem P, const T *q, const T *sw, T *arena, 
             double cs = 0.0;
             for (int64_t i = 0; i < L; ++i) {
                 const int64_t o = st + i * s;
-                cs += (double)q[D.q_off + a * D.q_cstride + o];
+                cs += (double)((const T *)D.q)[a * D.q_cstride + o];
                 double v = P.w[a] > 0.0 ? -cs / P.w[a] : 0.0;
                 double r = S.gamma;
-                if (P.sw) r = S.gamma * (double)sw[D.sw_off + a * D.sw_cstride + o];
+                if (P.sw) r = S.gamma * (double)((const T *)D.sw)[a * D.sw_cstride + o];
                 v = r > 0.0 ? fmin(fmax(v, -r), r) : 0.0;
                 if (i + 1 == L || !isfinite(v)) v = 0.0;
                 U[a * N + o] = (T)v;
@@ -638,7 +645,7 @@ __device__ void condat(const T *y, int64_t ys, T *x, int64_t xs, int64_t n, doub
 }
 
 template <class T>
-__global__ void tv1d_kernel(Problem P, const T *d, T *z, T *q, const ImgDesc *desc, ImgState *state, int nimg) {
+__global__ void tv1d_kernel(Problem P, const ImgDesc *desc, ImgState *state, int nimg) {
     const int img = blockIdx.y;
     if (img >= nimg) return;
     ImgState &S = state[img];
@@ -654,15 +661,18 @@ __global__ void tv1d_kernel(Problem P, const T *d, T *z, T *q, const ImgDesc *de
     for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
         const int64_t outer = f / s, inner = f % s;
         const int64_t st = outer * L * s + inner;
-        const T *y = d + D.d_off + st;
-        T *x = z + D.z_off + st;
+        const T *d = (const T *)D.d;
+        T *z = (T *)D.z;
+        T *q = (T *)D.q;
+        const T *y = d + st;
+        T *x = z + st;
         condat(y, s, x, s, L, S.tv1d_lam, e);
         double cs = 0.0;
         for (int64_t i = 0; i < L; ++i) {
             const int64_t o = st + i * s;
-            cs += ldexp((double)d[D.d_off + o], -e) - ldexp((double)z[D.z_off + o], -e);
+            cs += ldexp((double)d[o], -e) - ldexp((double)z[o], -e);
             for (int b = 0; b < k; ++b)
-                q[D.q_off + b * D.q_cstride + o] = b == a ? (T)(i + 1 < L ? ldexp(-cs / P.w[a], e) : 0.0) : (T)0;
+                q[b * D.q_cstride + o] = b == a ? (T)(i + 1 < L ? ldexp(-cs / P.w[a], e) : 0.0) : (T)0;
         }
     }
 }
@@ -722,9 +732,9 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
         for (int r = 0; r < RY; ++r)
             if ((intr >> r) & 1u) {
                 const int64_t o = (int64_t)(y0 + r) * W + x;
-                A.z[D.z_off + o] = nanv;
-                A.q[D.q_off + o] = nanv;
-                A.q[D.q_off + D.q_cstride + o] = nanv;
+                ((T *)D.z)[o] = nanv;
+                ((T *)D.q)[o] = nanv;
+                ((T *)D.q)[D.q_cstride + o] = nanv;
             }
         return;
     }
@@ -736,7 +746,7 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
     const T tau = (T)S.tau;
     const double gamd = S.gamma, boundd = S.bound;
     const T gam = (T)gamd;
-    const T *dp = A.d + D.d_off;
+    const T *dp = (const T *)D.d;
     const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
     const T *V = S.vsrc_u ? U : U + 2 * N;
 
@@ -756,7 +766,7 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
             v1[r] = in ? V[N + o] : (T)0;
         }
         if constexpr (SW) {
-            const T *sw = A.sw + D.sw_off;
+            const T *sw = (const T *)D.sw;
             if constexpr (ISO) {
                 ra[r] = in ? (T)iso_radius(gamd, (double)sw[o], boundd) : (T)0;
             } else {
@@ -862,13 +872,13 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
             const double zz = ((double)d[r] + w0d * ((double)u0[r] - up)) + w1d * ((double)u1[r] - ul);
             if ((intr >> r) & 1u) {
                 const int64_t o = (int64_t)(y0 + r) * W + x;
-                A.z[D.z_off + o] = (T)ldexp(zz, e);
+                ((T *)D.z)[o] = (T)ldexp(zz, e);
                 if constexpr (ISO) {
-                    A.q[D.q_off + o] = (T)ldexp((double)u0[r] / wmax, e);
-                    A.q[D.q_off + D.q_cstride + o] = (T)ldexp((double)u1[r] / wmax, e);
+                    ((T *)D.q)[o] = (T)ldexp((double)u0[r] / wmax, e);
+                    ((T *)D.q)[D.q_cstride + o] = (T)ldexp((double)u1[r] / wmax, e);
                 } else {
-                    A.q[D.q_off + o] = (T)(w0d * (up - (double)u0[r]));
-                    A.q[D.q_off + D.q_cstride + o] = (T)(w1d * (ul - (double)u1[r]));
+                    ((T *)D.q)[o] = (T)(w0d * (up - (double)u0[r]));
+                    ((T *)D.q)[D.q_cstride + o] = (T)(w1d * (ul - (double)u1[r]));
                 }
             }
         }
@@ -905,6 +915,314 @@ __global__ __launch_bounds__(256) void step2d_kernel(StepArgs<T> A) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The streaming 2D step kernel. A wavefront owns a strip of 64 columns (one
+// per lane; TT halo columns on each side) and walks down a segment of rows.
+// The TT iteration levels are pipelined one row apart: at each step the wave
+// loads one new row (level 0) and every level m computes the row m above it
+// from level m-1's last two rows. Each level keeps only its latest row
+// (v, u, z and, when the gap is due, z(u)) in registers, so TT iterations
+// cost one HBM round trip of the state without a vertical halo; the rows of
+// the segment's first and last TT steps are the pipeline's fill and drain.
+// ---------------------------------------------------------------------------
+
+template <class T>
+__device__ __forceinline__ T from_left(T v) {  // lane i receives lane i - 1's value (lane 0: 0)
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+}
+template <class T>
+__device__ __forceinline__ T from_right(T v) {  // lane i receives lane i + 1's value (lane 63: 0)
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+    } else {
+        const long long b = __builtin_bit_cast(long long, v);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+        return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+}
+
+constexpr int kAcc = 6;  // partial-sum slots per tile (flagged levels of one launch)
+
+__device__ __forceinline__ void acc_add(double (&a)[kAcc], int slot, double v) {
+#pragma unroll
+    for (int i = 0; i < kAcc; ++i) a[i] += (i == slot) ? v : 0.0;
+}
+
+template <class T, bool ISO, bool SW, int TT>
+__device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl, const int lane, const int SEG) {
+    constexpr int TXI = 64 - 2 * TT;
+    const int img = __builtin_amdgcn_readfirstlane(tl.img);
+    const int tx = __builtin_amdgcn_readfirstlane(tl.tx);
+    const int ty = __builtin_amdgcn_readfirstlane(tl.ty);
+    const int tloc = __builtin_amdgcn_readfirstlane(tl.pad_);
+    const ImgDesc &D = A.desc[img];
+    ImgState &S = A.state[img];
+    const int mode = S.mode;
+    const int H = D.L[1], W = D.L[2];
+    const int64_t N = (int64_t)H * W;
+    const int x = tx * TXI - TT + lane;
+    const bool xin = x >= 0 && x < W;
+    const bool lane_in = lane >= TT && lane < 64 - TT && xin;
+    const int y0 = ty * SEG;
+    const int yend = min(y0 + SEG, H);  // rows [y0, yend) are this tile's
+    double *part = A.part + (int64_t)(D.tile_base + tloc) * kSlots;
+
+    if (mode == kOutputNaN) {
+        if (lane_in)
+            for (int r = y0; r < yend; ++r) {
+                const int64_t o = (int64_t)r * W + x;
+                ((T *)D.z)[o] = (T)NAN;
+                ((T *)D.q)[o] = (T)NAN;
+                ((T *)D.q)[D.q_cstride + o] = (T)NAN;
+            }
+        return;
+    }
+
+    const int e = S.e;
+    const int cnt = S.cnt;
+    const uint32_t tm = S.test_mask, gm = S.gap_mask;
+    const bool fresh = S.fresh;
+    const T tau = (T)S.tau;
+    const double gamd = S.gamma, boundd = S.bound;
+    const T gam = (T)gamd;
+    const double w0d = S.wa[0], w1d = S.wa[1];
+    const T w0 = (T)w0d, w1 = (T)w1d;
+    const bool act0 = w0d != 0.0, act1 = w1d != 0.0;
+    const bool out_mode = mode == kOutput;
+    T beta[TT + 1];
+    int gslot[TT + 1], dslot[TT + 1];
+    {
+        int ns = 0;
+#pragma unroll
+        for (int m = 0; m <= TT; ++m) {
+            beta[m] = m >= 1 ? (T)S.beta[m] : (T)0;
+            gslot[m] = ((gm >> m) & 1u) ? ns++ : -1;
+        }
+#pragma unroll
+        for (int m = 0; m <= TT; ++m) dslot[m] = ((tm >> m) & 1u) ? ns++ : -1;
+    }
+    const T *dp = (const T *)D.d;
+    const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
+    const T *V = S.vsrc_u ? U : U + 2 * N;
+    T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N;
+    const T *sw = SW ? (const T *)D.sw : nullptr;
+
+    // Per level m < TT: its latest row's v, u, z(v) and z(u) (FP64).
+    T ov0[TT], ov1[TT], ou0[TT], ou1[TT], oz[TT];
+    double ozu[TT];
+    T dw[TT + 2];                                  // dw[m] = x_s at row ys - m
+    T ra[SW ? TT + 2 : 1], rb[(SW && !ISO) ? TT + 2 : 1];  // radii at row ys - m
+#pragma unroll
+    for (int m = 0; m < TT; ++m) {
+        ov0[m] = ov1[m] = ou0[m] = ou1[m] = oz[m] = (T)0;
+        ozu[m] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < TT + 2; ++m) {
+        dw[m] = (T)0;
+        if constexpr (SW) ra[m] = (T)0;
+        if constexpr (SW && !ISO) rb[m] = (T)0;
+    }
+    T uf0 = (T)0;  // the final level's previous row, u0
+    double acc[kAcc];
+#pragma unroll
+    for (int i = 0; i < kAcc; ++i) acc[i] = 0.0;
+
+    const int ystart = y0 - TT, ylast = yend - 1 + TT;
+    for (int ys = ystart; ys <= ylast; ++ys) {
+#pragma unroll
+        for (int m = TT + 1; m >= 1; --m) {
+            dw[m] = dw[m - 1];
+            if constexpr (SW) ra[m] = ra[m - 1];
+            if constexpr (SW && !ISO) rb[m] = rb[m - 1];
+        }
+        // Level 0: the new row.
+        const bool in = xin && ys >= 0 && ys < H;
+        const int64_t o = in ? (int64_t)ys * W + x : 0;
+        T nv0 = (T)0, nv1 = (T)0, nu0 = (T)0, nu1 = (T)0;
+        dw[0] = in ? scale_in(dp[o], e) : (T)0;
+        if (!fresh && in) {
+            nu0 = U[o];
+            nu1 = U[N + o];
+            nv0 = V[o];
+            nv1 = V[N + o];
+        }
+        if constexpr (SW) {
+            if constexpr (ISO) {
+                ra[0] = in ? (T)iso_radius(gamd, (double)sw[o], boundd) : (T)0;
+            } else {
+                ra[0] = in ? (T)(gamd * (double)sw[o]) : (T)0;
+                rb[0] = in ? (T)(gamd * (double)sw[D.sw_cstride + o]) : (T)0;
+            }
+        }
+        T zn = (dw[0] + w0 * (nv0 - ov0[0])) + w1 * (nv1 - from_left(nv1));
+        double zun = 0.0;
+        // The gap of level 0 (the loaded state) on row ys - 1.
+        if (gslot[0] >= 0) {
+            zun = ((double)dw[0] + w0d * ((double)nu0 - (double)ou0[0])) + w1d * ((double)nu1 - from_left((double)nu1));
+            const int r = ys - 1;
+            const double zc = ozu[0], zr = from_right(zc);
+            const bool cnt_r = lane_in && r >= y0 && r < yend;
+            const double g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0d * (zun - zc) : 0.0;
+            const double g1 = (act1 && xin && r >= 0 && r < H && x + 1 < W) ? w1d * (zr - zc) : 0.0;
+            double gp;
+            if constexpr (ISO) {
+                const double rp = SW ? (double)ra[1] : gamd;
+                gp = rp * sqrt(g0 * g0 + g1 * g1) - (g0 * (double)ou0[0] + g1 * (double)ou1[0]);
+            } else {
+                const double r0 = SW ? (double)ra[1] : gamd, r1 = SW ? (double)rb[1] : gamd;
+                gp = (r0 * fabs(g0) - g0 * (double)ou0[0]) + (r1 * fabs(g1) - g1 * (double)ou1[0]);
+            }
+            acc_add(acc, gslot[0], cnt_r ? gp : 0.0);
+        }
+        // The new row's values rise through the levels.
+        T pv0 = nv0, pv1 = nv1, pu0 = nu0, pu1 = nu1;  // level m - 1's new row
+        T pz = zn;
+        double pzu = zun;
+#pragma unroll
+        for (int m = 1; m <= TT; ++m) {
+            const int r = ys - m;  // the row level m produces
+            const T zc = oz[m - 1];
+            const T zr = from_right(zc);
+            T v0n, v1n, u0n, u1n;
+            if (m <= cnt) {
+                const bool rin = r >= 0 && r < H;
+                const T g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0 * (pz - zc) : (T)0;
+                const T g1 = (act1 && xin && rin && x + 1 < W) ? w1 * (zr - zc) : (T)0;
+                const T c0 = act0 ? ov0[m - 1] + tau * g0 : (T)0;
+                const T c1 = act1 ? ov1[m - 1] + tau * g1 : (T)0;
+                T n0, n1;
+                if constexpr (ISO) {
+                    const T nn = c0 * c0 + c1 * c1;
+                    const T nr = sqrt(nn);
+                    const T rp = SW ? ra[m] : gam;
+                    const T sc = nr > rp ? rp / nr : (T)1;
+                    n0 = c0 * sc;
+                    n1 = c1 * sc;
+                } else {
+                    const T r0 = SW ? ra[m] : gam, r1 = SW ? rb[m] : gam;
+                    n0 = c0 > r0 ? r0 : (c0 < -r0 ? -r0 : c0);
+                    n1 = c1 > r1 ? r1 : (c1 < -r1 ? -r1 : c1);
+                }
+                if (dslot[m] >= 0) {
+                    const bool cnt_r = lane_in && r >= y0 && r < yend;
+                    const double dd = ((double)ov0[m - 1] - (double)n0) * ((double)n0 - (double)ou0[m - 1]) +
+                                      ((double)ov1[m - 1] - (double)n1) * ((double)n1 - (double)ou1[m - 1]);
+                    acc_add(acc, dslot[m], cnt_r ? dd : 0.0);
+                }
+                v0n = n0 + beta[m] * (n0 - ou0[m - 1]);
+                v1n = n1 + beta[m] * (n1 - ou1[m - 1]);
+                u0n = n0;
+                u1n = n1;
+            } else {
+                v0n = ov0[m - 1];
+                v1n = ov1[m - 1];
+                u0n = ou0[m - 1];
+                u1n = ou1[m - 1];
+            }
+            // Level m - 1's latest row is now the one that just rose from it.
+            ov0[m - 1] = pv0;
+            ov1[m - 1] = pv1;
+            ou0[m - 1] = pu0;
+            ou1[m - 1] = pu1;
+            oz[m - 1] = pz;
+            ozu[m - 1] = pzu;
+            if (m < TT) {
+                if (m < cnt) pz = (dw[m] + w0 * (v0n - ov0[m])) + w1 * (v1n - from_left(v1n));
+                if (gslot[m] >= 0) {
+                    // The gap of level m on row r - 1.
+                    pzu = ((double)dw[m] + w0d * ((double)u0n - (double)ou0[m])) +
+                          w1d * ((double)u1n - from_left((double)u1n));
+                    const int rg = r - 1;
+                    const double zc2 = ozu[m], zr2 = from_right(zc2);
+                    const bool cnt_r = lane_in && rg >= y0 && rg < yend;
+                    const double g0 = (act0 && xin && rg >= 0 && rg + 1 < H) ? w0d * (pzu - zc2) : 0.0;
+                    const double g1 = (act1 && xin && rg >= 0 && rg < H && x + 1 < W) ? w1d * (zr2 - zc2) : 0.0;
+                    double gp;
+                    if constexpr (ISO) {
+                        const double rp = SW ? (double)ra[m + 1] : gamd;
+                        gp = rp * sqrt(g0 * g0 + g1 * g1) - (g0 * (double)ou0[m] + g1 * (double)ou1[m]);
+                    } else {
+                        const double r0 = SW ? (double)ra[m + 1] : gamd, r1 = SW ? (double)rb[m + 1] : gamd;
+                        gp = (r0 * fabs(g0) - g0 * (double)ou0[m]) + (r1 * fabs(g1) - g1 * (double)ou1[m]);
+                    }
+                    acc_add(acc, gslot[m], cnt_r ? gp : 0.0);
+                }
+                pv0 = v0n;
+                pv1 = v1n;
+                pu0 = u0n;
+                pu1 = u1n;
+            } else {
+                // The final level: row r leaves the wave.
+                const bool st = lane_in && r >= y0 && r < yend;
+                const int64_t so = st ? (int64_t)r * W + x : 0;
+                if (out_mode) {
+                    const double ul = from_left((double)u1n);
+                    const double zz =
+                        ((double)dw[TT] + w0d * ((double)u0n - (double)uf0)) + w1d * ((double)u1n - ul);
+                    if (st) {
+                        ((T *)D.z)[so] = (T)ldexp(zz, e);
+                        if constexpr (ISO) {
+                            ((T *)D.q)[so] = (T)ldexp((double)u0n / S.wmax, e);
+                            ((T *)D.q)[D.q_cstride + so] = (T)ldexp((double)u1n / S.wmax, e);
+                        } else {
+                            ((T *)D.q)[so] = (T)(w0d * ((double)uf0 - (double)u0n));
+                            ((T *)D.q)[D.q_cstride + so] = (T)(w1d * (ul - (double)u1n));
+                        }
+                    }
+                } else if (st && cnt > 0) {
+                    Uo[so] = u0n;
+                    Uo[N + so] = u1n;
+                    Uo[2 * N + so] = v0n;
+                    Uo[3 * N + so] = v1n;
+                }
+                uf0 = u0n;
+            }
+        }
+    }
+    // The tile's partial sums of the flagged levels.
+#pragma unroll
+    for (int m = 0; m <= TT; ++m) {
+        if (gslot[m] >= 0) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < kAcc; ++i) v += (i == gslot[m]) ? acc[i] : 0.0;
+            v = wave_sum(v);
+            if (lane == 0) part[2 * m] = v;
+        }
+        if (dslot[m] >= 0) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < kAcc; ++i) v += (i == dslot[m]) ? acc[i] : 0.0;
+            v = wave_sum(v);
+            if (lane == 0) part[2 * m + 1] = v;
+        }
+    }
+}
+
+template <class T, bool ISO, bool SW, int TT>
+__global__ __launch_bounds__(256) void stream2d_kernel(StepArgs<T> A, int SEG) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;
+    const int ntiles = *A.count_in;
+    const int nchunks = (ntiles + 3) >> 2;
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, j8 = blockIdx.x >> 3;
+    const int cpx = (nchunks + 7) >> 3;
+    const int cend = min((xcd + 1) * cpx, nchunks);
+    for (int c = xcd * cpx + j8; c < cend; c += per) {
+        const int ti = c * 4 + wv;
+        if (ti < ntiles) stream_tile<T, ISO, SW, TT>(A, A.list_in[ti], lane, SEG);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // The generic step kernel: one iteration per launch, one thread per pixel,
@@ -955,8 +1273,8 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
         double *part = A.part + (int64_t)(D.tile_base + tl.pad_) * kSlots;
         if (mode == kOutputNaN) {
             if (in) {
-                A.z[D.z_off + p] = (T)NAN;
-                for (int a = 0; a < k; ++a) A.q[D.q_off + a * D.q_cstride + p] = (T)NAN;
+                ((T *)D.z)[p] = (T)NAN;
+                for (int a = 0; a < k; ++a) ((T *)D.q)[a * D.q_cstride + p] = (T)NAN;
             }
             continue;
         }
@@ -970,11 +1288,11 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
             w[a] = (T)wd[a];
         }
         for (int a = 0; a < k; ++a) has[a] = in && c[3 - k + a] + 1 < D.L[3 - k + a] && wd[a] != 0.0;
-        const T *dp = A.d + D.d_off;
+        const T *dp = (const T *)D.d;
         const bool fresh = S.fresh;
         const T *U = A.arena + D.st_off + (int64_t)S.cur * 2 * k * N;
         const T *V = S.vsrc_u ? U : U + k * N;
-        const T *sw = SW ? A.sw + D.sw_off : nullptr;
+        const T *sw = SW ? (const T *)D.sw : nullptr;
         const double gamd = S.gamma;
         double rad[3] = {gamd, gamd, gamd};
         if (SW && in) {
@@ -1093,9 +1411,9 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
                 double qv;
                 if (ISO) qv = ldexp(fc / S.wmax, e);
                 else qv = wd[a] * (fp - fc);
-                A.q[D.q_off + a * D.q_cstride + p] = (T)qv;
+                ((T *)D.q)[a * D.q_cstride + p] = (T)qv;
             }
-            A.z[D.z_off + p] = (T)ldexp(zz, e);
+            ((T *)D.z)[p] = (T)ldexp(zz, e);
             continue;
         }
         if (!step) continue;
@@ -1170,9 +1488,9 @@ static int grid_for(int64_t n, int block, int cap = 65535) {
 }
 
 template <class T>
-void launch_prologue(const Problem &P, const T *d, const ImgDesc *desc, ImgState *state, const double *gammas,
-                     int nimg, hipStream_t s) {
-    hipLaunchKernelGGL(prologue_kernel<T>, dim3(nimg), dim3(256), 0, s, P, d, desc, state, gammas, P.TT);
+void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(prologue_kernel<T>, dim3(nimg), dim3(256), 0, s, P, desc, state, gammas, P.TT);
 }
 
 void launch_reset_counts(int *counts, hipStream_t s) {
@@ -1185,21 +1503,21 @@ void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list,
 }
 
 template <class T>
-void launch_warm(const Problem &P, const T *q, const T *sw, T *arena, const ImgDesc *desc, const ImgState *state,
-                 int nimg, int64_t maxN, hipStream_t s) {
+void launch_warm(const Problem &P, T *arena, const ImgDesc *desc, const ImgState *state, int nimg, int64_t maxN,
+                 hipStream_t s) {
     if (P.iso)
-        hipLaunchKernelGGL(warm_iso_kernel<T>, dim3(grid_for(maxN, 256, 4096), nimg), dim3(256), 0, s, P, q, sw,
-                           arena, desc, state, nimg);
+        hipLaunchKernelGGL(warm_iso_kernel<T>, dim3(grid_for(maxN, 256, 4096), nimg), dim3(256), 0, s, P, arena,
+                           desc, state, nimg);
     else
-        hipLaunchKernelGGL(warm_aniso_kernel<T>, dim3(grid_for(maxN, 256, 1024), nimg), dim3(256), 0, s, P, q, sw,
-                           arena, desc, state, nimg);
+        hipLaunchKernelGGL(warm_aniso_kernel<T>, dim3(grid_for(maxN, 256, 1024), nimg), dim3(256), 0, s, P, arena,
+                           desc, state, nimg);
 }
 
 template <class T>
-void launch_tv1d(const Problem &P, const T *d, T *z, T *q, const ImgDesc *desc, ImgState *state, int nimg,
-                 int64_t maxN, double u_round, double eta, hipStream_t s) {
-    hipLaunchKernelGGL(tv1d_kernel<T>, dim3(grid_for(maxN, 64, 4096), nimg), dim3(64), 0, s, P, d, z, q, desc,
-                       state, nimg);
+void launch_tv1d(const Problem &P, const ImgDesc *desc, ImgState *state, int nimg, int64_t maxN, double u_round,
+                 double eta, hipStream_t s) {
+    hipLaunchKernelGGL(tv1d_kernel<T>, dim3(grid_for(maxN, 64, 4096), nimg), dim3(64), 0, s, P, desc, state,
+                       nimg);
     hipLaunchKernelGGL(tv1d_finish_kernel, dim3(grid_for(nimg, 64)), dim3(64), 0, s, desc, state, nimg, u_round,
                        eta);
 }
@@ -1216,22 +1534,29 @@ static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid
         hipLaunchKernelGGL((step1_kernel<T, ISO, SW>), dim3(grid), dim3(256), 0, s, A);
         return;
     }
+    if (kc.stream) {
+#define FPR_SCASE(TTv)                                                                                   \
+    if (kc.TT == TTv) {                                                                                  \
+        hipLaunchKernelGGL((stream2d_kernel<T, ISO, SW, TTv>), dim3(grid), dim3(256), 0, s, A, kc.RY);   \
+        return;                                                                                          \
+    }
+        FPR_SCASE(2)
+        FPR_SCASE(4)
+        FPR_SCASE(6)
+        FPR_SCASE(8)
+#undef FPR_SCASE
+        return;
+    }
 #define FPR_CASE(TTv, RYv)                                                                              \
     if (kc.TT == TTv && kc.RY == RYv) {                                                                 \
         hipLaunchKernelGGL((step2d_kernel<T, ISO, SW, TTv, RYv>), dim3(grid), dim3(256), 0, s, A);      \
         return;                                                                                         \
     }
     if constexpr (sizeof(T) == 4) {
-        FPR_CASE(1, 32)
         FPR_CASE(2, 32)
         FPR_CASE(4, 32)
-        FPR_CASE(4, 24)
-        FPR_CASE(8, 32)
     } else {
-        FPR_CASE(1, 16)
         FPR_CASE(2, 16)
-        FPR_CASE(4, 16)
-        FPR_CASE(2, 24)
     }
 #undef FPR_CASE
 }
@@ -1247,11 +1572,10 @@ void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A
     }
 }
 
-bool kernel_available(int dtype_bytes, int TT, int RY) {
-    if (dtype_bytes == 4)
-        return (TT == 1 && RY == 32) || (TT == 2 && RY == 32) || (TT == 4 && RY == 32) || (TT == 4 && RY == 24) ||
-               (TT == 8 && RY == 32);
-    return (TT == 1 && RY == 16) || (TT == 2 && RY == 16) || (TT == 4 && RY == 16) || (TT == 2 && RY == 24);
+bool kernel_available(int dtype_bytes, bool stream, int TT, int RY) {
+    if (stream) return (TT == 2 || TT == 4 || TT == 6 || TT == 8) && RY >= 1 && RY <= 4096;
+    if (dtype_bytes == 4) return (TT == 2 && RY == 32) || (TT == 4 && RY == 32);
+    return TT == 2 && RY == 16;
 }
 
 template <class T>
@@ -1268,12 +1592,10 @@ void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
 }
 
 #define FPR_INST(T)                                                                                              \
-    template void launch_prologue<T>(const Problem &, const T *, const ImgDesc *, ImgState *, const double *, int, \
-                                     hipStream_t);                                                              \
-    template void launch_warm<T>(const Problem &, const T *, const T *, T *, const ImgDesc *, const ImgState *, int, \
-                                 int64_t, hipStream_t);                                                         \
-    template void launch_tv1d<T>(const Problem &, const T *, T *, T *, const ImgDesc *, ImgState *, int, int64_t,  \
-                                 double, double, hipStream_t);                                                  \
+    template void launch_prologue<T>(const Problem &, const ImgDesc *, ImgState *, const double *, int, hipStream_t);                                                              \
+    template void launch_warm<T>(const Problem &, T *, const ImgDesc *, const ImgState *, int, int64_t, hipStream_t);                                                         \
+    template void launch_tv1d<T>(const Problem &, const ImgDesc *, ImgState *, int, int64_t, double, double,      \
+                                 hipStream_t);                                                  \
     template void launch_step<T>(const KernelChoice &, bool, bool, const StepArgs<T> &, int, hipStream_t);       \
     template void launch_gather<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);            \
     template void launch_scatter<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);

@@ -19,29 +19,32 @@ struct StridedShape {
 // The step kernel of a solve: the register-tile 2D kernel with TT iterations
 // per launch on tiles of (64 - 2 TT) x (RY - 2 TT) pixels, or the generic
 // one-iteration kernel (2D and 3D).
+// stream: the streaming kernel (TT iterations, segments of RY rows); else
+// the block kernel (tiles of RY rows in registers, TT iterations).
 struct KernelChoice {
     bool generic = false;
+    bool stream = true;
     int TT = 4;
-    int RY = 32;
+    int RY = 64;
     int tile_x() const { return generic ? 64 : 64 - 2 * TT; }
-    int tile_y() const { return generic ? 4 : RY - 2 * TT; }
+    int tile_y() const { return generic ? 4 : (stream ? RY : RY - 2 * TT); }
     int iters() const { return generic ? 1 : TT; }
 };
 
-bool kernel_available(int dtype_bytes, int TT, int RY);
+bool kernel_available(int dtype_bytes, bool stream, int TT, int RY);
 
 template <class T>
-void launch_prologue(const Problem &P, const T *d, const ImgDesc *desc, ImgState *state, const double *gammas,
-                     int nimg, hipStream_t s);
+void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
+                     hipStream_t s);
 void launch_reset_counts(int *counts, hipStream_t s);
 void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list, int *count, int nimg,
                          hipStream_t s);
 template <class T>
-void launch_warm(const Problem &P, const T *q, const T *sw, T *arena, const ImgDesc *desc, const ImgState *state,
-                 int nimg, int64_t maxN, hipStream_t s);
+void launch_warm(const Problem &P, T *arena, const ImgDesc *desc, const ImgState *state, int nimg, int64_t maxN,
+                 hipStream_t s);
 template <class T>
-void launch_tv1d(const Problem &P, const T *d, T *z, T *q, const ImgDesc *desc, ImgState *state, int nimg,
-                 int64_t maxN, double u_round, double eta, hipStream_t s);
+void launch_tv1d(const Problem &P, const ImgDesc *desc, ImgState *state, int nimg, int64_t maxN, double u_round,
+                 double eta, hipStream_t s);
 template <class T>
 void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s);
 void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out, int *count_out,

@@ -1,0 +1,441 @@
+"""PyTorch entry points for image TV on the GPU, with the signatures and
+semantics of fastprox.torch.tv_iso / tv_aniso
+(bindings/python/fastprox/torch.py:305-351 in the reference).
+
+    z = tv_iso(x, 0.1)                       # x (..., H, W) on cuda, f32/f64
+    z, info = tv_aniso(x, 0.05, weights=[1.0, 0.5], tol=1e-6, return_info=True)
+
+Every solve runs in libfastprox_rocm (HIP kernels) on the tensor's device and
+torch's current stream; the derivatives (reverse and forward mode) are the
+implicit-differentiation systems of the reference, also on the device. There
+is no CPU fallback: a CPU tensor or a missing library raises."""
+
+from __future__ import annotations
+
+import contextlib
+import contextvars
+import ctypes as C
+import math
+import threading
+import warnings
+
+import numpy as np
+import torch
+
+from . import IterInfo
+from . import _capi as A
+from .core import FastproxError, Op, _check, lib
+
+
+class AccuracyWarning(UserWarning):
+    """An iterative prox certified less accuracy than asked for
+    (bindings/python/fastprox/__init__.py:602-605)."""
+
+
+class ConvergenceWarning(AccuracyWarning):
+    """A solve stopped at max_iter before its tolerance
+    (bindings/python/fastprox/__init__.py:608-612)."""
+
+
+_ACCURACY = contextvars.ContextVar("fastprox_amd_inner_accuracy", default=None)
+
+
+@contextlib.contextmanager
+def inner_accuracy(target, factor=0.1):
+    """Calls in the block that set neither tol nor ztol use ztol = factor *
+    target (bindings/python/fastprox/__init__.py:618-645)."""
+    target, factor = float(target), float(factor)
+    if not (0.0 < target < float("inf") and 0.0 < factor < float("inf")):
+        raise ValueError(f"target and factor must be finite and > 0, got {target} and {factor}")
+    ztol = factor * target
+    _sq_tol(ztol)
+    token = _ACCURACY.set(ztol)
+    try:
+        yield ztol
+    finally:
+        _ACCURACY.reset(token)
+
+
+def _sq_tol(ztol):
+    ztol = float(ztol)
+    if not 0.0 <= ztol < float("inf"):
+        raise ValueError(f"ztol must be finite and >= 0, got {ztol}")
+    if ztol > 0.0 and ztol * ztol < np.finfo(np.float64).tiny:
+        raise ValueError(f"ztol={ztol} is below 1e-154: its square, the gap tolerance, underflows")
+    return ztol * ztol
+
+
+def _call_tol(tol, ztol, default, warn=True):
+    """__init__.py:663-679: the call's tol or ztol, else inner_accuracy's ztol, else the default."""
+    if tol is not None and ztol is not None:
+        raise ValueError("pass tol or ztol, not both")
+    need = _ACCURACY.get()
+    if tol is None and ztol is None:
+        return default if need is None else _sq_tol(need)
+    t = float(tol) if ztol is None else _sq_tol(ztol)
+    if need is not None and warn and t > _sq_tol(need):
+        own = f"tol={float(tol):g}" if ztol is None else f"ztol={float(ztol):g}"
+        warnings.warn(f"{own} certifies less than inner_accuracy's ztol={need:g}", AccuracyWarning, stacklevel=4)
+    return t
+
+
+def _check_floor(x, tol, ztol, warn):
+    """__init__.py:82-94: a ztol below the dtype's unit roundoff cannot be certified."""
+    if not warn:
+        return
+    need = ztol if ztol is not None else (_ACCURACY.get() if tol is None else None)
+    floor = 0.5 * float(torch.finfo(x.dtype).eps)
+    if need is not None and 0.0 < float(need) < floor:
+        warnings.warn(f"ztol={float(need):g} is below the unit roundoff {floor:.3g} of {x.dtype}",
+                      AccuracyWarning, stacklevel=4)
+
+
+def tv_iso(x, gamma=1.0, *, ndim=2, weights=None, pixel_weights=None, tol=None, ztol=None, max_iter=1000,
+           warm_start=None, return_info=False, deriv_tol=None, deriv_max_iter=10000, warn=True):
+    r"""Isotropic TV of the last ``ndim`` axes of ``x`` (torch.py:305-332 in the
+    reference): prox of gamma sum_p alpha_p |(w_a D_a z)_p|_2, each image
+    stopping when its duality gap is at most tol |x|^2 / 2. ``gamma``: a
+    number or one step per image (broadcasting to the batch shape);
+    ``pixel_weights`` broadcast to x.shape. Differentiable in x, gamma,
+    weights and pixel_weights."""
+    t = _call_tol(tol, ztol, 1e-6, warn)
+    _check_floor(x, tol, ztol, warn)
+    return _device_tv(True, x, gamma, ndim, weights, t, max_iter, warm_start, return_info, deriv_tol,
+                      deriv_max_iter, pixel_weights, warn and "fastprox_amd.torch.tv_iso")
+
+
+def tv_aniso(x, gamma=1.0, *, ndim=2, weights=None, edge_weights=None, tol=None, ztol=None, max_iter=1000,
+             warm_start=None, return_info=False, deriv_tol=None, deriv_max_iter=10000, warn=True):
+    r"""Anisotropic TV of the last ``ndim`` axes (torch.py:335-351 in the
+    reference), by the device method (dual FISTA on the edge duals); the
+    dual is in the C form q_a with x - z = sum_a q_a. ``edge_weights``:
+    (ndim, ...) broadcasting to (ndim, *x.shape)."""
+    t = _call_tol(tol, ztol, 1e-6, warn)
+    _check_floor(x, tol, ztol, warn)
+    return _device_tv(False, x, gamma, ndim, weights, t, max_iter, warm_start, return_info, deriv_tol,
+                      deriv_max_iter, edge_weights, warn and "fastprox_amd.torch.tv_aniso")
+
+
+class _Cfg:
+    def __init__(self, iso, k, w, tol, max_iter, deriv_tol, deriv_max_iter):
+        self.iso, self.k, self.w = iso, k, w
+        self.tol, self.max_iter = float(tol), int(max_iter)
+        self.deriv_tol, self.deriv_max_iter = deriv_tol, int(deriv_max_iter)
+        self.info = None
+        self.deriv = None
+
+
+def _device_tv(iso, x, gamma, ndim, weights, tol, max_iter, warm_start, return_info, deriv_tol, deriv_max_iter,
+               spatial, warn):
+    if not isinstance(x, torch.Tensor):
+        raise TypeError("x must be a torch.Tensor")
+    if x.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"fastprox operates on float32 and float64 tensors, got {x.dtype}")
+    if ndim not in (2, 3):
+        raise ValueError("ndim must be 2 or 3")
+    if x.dim() < ndim:
+        raise ValueError(f"needs a tensor with at least {ndim} dimensions")
+    if not tol >= 0 or max_iter < 0:
+        raise ValueError("tol must be >= 0 and max_iter >= 0")
+    if not x.is_cuda:
+        raise ValueError("fastprox_amd runs on the GPU: x must be a cuda (ROCm) tensor")
+    wt = None
+    if weights is None:
+        w = [1.0] * ndim
+    elif isinstance(weights, torch.Tensor) and weights.is_floating_point():
+        wt = weights
+        w = [float(a) for a in weights.detach().cpu().reshape(-1)]
+    else:
+        w = [float(a) for a in np.atleast_1d(np.asarray(weights, np.float64))]
+    if len(w) == 1:
+        w = w * ndim
+    if len(w) != ndim or min(w) < 0 or not all(math.isfinite(a) for a in w):
+        raise ValueError("weights must be one or ndim finite values >= 0")
+    gb = _per_image(gamma, x, ndim)
+    warm = None
+    if warm_start is not None:
+        warm = torch.as_tensor(warm_start).to(dtype=x.dtype, device=x.device)
+        if tuple(warm.shape) != (ndim, *x.shape):
+            raise ValueError(f"warm_start must have shape {(ndim, *x.shape)}")
+    if deriv_tol is not None:
+        if not deriv_tol > 0:
+            raise ValueError("deriv_tol must be > 0 or None")
+        deriv_tol = float(deriv_tol)
+    cfg = _Cfg(iso, ndim, w, tol, max_iter, deriv_tol, deriv_max_iter)
+    sw = None if spatial is None else _spatial(spatial, x, ndim, iso)
+    if x.numel() == 0:
+        z = x.clone()
+        for t in (gb, wt, sw):
+            if t is not None:
+                z = z + 0 * t.sum()
+        cfg.info = IterInfo(0.0, 0, 0, True, torch.zeros((ndim, *x.shape), dtype=x.dtype, device=x.device), 0.0)
+        return (z, cfg.info) if return_info else z
+    if bool(torch.isinf(x.detach()).any()):
+        raise ValueError("x contains ±Inf; image operators reject infinite values")
+    z = _DeviceTV.apply(cfg, x, gb, wt, sw, warm)
+    if warn and not cfg.info.converged:
+        warnings.warn(f"{warn} stopped at max_iter={max_iter} before its tolerance; the certified relative "
+                      f"error of z is {cfg.info.zerr:.3g}", ConvergenceWarning, stacklevel=3)
+    return (z, cfg.info) if return_info else z
+
+
+def _spatial(sw, x, ndim, iso):
+    """torch.py:416-440: pixel weights as (B, *img), edge weights as (ndim, B, *img)."""
+    name = "pixel_weights" if iso else "edge_weights"
+    full = tuple(x.shape) if iso else (ndim, *x.shape)
+    t = sw if isinstance(sw, torch.Tensor) else torch.as_tensor(np.asarray(sw, np.float64))
+    t = t.to(dtype=x.dtype, device=x.device)
+    try:
+        if not iso:
+            if t.dim() < 1 or t.shape[0] != ndim or t.dim() - 1 > x.dim():
+                raise RuntimeError
+            t = t.reshape((ndim,) + (1,) * (x.dim() + 1 - t.dim()) + tuple(t.shape[1:]))
+        if torch.broadcast_shapes(t.shape, full) != full:
+            raise RuntimeError
+    except RuntimeError:
+        what = "broadcast to" if iso else f"have shape ({ndim}, ...) broadcasting to"
+        raise ValueError(f"{name} must {what} {full}, got shape {tuple(getattr(sw, 'shape', ()))}") from None
+    td = t.detach()
+    if bool(((td < 0) | ~torch.isfinite(td)).any()):
+        raise ValueError(f"{name} must be finite and >= 0")
+    img = tuple(x.shape[-ndim:])
+    return t.expand(full).reshape((-1,) + img if iso else (ndim, -1) + img)
+
+
+def _per_image(gamma, x, ndim):
+    """torch.py:443-463: gamma as one value per image, (B,)."""
+    batch = tuple(x.shape[:-ndim])
+    g = gamma if isinstance(gamma, torch.Tensor) else torch.as_tensor(np.asarray(gamma, np.float64))
+    if not g.is_floating_point():
+        g = g.to(torch.float64)
+    g = g.to(dtype=x.dtype, device=x.device)
+    if g.numel() == 1:
+        g = g.reshape(())
+    try:
+        if torch.broadcast_shapes(g.shape, batch) != batch:
+            raise RuntimeError
+    except RuntimeError:
+        raise ValueError(f"gamma must be a number or broadcast to the batch shape {batch}, "
+                         f"got shape {tuple(g.shape)}") from None
+    gd = g.detach()
+    if bool(((gd < 0) | ~torch.isfinite(gd)).any()):
+        raise ValueError("gamma must be finite and >= 0")
+    return g.expand(batch).reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# Batches through the C ABI (fpr_batch_*), cached per configuration
+# ---------------------------------------------------------------------------
+
+class _FprImage(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("z", C.c_void_p), ("dual", C.c_void_p), ("weights", C.c_void_p),
+                ("gamma", C.c_double)]
+
+
+def _declare_batch(L):
+    if getattr(L, "_batch_declared", False):
+        return
+    P = C.c_void_p
+    L.fpr_batch_create.restype = C.c_int
+    L.fpr_batch_create.argtypes = [P, C.c_int, C.c_int64, C.POINTER(C.c_int64), C.c_int, C.POINTER(P)]
+    L.fpr_batch_free.restype = None
+    L.fpr_batch_free.argtypes = [P]
+    L.fpr_batch_solve.restype = C.c_int
+    L.fpr_batch_solve.argtypes = [P, C.POINTER(_FprImage), C.POINTER(A.fp_options), C.c_int, P]
+    L.fpr_batch_image_info.restype = C.c_int
+    L.fpr_batch_image_info.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int), C.c_int64]
+    L.fpr_batch_set_kernel.restype = C.c_int
+    L.fpr_batch_set_kernel.argtypes = [P, C.c_int, C.c_int, C.c_int]
+    L.fpr_batch_stats.restype = C.c_int
+    L.fpr_batch_stats.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fpr_batch_set_timing.restype = C.c_int
+    L.fpr_batch_set_timing.argtypes = [P, C.c_int]
+    L.fpr_batch_timing.restype = C.c_int
+    L.fpr_batch_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L._batch_declared = True
+
+
+class Batch:
+    """fpr_batch_create: one persistent solve over images of (possibly)
+    different shapes, each with its own step and spatial weights."""
+
+    def __init__(self, op, dtype, shapes, weighted=False, kernel=None):
+        L = lib()
+        _declare_batch(L)
+        self.op = op
+        self.k = op.k
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        flat = np.ascontiguousarray(np.asarray(self.shapes, np.int64).reshape(-1))
+        self.h = C.c_void_p()
+        dt = A.FP_F32 if dtype == torch.float32 else A.FP_F64
+        _check(L.fpr_batch_create(op.h, dt, len(self.shapes), flat.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  int(weighted), C.byref(self.h)), "fpr_batch_create")
+        self.weighted = bool(weighted)
+        if kernel is not None:
+            self.set_kernel(*kernel)
+
+    def set_kernel(self, generic, tt=1, ry=4):
+        _check(lib().fpr_batch_set_kernel(self.h, int(generic), int(tt), int(ry)), "fpr_batch_set_kernel")
+
+    def solve(self, xs, zs, duals, gammas, weights=None, tol=1e-6, max_iter=1000, warm=False, stream=None):
+        """Lists of device tensors (contiguous); returns the status."""
+        n = len(self.shapes)
+        imgs = (_FprImage * max(n, 1))()
+        for i in range(n):
+            imgs[i].x = xs[i].data_ptr()
+            imgs[i].z = zs[i].data_ptr()
+            imgs[i].dual = duals[i].data_ptr()
+            imgs[i].weights = weights[i].data_ptr() if weights is not None else None
+            imgs[i].gamma = float(gammas[i])
+        opt = A.fp_options()
+        lib().fp_options_init(C.byref(opt))
+        opt.tol, opt.max_iter = float(tol), int(max_iter)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        s = lib().fpr_batch_solve(self.h, imgs, C.byref(opt), int(warm), C.c_void_p(stream or None))
+        return _check(s, "fpr_batch_solve", ok=(A.FP_OK, A.FP_ENOTCONV))
+
+    def image_info(self):
+        n = len(self.shapes)
+        gaps, its, zerrs, conv = np.empty(n), np.empty(n, np.int32), np.empty(n), np.empty(n, np.int32)
+        _check(lib().fpr_batch_image_info(self.h, gaps.ctypes.data_as(C.POINTER(C.c_double)),
+                                          its.ctypes.data_as(C.POINTER(C.c_int)),
+                                          zerrs.ctypes.data_as(C.POINTER(C.c_double)),
+                                          conv.ctypes.data_as(C.POINTER(C.c_int)), n), "fpr_batch_image_info")
+        return gaps, its, zerrs, conv.astype(bool)
+
+    def stats(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().fpr_batch_stats(self.h, C.byref(a), C.byref(b)), "fpr_batch_stats")
+        return a.value, b.value
+
+    def set_timing(self, on):
+        _check(lib().fpr_batch_set_timing(self.h, int(on)), "fpr_batch_set_timing")
+
+    def timing(self):
+        """(step-kernel ms, epilogue ms, step launches) of the last solve."""
+        a, b, c = C.c_double(), C.c_double(), C.c_int()
+        _check(lib().fpr_batch_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), "fpr_batch_timing")
+        return a.value, b.value, c.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            try:
+                lib().fpr_batch_free(self.h)
+            except Exception:
+                pass
+            self.h = C.c_void_p()
+
+
+_CACHE = {}
+_CACHE_LOCK = threading.Lock()
+_CACHE_MAX = 16
+
+
+def _batch_for(iso, k, w, dtype, device, B, img, weighted):
+    key = (iso, k, tuple(w), dtype, device, B, img, weighted)
+    with _CACHE_LOCK:
+        hit = _CACHE.pop(key, None)
+        if hit is None:
+            with torch.cuda.device(device):
+                op = Op(iso, k, w)
+                hit = Batch(op, dtype, [img] * B, weighted=weighted)
+        _CACHE[key] = hit
+        while len(_CACHE) > _CACHE_MAX:
+            _CACHE.pop(next(iter(_CACHE)))
+        return hit
+
+
+def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None):
+    """(z, dual (k, B, *img) in the C form, gaps, iters, zerrs, converged)
+    for x (B, *img) on the GPU; gamma (B,); warm: a dual (k, B, *img) or None;
+    sw: pixel weights (B, *img) or edge weights (k, B, *img)."""
+    k = len(w)
+    B = x.shape[0]
+    img = tuple(x.shape[1:])
+    xb = x.detach().contiguous()
+    z = torch.empty_like(xb)
+    dual = torch.empty((B, k) + img, dtype=x.dtype, device=x.device)
+    if warm is not None:
+        dual.copy_(warm.transpose(0, 1))
+    wlist = None
+    if sw is not None:
+        swd = sw.detach().to(dtype=x.dtype)
+        wlist = [swd[b].contiguous() for b in range(B)] if iso else \
+            [swd[:, b].contiguous() for b in range(B)]
+    bt = _batch_for(iso, k, w, x.dtype, x.device.index or 0, B, img, sw is not None)
+    g = gamma.detach().double().cpu().numpy().reshape(-1)
+    with torch.cuda.device(x.device):
+        bt.solve(list(xb), list(z), list(dual), g, weights=wlist, tol=tol, max_iter=max_iter,
+                 warm=warm is not None)
+    gaps, its, zerrs, conv = bt.image_info()
+    return z, dual.transpose(0, 1), gaps, its, zerrs, conv
+
+
+class _DeviceTV(torch.autograd.Function):
+    """torch.py:466-524: forward solves on the device; backward and forward
+    mode are the derivative systems at the solution (deriv.py)."""
+
+    @staticmethod
+    def forward(ctx, cfg, x, gamma, wt, sw, warm):
+        k = cfg.k
+        img = tuple(x.shape[-k:])
+        xb = x.detach().reshape((-1,) + img)
+        wb = None if warm is None else warm.reshape((k, -1) + img)
+        z, dual, gaps, its, zerrs, conv = solve(xb, gamma.detach(), cfg.w, cfg.iso, cfg.tol, cfg.max_iter, wb,
+                                                None if sw is None else sw.detach())
+        cfg.info = IterInfo(float(np.sum(gaps)), int(its.max()) if its.size else 0, 0, bool(conv.all()),
+                            dual.reshape((k,) + tuple(x.shape)), float(np.max(zerrs)) if zerrs.size else 0.0,
+                            gaps, its)
+        ctx.cfg, ctx.xb, ctx.z, ctx.dual, ctx.gam = cfg, xb, z, dual, gamma.detach()
+        ctx.gaps = gaps
+        ctx.shape = tuple(x.shape)
+        ctx.wt = wt
+        ctx.sw = None if sw is None else sw.detach()
+        ctx.S = None
+        return z.reshape(x.shape)
+
+    @staticmethod
+    def _structure(ctx):
+        if ctx.S is None:
+            from .deriv import Structure
+            cfg = ctx.cfg
+            ctx.S = Structure(ctx.xb, ctx.z, ctx.dual, ctx.gaps, ctx.gam, cfg.w, cfg.iso, ctx.sw)
+        return ctx.S
+
+    @staticmethod
+    def backward(ctx, c):
+        if torch.is_grad_enabled():
+            raise RuntimeError("fastprox_amd: second-order differentiation through the prox is not supported; "
+                               "use create_graph=False")
+        cfg = ctx.cfg
+        S = _DeviceTV._structure(ctx)
+        with torch.no_grad():
+            r, ggamma, gw, gsw, cfg.deriv = S.vjp(c.reshape(ctx.z.shape).contiguous(), cfg.deriv_tol,
+                                                 cfg.deriv_max_iter)
+        out_g = ggamma if ctx.needs_input_grad[2] else None
+        out_w = None
+        if ctx.wt is not None and ctx.needs_input_grad[3]:
+            v = gw.to(dtype=ctx.wt.dtype, device=ctx.wt.device)
+            out_w = v.sum().reshape(ctx.wt.shape) if ctx.wt.numel() == 1 else v.reshape(ctx.wt.shape)
+        out_x = r.reshape(ctx.shape) if ctx.needs_input_grad[1] else None
+        out_s = gsw if ctx.sw is not None and ctx.needs_input_grad[4] else None
+        return None, out_x, out_g, out_w, out_s, None
+
+    @staticmethod
+    def jvp(ctx, _cfg, dx, dgamma, dwt, dsw, _dwarm):
+        cfg = ctx.cfg
+        S = _DeviceTV._structure(ctx)
+        dw = None
+        if dwt is not None:
+            vals = [float(a) for a in dwt.detach().cpu().reshape(-1)]
+            dw = vals * cfg.k if len(vals) == 1 else vals
+        with torch.no_grad():
+            dz, cfg.deriv = S.jvp(None if dx is None else dx.reshape(ctx.z.shape), dgamma, dw, cfg.deriv_tol,
+                                  cfg.deriv_max_iter, dsw)
+        return dz.reshape(ctx.shape)
+
+
+__all__ = ["tv_iso", "tv_aniso", "inner_accuracy", "AccuracyWarning", "ConvergenceWarning", "Batch", "solve",
+           "FastproxError"]

@@ -173,6 +173,9 @@ FP_API const fpr_plugin_table *fastprox_rocm_plugin(void);
  * Extensions (no counterpart in the reference).
  * ---------------------------------------------------------------------- */
 
+/* A short description of why the last failing fpr_ call on this thread failed. */
+FP_API const char *fpr_last_error(void);
+
 /* The step kernel of a workspace: generic = 1 selects the one-iteration
  * kernel; otherwise the register-tile kernel with tt iterations per launch
  * and ry rows per tile (FP_EINVAL for a combination not built). The default
@@ -181,6 +184,45 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
 FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, int *tt, int *ry);
 /* Launch statistics of the last solve on ws: step launches, host syncs. */
 FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs);
+
+/* A batch of images of uneven sizes, each with its own step and, for a
+ * weighted batch, its own spatial weights, solved in one persistent solve
+ * (the same dual FISTA per image as fp_prox). Every pointer is a device
+ * pointer of the batch's dtype to contiguous C-order data. */
+typedef struct fpr_batch fpr_batch;
+typedef struct {
+    const void *x;        /* the image (shape from fpr_batch_create) */
+    void *z;              /* its output */
+    void *dual;           /* its dual: ndim planes of the image's size, in the
+                             form of fp_workspace_get_dual; read first when
+                             fpr_batch_solve is called with warm = 1 */
+    const void *weights;  /* weighted batches: pixel weights (iso, the
+                             image's size) or edge weights (aniso, ndim
+                             planes); else ignored */
+    double gamma;         /* its step */
+} fpr_image;
+/* shapes: count rows of op's ndim extents. The batch allocates all device
+ * memory its solves need; fpr_batch_solve allocates nothing. */
+FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count, const int64_t *shapes,
+                                  int weighted, fpr_batch **out);
+FP_API void      fpr_batch_free(fpr_batch *b);
+/* Solves every image on stream (NULL: the null stream) and waits for it.
+ * FP_ENOTCONV when an image stopped at max_iter; FP_EINVAL for an infinite
+ * entry; FP_EPARAM for a step that is negative, not finite or out of range. */
+FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp_options *opt, int warm,
+                                 void *stream);
+/* Per image, the results of the last solve (any pointer may be NULL). */
+FP_API fp_status fpr_batch_image_info(const fpr_batch *b, double *gaps, int *iters, double *zerrs,
+                                      int *converged, int64_t count);
+FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry);
+FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry);
+FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs);
+/* Kernel timing of the solves on b (HIP events on the solve's stream around
+ * every step and epilogue launch): on = 1 enables it for later solves; the
+ * last solve's summed step-kernel and epilogue milliseconds and the number of
+ * step launches timed. */
+FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on);
+FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches);
 
 #ifdef __cplusplus
 }
