@@ -73,7 +73,18 @@ def main():
     ap.add_argument("--kernel", default=None, help="'generic', 'stream:TT,SEG' or 'block:TT,RY'")
     ap.add_argument("--plain", type=int, default=8, help="images for the plain per-call rate (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="copy 1 GiB five times (the HBM counters' scale for the traffic figure)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 5 x 1 GiB (read 1 GiB, write 1 GiB per copy)")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (images are independent; "
               f"the multi-GPU form is data-parallel over images, see DESIGN.md)", file=sys.stderr)
@@ -131,7 +142,11 @@ def main():
     alg_bytes = float(np.sum(its.astype(np.float64)) * S * S * BYTES_PER_PIXEL_ITER[(2, 4)]) * args.steps
     step_s = step_ms_sum / 1e3
     achieved = alg_bytes / step_s / 1e9 if step_s > 0 else 0.0
-    kern = bt.stats()
+    traffic, traffic_src = None, None
+    prof = ROOT / "profiles" / "stream2d_iso_f32_512x512_b64_traffic.json"
+    if prof.exists() and kernel_name(bt) == "stream:TT=4,SEG=32" and B == 64 and S == 512:
+        rec = json.loads(prof.read_text())
+        traffic, traffic_src = rec["hbm_bytes_per_dispatch"], str(prof.relative_to(ROOT))
     log(f"batched: {value:.1f} images/s, {ms_per_step:.2f} ms per batch; iters mean {its.mean():.1f} "
         f"max {its.max()}; step kernels {step_ms_sum / args.steps:.2f} ms, epilogues {epi_ms_sum / args.steps:.2f}"
         f" ms per batch, {launches_sum / args.steps:.0f} launches; converged {bool(conv.all())}")
@@ -179,7 +194,7 @@ def main():
         "plain_images_per_s": plain,
         "speedup_vs_plain": (value / plain) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": {0: "stream2d_kernel", 1: "step1_kernel", 2: "step2d_kernel"}[bt_kernel(bt)[0]],
                      "step_kernel_ms_per_batch": step_ms_sum / args.steps,
                      "epilogue_ms_per_batch": epi_ms_sum / args.steps,

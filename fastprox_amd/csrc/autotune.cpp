@@ -47,8 +47,8 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
         }
     }
     kc.stream = true;
-    kc.TT = tsize == 4 ? 8 : 4;
-    kc.RY = 64;
+    kc.TT = 4;
+    kc.RY = 32;
     return kc;
 }
 

@@ -25,6 +25,7 @@ void Engine::release() {
     f(list_[1]);
     f(counts_);
     f(gam_);
+    f(propart_);
     if (hstate_) (void)hipHostFree(hstate_);
     if (hdesc_pinned_) (void)hipHostFree(hdesc_pinned_);
     if (hcount_) (void)hipHostFree(hcount_);
@@ -36,6 +37,7 @@ void Engine::release() {
     list_[0] = list_[1] = nullptr;
     counts_ = nullptr;
     gam_ = nullptr;
+    propart_ = nullptr;
     hstate_ = nullptr;
     hdesc_pinned_ = nullptr;
     hcount_ = nullptr;
@@ -93,6 +95,9 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
         if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)std::max(tiles, 1)))) return e;
     if ((e = hipMalloc((void **)&counts_, 64))) return e;
     if ((e = hipMalloc((void **)&gam_, sizeof(double) * (size_t)std::max(B, 1)))) return e;
+    if ((e = hipMalloc((void **)&propart_,
+                       sizeof(double) * 4 * (size_t)std::max(B, 1) * (size_t)prologue_blocks(std::max<int64_t>(maxN_, 1)))))
+        return e;
     if ((e = hipHostMalloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(B, 1), 0))) return e;
     if ((e = hipHostMalloc((void **)&hdesc_pinned_, sizeof(ImgDesc) * (size_t)std::max(B, 1), 0))) return e;
     if ((e = hipHostMalloc((void **)&hcount_, 64, 0))) return e;
@@ -103,11 +108,11 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
 template <class T>
 static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, void *arena,
                      ImgDesc *desc, ImgState *state, double *part, Tile **list,
-                     int *counts, double *gam, int B, int total_tiles, int64_t maxN, bool tv1d, int *hcount,
-                     int steps_per_sync, int max_iter) {
+                     int *counts, double *gam, double *propart, int B, int total_tiles, int64_t maxN, bool tv1d,
+                     int *hcount, int steps_per_sync, int max_iter) {
     const double u_round = 0.5 * (sizeof(T) == 4 ? (double)FLT_EPSILON : DBL_EPSILON);
     const double eta = 0.5 * (sizeof(T) == 4 ? (double)FLT_TRUE_MIN : DBL_TRUE_MIN);
-    launch_prologue<T>(P, desc, state, gam, B, s);
+    launch_prologue<T>(P, desc, state, gam, B, propart, maxN, s);
     launch_reset_counts(counts, s);
     launch_initial_list(desc, state, list[0], counts, B, s);
     if (P.warm) launch_warm<T>(P, (T *)arena, desc, state, B, maxN, s);
@@ -128,60 +133,84 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     const int64_t cap = 8LL * ((int64_t)max_iter / std::max(1, TT) + 8) + 64;
     int p = 0;
     int64_t launched = 0;
+    (void)0;
     hipError_t err = hipGetLastError();
     if (err) return err;
     // Optional timing: an event before and after every step and epilogue,
     // read back after each chunk of launches.
     const bool timing = E.timing;
     std::vector<hipEvent_t> ev;
+    hipEvent_t done_ev[2] = {nullptr, nullptr};
     if (timing) {
-        ev.resize((size_t)(3 * steps_per_sync));
+        ev.resize((size_t)(2 * 3 * steps_per_sync));
         for (auto &x : ev)
             if ((err = hipEventCreate(&x))) return err;
     }
+    for (auto &x : done_ev)
+        if ((err = hipEventCreateWithFlags(&x, hipEventDisableTiming))) return err;
     auto destroy = [&]() {
         for (auto &x : ev) (void)hipEventDestroy(x);
+        for (auto &x : done_ev)
+            if (x) (void)hipEventDestroy(x);
     };
-    for (;;) {
+    // Chunks of launches are pipelined: chunk c + 1 is enqueued before the
+    // host reads chunk c's count of live tiles, so the device never waits on
+    // the host; the chunk enqueued after the last one only meets empty lists.
+    auto enqueue = [&](int slot) -> hipError_t {
         for (int i = 0; i < steps_per_sync; ++i) {
             A.list_in = list[p];
             A.count_in = counts + p;
             A.count_out = counts + (p ^ 1);
-            if (timing) (void)hipEventRecord(ev[(size_t)(3 * i)], s);
+            const size_t eb = (size_t)(3 * (slot * steps_per_sync + i));
+            if (timing) (void)hipEventRecord(ev[eb], s);
             launch_step<T>(kc, P.iso != 0, P.sw != 0, A, grid, s);
-            if (timing) (void)hipEventRecord(ev[(size_t)(3 * i + 1)], s);
+            if (timing) (void)hipEventRecord(ev[eb + 1], s);
             launch_epilogue(desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s);
-            if (timing) (void)hipEventRecord(ev[(size_t)(3 * i + 2)], s);
+            if (timing) (void)hipEventRecord(ev[eb + 2], s);
             p ^= 1;
             ++launched;
         }
         E.stats.launches += steps_per_sync;
-        if ((err = hipMemcpyAsync(hcount, counts + p, sizeof(int), hipMemcpyDeviceToHost, s))) {
-            destroy();
-            return err;
+        hipError_t e2 = hipMemcpyAsync(hcount + slot, counts + p, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (!e2) e2 = hipEventRecord(done_ev[slot], s);
+        return e2;
+    };
+    auto read_timing = [&](int slot) {
+        if (!timing) return;
+        for (int i = 0; i < steps_per_sync; ++i) {
+            const size_t eb = (size_t)(3 * (slot * steps_per_sync + i));
+            float a = 0.f, b = 0.f;
+            (void)hipEventElapsedTime(&a, ev[eb], ev[eb + 1]);
+            (void)hipEventElapsedTime(&b, ev[eb + 1], ev[eb + 2]);
+            E.stats.step_ms += a;
+            E.stats.epi_ms += b;
+            ++E.stats.timed;
         }
-        if ((err = hipStreamSynchronize(s))) {
-            destroy();
-            return err;
-        }
+    };
+    int slot = 0;
+    if ((err = enqueue(slot))) {
+        destroy();
+        return err;
+    }
+    for (;;) {
+        if ((err = enqueue(slot ^ 1))) break;
+        if ((err = hipEventSynchronize(done_ev[slot]))) break;
         ++E.stats.host_syncs;
-        if (timing)
-            for (int i = 0; i < steps_per_sync; ++i) {
-                float a = 0.f, b = 0.f;
-                (void)hipEventElapsedTime(&a, ev[(size_t)(3 * i)], ev[(size_t)(3 * i + 1)]);
-                (void)hipEventElapsedTime(&b, ev[(size_t)(3 * i + 1)], ev[(size_t)(3 * i + 2)]);
-                E.stats.step_ms += a;
-                E.stats.epi_ms += b;
-                ++E.stats.timed;
-            }
-        if (*hcount == 0) break;
-        if (launched > cap) {
-            destroy();
-            return -1;  // the schedule did not finish: an internal error
+        read_timing(slot);
+        if (hcount[slot] == 0) {
+            // The chunk in flight finds no tiles; let it drain.
+            if ((err = hipEventSynchronize(done_ev[slot ^ 1]))) break;
+            read_timing(slot ^ 1);
+            break;
         }
+        if (launched > cap) {
+            err = (hipError_t)-1;  // the schedule did not finish: an internal error
+            break;
+        }
+        slot ^= 1;
     }
     destroy();
-    return hipSuccess;
+    return err == (hipError_t)-1 ? -1 : err;
 }
 
 int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol,
@@ -206,11 +235,11 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
     if ((e = hipMemcpyAsync(desc_, hdesc_pinned_, sizeof(ImgDesc) * (size_t)B, hipMemcpyHostToDevice, s))) return e;
     int r;
     if (tsize_ == 4)
-        r = run_solve<float>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, B, total_tiles_,
-                             maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
+        r = run_solve<float>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, propart_, B,
+                             total_tiles_, maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
     else
-        r = run_solve<double>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, B, total_tiles_,
-                              maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
+        r = run_solve<double>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, propart_, B,
+                              total_tiles_, maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
     if (r) return r;
     if ((e = hipMemcpyAsync(hstate_, state_, sizeof(ImgState) * (size_t)B, hipMemcpyDeviceToHost, s))) return e;
     return hipStreamSynchronize(s);

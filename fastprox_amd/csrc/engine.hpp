@@ -80,6 +80,7 @@ private:
     Tile *list_[2] = {nullptr, nullptr};
     int *counts_ = nullptr;
     double *gam_ = nullptr;
+    double *propart_ = nullptr;
     ImgDesc *hdesc_pinned_ = nullptr;  // pinned
     ImgState *hstate_ = nullptr;  // pinned
     int *hcount_ = nullptr;       // pinned

@@ -309,96 +309,192 @@ __device__ int rescale_exponent(double m, double gamma, double step) {  // image
     return e;
 }
 
+// The prologue's reductions run over many workgroups per image: pass p
+// (0: NaN, Inf and max |x|; 1: sum and sum of squares of x_s = 2^-e x;
+// 2: sum |x_s - mean| and sum (x_s / max)^2) writes one partial per block,
+// and a combine kernel (one thread per image) adds them in block order.
+constexpr int kProBlock = 256;
+constexpr int kProChunk = kProBlock * 16;  // entries per block
+
 template <class T>
-__global__ __launch_bounds__(256) void prologue_kernel(Problem P, const ImgDesc *desc,
-                                                       ImgState *state, const double *gammas, int TT) {
-    const int img = blockIdx.x;
-    const int tid = threadIdx.x;
+__global__ __launch_bounds__(kProBlock) void prologue_pass(const ImgDesc *desc, const ImgState *state,
+                                                           double *part, int nblk, int pass) {
+    const int img = blockIdx.y, blk = blockIdx.x, tid = threadIdx.x;
     const ImgDesc &D = desc[img];
-    ImgState &S = state[img];
-    const int k = P.k;
     const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const int64_t lo = (int64_t)blk * kProChunk;
+    if (lo >= N) return;
+    const int64_t hi = lo + kProChunk < N ? lo + kProChunk : N;
     const T *x = (const T *)D.d;
     __shared__ double sh[4];
-    __shared__ double shared_e, shared_mean;
-    __shared__ int flags;
-    if (tid == 0) flags = 0;
-    __syncthreads();
-    // Pass 1: NaN, Inf, max |x|.
+    double *out = part + ((int64_t)img * nblk + blk) * 4;
+    if (pass == 0) {
+        double m = 0.0;
+        int nan = 0, inf = 0;
+        for (int64_t i = lo + tid; i < hi; i += kProBlock) {
+            const double v = (double)x[i];
+            nan |= v != v;
+            inf |= isinf(v);
+            m = fmax(m, fabs(v));
+        }
+        __shared__ int flags;
+        if (tid == 0) flags = 0;
+        __syncthreads();
+        if (nan) atomicOr(&flags, 1);
+        if (inf) atomicOr(&flags, 2);
+        m = block_max(m, sh);
+        __syncthreads();
+        if (tid == 0) {
+            out[0] = m;
+            out[1] = (double)flags;
+        }
+        return;
+    }
+    const ImgState &S = state[img];
+    if (S.status || S.mode == kOutputNaN) return;
+    const int e = S.e;
+    double a = 0.0, b = 0.0;
+    if (pass == 1) {
+        for (int64_t i = lo + tid; i < hi; i += kProBlock) {
+            const double v = ldexp((double)x[i], -e);
+            a += v;
+            b += v * v;
+        }
+    } else {
+        const double mean = S.t, ms = S.maxabs;  // the mean is parked in t until the combine of pass 2
+        for (int64_t i = lo + tid; i < hi; i += kProBlock) {
+            const double v = ldexp((double)x[i], -e);
+            a += fabs(v - mean);
+            const double r = ms > 0.0 ? v / ms : 0.0;
+            b += r * r;
+        }
+    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (tid == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// Pass-0 combine: flags, max, the active axes and the rescaling exponent
+// (tv_iso.cpp:341-377).
+__global__ void prologue_combine0(Problem P, const ImgDesc *desc, ImgState *state, const double *part, int nblk,
+                                  const double *gammas, int nimg) {
+    const int img = blockIdx.x * blockDim.x + threadIdx.x;
+    if (img >= nimg) return;
+    const ImgDesc &D = desc[img];
+    ImgState &S = state[img];
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const int nb = (int)((N + kProChunk - 1) / kProChunk);
     double m = 0.0;
-    int nan = 0, inf = 0;
-    for (int64_t i = tid; i < N; i += 256) {
-        const double v = (double)x[i];
-        nan |= v != v;
-        inf |= isinf(v);
-        m = fmax(m, fabs(v));
+    int fl = 0;
+    for (int j = 0; j < nb; ++j) {
+        const double *q = part + ((int64_t)img * nblk + j) * 4;
+        m = fmax(m, q[0]);
+        fl |= (int)q[1];
     }
-    if (nan) atomicOr(&flags, 1);
-    if (inf) atomicOr(&flags, 2);
-    m = block_max(m, sh);
-    double amax = P.alpha_max;
-    if (P.sw && P.sw_per_image && P.iso) {  // the image's own largest pixel weight (the step scale)
-        double am = 0.0;
+    const int k = P.k;
+    const double gamma0 = gammas[img];
+    double L[3];
+    for (int a = 0; a < k; ++a) L[a] = (double)D.L[3 - k + a];
+    int e = 0;
+    if (P.iso && !fl) {
+        double wmaxL = 0.0, wa[3] = {0.0, 0.0, 0.0};
+        int na = 0;
+        for (int a = 0; a < k; ++a)
+            if (L[a] > 1) wmaxL = fmax(wmaxL, P.w[a]);
+        for (int a = 0; a < k; ++a)
+            if (gamma0 > 0.0 && P.w[a] > 0.0 && L[a] > 1 && P.w[a] >= 1e-100 * wmaxL) {
+                wa[a] = P.w[a];
+                ++na;
+            }
+        double ss = 0.0;
+        for (int a = 0; a < k; ++a) ss = fmax(ss, P.w[a]);
+        e = na ? rescale_exponent(m, gamma0, gamma0 * ss * S.sumsq_m) : 0;  // sumsq_m: the weight max, for now
+        for (int a = 0; a < k && e; ++a)
+            if (wa[a] > 0.0 && !(ldexp(gamma0 * wa[a], -e) > 0.0)) e = 0;
+    }
+    S.e = e;
+    S.status = (fl & 2) ? 1 : 0;
+    S.mode = (fl & 2) ? kDone : ((fl & 1) ? kOutputNaN : kIterate);
+    S.maxabs = ldexp(m, -e);
+}
+
+// The pixel weights' max per image (the step scale of the rescaling).
+template <class T>
+__global__ __launch_bounds__(256) void weight_max_kernel(Problem P, const ImgDesc *desc, ImgState *state, int nimg) {
+    const int img = blockIdx.x;
+    if (img >= nimg) return;
+    const ImgDesc &D = desc[img];
+    ImgState &S = state[img];
+    __shared__ double sh[4];
+    double am = 0.0;
+    if (P.sw && P.sw_per_image && P.iso) {
+        const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
         const T *al = (const T *)D.sw;
-        for (int64_t i = tid; i < N; i += 256) am = fmax(am, (double)al[i]);
-        amax = block_max(am, sh);
+        for (int64_t i = threadIdx.x; i < N; i += 256) am = fmax(am, (double)al[i]);
+        am = block_max(am, sh);
+    } else {
+        am = P.alpha_max;
     }
-    __syncthreads();
-    const int fl = flags;
-    // The image's axes (extents L[3-k..2]) and parameters.
+    if (threadIdx.x == 0) S.sumsq_m = am;
+}
+
+// Pass-1 combine: the mean (parked in t for pass 2) and |x_s|^2.
+__global__ void prologue_combine1(const ImgDesc *desc, ImgState *state, const double *part, int nblk, int nimg) {
+    const int img = blockIdx.x * blockDim.x + threadIdx.x;
+    if (img >= nimg) return;
+    const ImgDesc &D = desc[img];
+    ImgState &S = state[img];
+    if (S.status || S.mode == kOutputNaN) return;
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const int nb = (int)((N + kProChunk - 1) / kProChunk);
+    double a = 0.0, b = 0.0;
+    for (int j = 0; j < nb; ++j) {
+        const double *q = part + ((int64_t)img * nblk + j) * 4;
+        a += q[0];
+        b += q[1];
+    }
+    S.t = a / (double)N;
+    S.sumsq = b;
+}
+
+// Pass-2 combine: set the image up (tv_iso.cpp:333-415, 424-483;
+// _tv_torch.py:229-257) and plan its first launch.
+__global__ void prologue_combine2(Problem P, const ImgDesc *desc, ImgState *state, const double *part, int nblk,
+                                  const double *gammas, int nimg) {
+    const int img = blockIdx.x * blockDim.x + threadIdx.x;
+    if (img >= nimg) return;
+    const ImgDesc &D = desc[img];
+    ImgState &S = state[img];
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const int nb = (int)((N + kProChunk - 1) / kProChunk);
+    double s3 = 0.0, s4 = 0.0;
+    if (!S.status && S.mode != kOutputNaN)
+        for (int j = 0; j < nb; ++j) {
+            const double *q = part + ((int64_t)img * nblk + j) * 4;
+            s3 += q[0];
+            s4 += q[1];
+        }
+    const int k = P.k;
+    const int e = S.e;
     const double gamma0 = gammas[img];
     double wa[3] = {0.0, 0.0, 0.0}, L[3];
     for (int a = 0; a < k; ++a) L[a] = (double)D.L[3 - k + a];
-    int e = 0, na = 0, act = -1;
-    if (tid == 0) {
-        if (P.iso) {
-            double wmaxL = 0.0;
-            for (int a = 0; a < k; ++a)
-                if (L[a] > 1) wmaxL = fmax(wmaxL, P.w[a]);
-            for (int a = 0; a < k; ++a)
-                if (gamma0 > 0.0 && P.w[a] > 0.0 && L[a] > 1 && P.w[a] >= 1e-100 * wmaxL) {
-                    wa[a] = P.w[a];
-                    act = a;
-                    ++na;
-                }
-            double ss = 0.0;
-            for (int a = 0; a < k; ++a) ss = fmax(ss, P.w[a]);
-            e = na ? rescale_exponent(m, gamma0, gamma0 * ss * amax) : 0;
-            for (int a = 0; a < k && e; ++a)
-                if (wa[a] > 0.0 && !(ldexp(gamma0 * wa[a], -e) > 0.0)) e = 0;
-        }
-        if (fl) e = 0;
-        shared_e = (double)e;
+    int na = 0, act = -1;
+    if (P.iso) {
+        double wmaxL = 0.0;
+        for (int a = 0; a < k; ++a)
+            if (L[a] > 1) wmaxL = fmax(wmaxL, P.w[a]);
+        for (int a = 0; a < k; ++a)
+            if (gamma0 > 0.0 && P.w[a] > 0.0 && L[a] > 1 && P.w[a] >= 1e-100 * wmaxL) {
+                wa[a] = P.w[a];
+                act = a;
+                ++na;
+            }
     }
-    __syncthreads();
-    e = (int)shared_e;
-    // Pass 2: sum and sum of squares of x_s = 2^-e x.
-    double s1 = 0.0, s2 = 0.0;
-    if (!fl)
-        for (int64_t i = tid; i < N; i += 256) {
-            const double v = ldexp((double)x[i], -e);
-            s1 += v;
-            s2 += v * v;
-        }
-    s1 = block_sum(s1, sh);
-    s2 = block_sum(s2, sh);
-    if (tid == 0) shared_mean = s1 / (double)N;
-    __syncthreads();
-    const double mean = shared_mean, ms = ldexp(m, -e);
-    // Pass 3: the imbalance sum |x_s - mean| and sum (x_s / max)^2.
-    double s3 = 0.0, s4 = 0.0;
-    if (!fl)
-        for (int64_t i = tid; i < N; i += 256) {
-            const double v = ldexp((double)x[i], -e);
-            s3 += fabs(v - mean);
-            const double r = ms > 0.0 ? v / ms : 0.0;
-            s4 += r * r;
-        }
-    s3 = block_sum(s3, sh);
-    s4 = block_sum(s4, sh);
-    if (tid != 0) return;
-    // Set the image up (tv_iso.cpp:333-415, 424-483; _tv_torch.py:229-257).
-    S.e = e;
+    const double s2 = S.sumsq;
     S.it = 0;
     S.since = 0;
     S.last = 0;
@@ -413,11 +509,8 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const ImgDesc 
     S.sched = P.sched;
     S.max_iter = P.max_iter;
     S.converged = 1;
-    S.status = 0;
     S.t = 1.0;
     S.gp = INFINITY;
-    S.maxabs = ms;
-    S.sumsq = s2;
     S.sumsq_m = s4;
     S.zerr = 0.0;
     S.bound = INFINITY;
@@ -427,13 +520,11 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const ImgDesc 
     S.active_axis = -1;
     S.tv1d_lam = 0.0;
     for (int a = 0; a < 3; ++a) S.wa[a] = 0.0;
-    if (fl & 2) {  // an infinite entry: the call fails (FP_EINVAL)
-        S.status = 1;
+    if (S.status) {  // an infinite entry: the call fails (FP_EINVAL)
         S.mode = kDone;
         return;
     }
-    if (fl & 1) {  // not solved: NaN outputs, gap NaN, 0 iterations
-        S.mode = kOutputNaN;
+    if (S.mode == kOutputNaN) {  // not solved: NaN outputs, gap NaN, 0 iterations
         S.gp = NAN;
         return;
     }
@@ -488,7 +579,7 @@ __global__ __launch_bounds__(256) void prologue_kernel(Problem P, const ImgDesc 
     }
     S.mode = kIterate;
     S.fresh = P.warm ? 0 : 1;
-    plan_next(S, TT);
+    plan_next(S, P.TT);
 }
 
 // ---------------------------------------------------------------------------
@@ -950,15 +1041,38 @@ __device__ __forceinline__ T from_right(T v) {  // lane i receives lane i + 1's 
     }
 }
 
-constexpr int kAcc = 6;  // partial-sum slots per tile (flagged levels of one launch)
-
-__device__ __forceinline__ void acc_add(double (&a)[kAcc], int slot, double v) {
-#pragma unroll
-    for (int i = 0; i < kAcc; ++i) a[i] += (i == slot) ? v : 0.0;
+// The projection factor min(1, r / |c|) of the isotropic ball. float64
+// follows the reference's correctly rounded sqrt and division
+// (tv_iso.cpp:509-510); float32 uses the hardware's square root and
+// reciprocal (about 1 ulp each), its iterate being float32 anyway.
+template <class T>
+__device__ __forceinline__ T ball_scale(T nn, T rp) {
+    if constexpr (sizeof(T) == 4) {
+        const float nr = __builtin_amdgcn_sqrtf(nn);
+        return nr > rp ? rp * __builtin_amdgcn_rcpf(nr) : 1.0f;
+    } else {
+        const T nr = sqrt(nn);
+        return nr > rp ? rp / nr : (T)1;
+    }
 }
 
+// a * b + c: one fused rounding in float32; in float64 the reference's two
+// roundings (its builds do not contract, CMakeLists.txt:102-114), for its bits.
+template <class T>
+__device__ __forceinline__ T madd(T a, T b, T c) {
+    if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+    else return a * b + c;
+}
+
+// Per-wave partial sums in LDS: [gap | dot][level][lane], FP64.
+template <int TT>
+struct StreamAcc {
+    double v[2][TT + 1][64];
+};
+
 template <class T, bool ISO, bool SW, int TT>
-__device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl, const int lane, const int SEG) {
+__device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl, const int lane, const int SEG,
+                                            StreamAcc<TT> &acc) {
     constexpr int TXI = 64 - 2 * TT;
     const int img = __builtin_amdgcn_readfirstlane(tl.img);
     const int tx = __builtin_amdgcn_readfirstlane(tl.tx);
@@ -972,6 +1086,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     const int x = tx * TXI - TT + lane;
     const bool xin = x >= 0 && x < W;
     const bool lane_in = lane >= TT && lane < 64 - TT && xin;
+    const bool xnext = xin && x + 1 < W;
     const int y0 = ty * SEG;
     const int yend = min(y0 + SEG, H);  // rows [y0, yend) are this tile's
     double *part = A.part + (int64_t)(D.tile_base + tloc) * kSlots;
@@ -998,28 +1113,24 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     const T w0 = (T)w0d, w1 = (T)w1d;
     const bool act0 = w0d != 0.0, act1 = w1d != 0.0;
     const bool out_mode = mode == kOutput;
-    T beta[TT + 1];
-    int gslot[TT + 1], dslot[TT + 1];
-    {
-        int ns = 0;
-#pragma unroll
-        for (int m = 0; m <= TT; ++m) {
-            beta[m] = m >= 1 ? (T)S.beta[m] : (T)0;
-            gslot[m] = ((gm >> m) & 1u) ? ns++ : -1;
-        }
-#pragma unroll
-        for (int m = 0; m <= TT; ++m) dslot[m] = ((tm >> m) & 1u) ? ns++ : -1;
-    }
     const T *dp = (const T *)D.d;
     const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
     const T *V = S.vsrc_u ? U : U + 2 * N;
     T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N;
     const T *sw = SW ? (const T *)D.sw : nullptr;
+    T beta[TT + 1];
+#pragma unroll
+    for (int m = 1; m <= TT; ++m) beta[m] = (T)S.beta[m];
+#pragma unroll
+    for (int m = 0; m <= TT; ++m) {
+        if ((gm >> m) & 1u) acc.v[0][m][lane] = 0.0;
+        if ((tm >> m) & 1u) acc.v[1][m][lane] = 0.0;
+    }
 
     // Per level m < TT: its latest row's v, u, z(v) and z(u) (FP64).
     T ov0[TT], ov1[TT], ou0[TT], ou1[TT], oz[TT];
     double ozu[TT];
-    T dw[TT + 2];                                  // dw[m] = x_s at row ys - m
+    T dw[TT + 2];                                          // dw[m] = x_s at row ys - m
     T ra[SW ? TT + 2 : 1], rb[(SW && !ISO) ? TT + 2 : 1];  // radii at row ys - m
 #pragma unroll
     for (int m = 0; m < TT; ++m) {
@@ -1033,11 +1144,33 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
         if constexpr (SW && !ISO) rb[m] = (T)0;
     }
     T uf0 = (T)0;  // the final level's previous row, u0
-    double acc[kAcc];
-#pragma unroll
-    for (int i = 0; i < kAcc; ++i) acc[i] = 0.0;
+
+    // The gap term of one pixel (tv_iso.cpp:290-306; _tv_torch.py:179-186).
+    auto gap_term = [&](double g0, double g1, double u0, double u1, double r0, double r1) -> double {
+        if constexpr (ISO) return r0 * sqrt(g0 * g0 + g1 * g1) - (g0 * u0 + g1 * u1);
+        else return (r0 * fabs(g0) - g0 * u0) + (r1 * fabs(g1) - g1 * u1);
+    };
 
     const int ystart = y0 - TT, ylast = yend - 1 + TT;
+    // Row ys + 1's input, loaded one step ahead.
+    T qd = (T)0, qu0 = (T)0, qu1 = (T)0, qv0 = (T)0, qv1 = (T)0, qa = (T)0, qb = (T)0;
+    auto load_row = [&](int ys) {
+        const bool in = xin && ys >= 0 && ys < H;
+        const int64_t o = in ? (int64_t)ys * W + x : 0;
+        qd = in ? dp[o] : (T)0;
+        qu0 = qu1 = qv0 = qv1 = (T)0;
+        if (!fresh && in) {
+            qu0 = U[o];
+            qu1 = U[N + o];
+            qv0 = V[o];
+            qv1 = V[N + o];
+        }
+        if constexpr (SW) {
+            qa = in ? sw[o] : (T)0;
+            if constexpr (!ISO) qb = in ? sw[D.sw_cstride + o] : (T)0;
+        }
+    };
+    load_row(ystart);
     for (int ys = ystart; ys <= ylast; ++ys) {
 #pragma unroll
         for (int m = TT + 1; m >= 1; --m) {
@@ -1046,43 +1179,29 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             if constexpr (SW && !ISO) rb[m] = rb[m - 1];
         }
         // Level 0: the new row.
-        const bool in = xin && ys >= 0 && ys < H;
-        const int64_t o = in ? (int64_t)ys * W + x : 0;
-        T nv0 = (T)0, nv1 = (T)0, nu0 = (T)0, nu1 = (T)0;
-        dw[0] = in ? scale_in(dp[o], e) : (T)0;
-        if (!fresh && in) {
-            nu0 = U[o];
-            nu1 = U[N + o];
-            nv0 = V[o];
-            nv1 = V[N + o];
-        }
+        const T nv0 = qv0, nv1 = qv1, nu0 = qu0, nu1 = qu1;
+        dw[0] = scale_in(qd, e);
         if constexpr (SW) {
             if constexpr (ISO) {
-                ra[0] = in ? (T)iso_radius(gamd, (double)sw[o], boundd) : (T)0;
+                ra[0] = (T)iso_radius(gamd, (double)qa, boundd);
             } else {
-                ra[0] = in ? (T)(gamd * (double)sw[o]) : (T)0;
-                rb[0] = in ? (T)(gamd * (double)sw[D.sw_cstride + o]) : (T)0;
+                ra[0] = (T)(gamd * (double)qa);
+                rb[0] = (T)(gamd * (double)qb);
             }
         }
-        T zn = (dw[0] + w0 * (nv0 - ov0[0])) + w1 * (nv1 - from_left(nv1));
+        if (ys < ylast) load_row(ys + 1);
+        T zn = madd(w1, nv1 - from_left(nv1), madd(w0, nv0 - ov0[0], dw[0]));
         double zun = 0.0;
         // The gap of level 0 (the loaded state) on row ys - 1.
-        if (gslot[0] >= 0) {
+        if (gm & 1u) {
             zun = ((double)dw[0] + w0d * ((double)nu0 - (double)ou0[0])) + w1d * ((double)nu1 - from_left((double)nu1));
             const int r = ys - 1;
             const double zc = ozu[0], zr = from_right(zc);
-            const bool cnt_r = lane_in && r >= y0 && r < yend;
             const double g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0d * (zun - zc) : 0.0;
-            const double g1 = (act1 && xin && r >= 0 && r < H && x + 1 < W) ? w1d * (zr - zc) : 0.0;
-            double gp;
-            if constexpr (ISO) {
-                const double rp = SW ? (double)ra[1] : gamd;
-                gp = rp * sqrt(g0 * g0 + g1 * g1) - (g0 * (double)ou0[0] + g1 * (double)ou1[0]);
-            } else {
-                const double r0 = SW ? (double)ra[1] : gamd, r1 = SW ? (double)rb[1] : gamd;
-                gp = (r0 * fabs(g0) - g0 * (double)ou0[0]) + (r1 * fabs(g1) - g1 * (double)ou1[0]);
-            }
-            acc_add(acc, gslot[0], cnt_r ? gp : 0.0);
+            const double g1 = (act1 && xnext && r >= 0 && r < H) ? w1d * (zr - zc) : 0.0;
+            const double gp = gap_term(g0, g1, (double)ou0[0], (double)ou1[0], SW ? (double)ra[1] : gamd,
+                                       SW && !ISO ? (double)rb[1] : gamd);
+            if (lane_in && r >= y0 && r < yend) acc.v[0][0][lane] += gp;
         }
         // The new row's values rise through the levels.
         T pv0 = nv0, pv1 = nv1, pu0 = nu0, pu1 = nu1;  // level m - 1's new row
@@ -1095,17 +1214,15 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             const T zr = from_right(zc);
             T v0n, v1n, u0n, u1n;
             if (m <= cnt) {
-                const bool rin = r >= 0 && r < H;
+                // An inactive axis has w = 0 and v = 0 (cold start, or zeroed by the
+                // warm start), so its c is 0 as the reference's select makes it.
                 const T g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0 * (pz - zc) : (T)0;
-                const T g1 = (act1 && xin && rin && x + 1 < W) ? w1 * (zr - zc) : (T)0;
-                const T c0 = act0 ? ov0[m - 1] + tau * g0 : (T)0;
-                const T c1 = act1 ? ov1[m - 1] + tau * g1 : (T)0;
+                const T g1 = (act1 && xnext && r >= 0 && r < H) ? w1 * (zr - zc) : (T)0;
+                const T c0 = madd(tau, g0, ov0[m - 1]);
+                const T c1 = madd(tau, g1, ov1[m - 1]);
                 T n0, n1;
                 if constexpr (ISO) {
-                    const T nn = c0 * c0 + c1 * c1;
-                    const T nr = sqrt(nn);
-                    const T rp = SW ? ra[m] : gam;
-                    const T sc = nr > rp ? rp / nr : (T)1;
+                    const T sc = ball_scale(madd(c1, c1, c0 * c0), SW ? ra[m] : gam);
                     n0 = c0 * sc;
                     n1 = c1 * sc;
                 } else {
@@ -1113,14 +1230,13 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                     n0 = c0 > r0 ? r0 : (c0 < -r0 ? -r0 : c0);
                     n1 = c1 > r1 ? r1 : (c1 < -r1 ? -r1 : c1);
                 }
-                if (dslot[m] >= 0) {
-                    const bool cnt_r = lane_in && r >= y0 && r < yend;
+                if ((tm >> m) & 1u) {
                     const double dd = ((double)ov0[m - 1] - (double)n0) * ((double)n0 - (double)ou0[m - 1]) +
                                       ((double)ov1[m - 1] - (double)n1) * ((double)n1 - (double)ou1[m - 1]);
-                    acc_add(acc, dslot[m], cnt_r ? dd : 0.0);
+                    if (lane_in && r >= y0 && r < yend) acc.v[1][m][lane] += dd;
                 }
-                v0n = n0 + beta[m] * (n0 - ou0[m - 1]);
-                v1n = n1 + beta[m] * (n1 - ou1[m - 1]);
+                v0n = madd(beta[m], n0 - ou0[m - 1], n0);
+                v1n = madd(beta[m], n1 - ou1[m - 1], n1);
                 u0n = n0;
                 u1n = n1;
             } else {
@@ -1137,25 +1253,18 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             oz[m - 1] = pz;
             ozu[m - 1] = pzu;
             if (m < TT) {
-                if (m < cnt) pz = (dw[m] + w0 * (v0n - ov0[m])) + w1 * (v1n - from_left(v1n));
-                if (gslot[m] >= 0) {
+                if (m < cnt) pz = madd(w1, v1n - from_left(v1n), madd(w0, v0n - ov0[m], dw[m]));
+                if ((gm >> m) & 1u) {
                     // The gap of level m on row r - 1.
                     pzu = ((double)dw[m] + w0d * ((double)u0n - (double)ou0[m])) +
                           w1d * ((double)u1n - from_left((double)u1n));
                     const int rg = r - 1;
                     const double zc2 = ozu[m], zr2 = from_right(zc2);
-                    const bool cnt_r = lane_in && rg >= y0 && rg < yend;
                     const double g0 = (act0 && xin && rg >= 0 && rg + 1 < H) ? w0d * (pzu - zc2) : 0.0;
-                    const double g1 = (act1 && xin && rg >= 0 && rg < H && x + 1 < W) ? w1d * (zr2 - zc2) : 0.0;
-                    double gp;
-                    if constexpr (ISO) {
-                        const double rp = SW ? (double)ra[m + 1] : gamd;
-                        gp = rp * sqrt(g0 * g0 + g1 * g1) - (g0 * (double)ou0[m] + g1 * (double)ou1[m]);
-                    } else {
-                        const double r0 = SW ? (double)ra[m + 1] : gamd, r1 = SW ? (double)rb[m + 1] : gamd;
-                        gp = (r0 * fabs(g0) - g0 * (double)ou0[m]) + (r1 * fabs(g1) - g1 * (double)ou1[m]);
-                    }
-                    acc_add(acc, gslot[m], cnt_r ? gp : 0.0);
+                    const double g1 = (act1 && xnext && rg >= 0 && rg < H) ? w1d * (zr2 - zc2) : 0.0;
+                    const double gp = gap_term(g0, g1, (double)ou0[m], (double)ou1[m],
+                                               SW ? (double)ra[m + 1] : gamd, SW && !ISO ? (double)rb[m + 1] : gamd);
+                    if (lane_in && rg >= y0 && rg < yend) acc.v[0][m][lane] += gp;
                 }
                 pv0 = v0n;
                 pv1 = v1n;
@@ -1192,18 +1301,12 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     // The tile's partial sums of the flagged levels.
 #pragma unroll
     for (int m = 0; m <= TT; ++m) {
-        if (gslot[m] >= 0) {
-            double v = 0.0;
-#pragma unroll
-            for (int i = 0; i < kAcc; ++i) v += (i == gslot[m]) ? acc[i] : 0.0;
-            v = wave_sum(v);
+        if ((gm >> m) & 1u) {
+            const double v = wave_sum(acc.v[0][m][lane]);
             if (lane == 0) part[2 * m] = v;
         }
-        if (dslot[m] >= 0) {
-            double v = 0.0;
-#pragma unroll
-            for (int i = 0; i < kAcc; ++i) v += (i == dslot[m]) ? acc[i] : 0.0;
-            v = wave_sum(v);
+        if ((tm >> m) & 1u) {
+            const double v = wave_sum(acc.v[1][m][lane]);
             if (lane == 0) part[2 * m + 1] = v;
         }
     }
@@ -1211,6 +1314,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
 
 template <class T, bool ISO, bool SW, int TT>
 __global__ __launch_bounds__(256) void stream2d_kernel(StepArgs<T> A, int SEG) {
+    __shared__ StreamAcc<TT> accs[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;
     const int ntiles = *A.count_in;
@@ -1220,7 +1324,7 @@ __global__ __launch_bounds__(256) void stream2d_kernel(StepArgs<T> A, int SEG) {
     const int cend = min((xcd + 1) * cpx, nchunks);
     for (int c = xcd * cpx + j8; c < cend; c += per) {
         const int ti = c * 4 + wv;
-        if (ti < ntiles) stream_tile<T, ISO, SW, TT>(A, A.list_in[ti], lane, SEG);
+        if (ti < ntiles) stream_tile<T, ISO, SW, TT>(A, A.list_in[ti], lane, SEG, accs[wv]);
     }
 }
 
@@ -1487,10 +1591,21 @@ static int grid_for(int64_t n, int block, int cap = 65535) {
     return (int)(g > cap ? cap : g);
 }
 
+int prologue_blocks(int64_t maxN) { return (int)((maxN + kProChunk - 1) / kProChunk); }
+
 template <class T>
 void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(prologue_kernel<T>, dim3(nimg), dim3(256), 0, s, P, desc, state, gammas, P.TT);
+                     double *part, int64_t maxN, hipStream_t s) {
+    const int nblk = prologue_blocks(maxN);
+    const dim3 grid(nblk, nimg);
+    const int g1 = grid_for(nimg, 64);
+    hipLaunchKernelGGL(weight_max_kernel<T>, dim3(nimg), dim3(256), 0, s, P, desc, state, nimg);
+    hipLaunchKernelGGL(prologue_pass<T>, grid, dim3(kProBlock), 0, s, desc, state, part, nblk, 0);
+    hipLaunchKernelGGL(prologue_combine0, dim3(g1), dim3(64), 0, s, P, desc, state, part, nblk, gammas, nimg);
+    hipLaunchKernelGGL(prologue_pass<T>, grid, dim3(kProBlock), 0, s, desc, state, part, nblk, 1);
+    hipLaunchKernelGGL(prologue_combine1, dim3(g1), dim3(64), 0, s, desc, state, part, nblk, nimg);
+    hipLaunchKernelGGL(prologue_pass<T>, grid, dim3(kProBlock), 0, s, desc, state, part, nblk, 2);
+    hipLaunchKernelGGL(prologue_combine2, dim3(g1), dim3(64), 0, s, P, desc, state, part, nblk, gammas, nimg);
 }
 
 void launch_reset_counts(int *counts, hipStream_t s) {
@@ -1592,7 +1707,8 @@ void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
 }
 
 #define FPR_INST(T)                                                                                              \
-    template void launch_prologue<T>(const Problem &, const ImgDesc *, ImgState *, const double *, int, hipStream_t);                                                              \
+    template void launch_prologue<T>(const Problem &, const ImgDesc *, ImgState *, const double *, int, double *, \
+                                     int64_t, hipStream_t);                                                              \
     template void launch_warm<T>(const Problem &, T *, const ImgDesc *, const ImgState *, int, int64_t, hipStream_t);                                                         \
     template void launch_tv1d<T>(const Problem &, const ImgDesc *, ImgState *, int, int64_t, double, double,      \
                                  hipStream_t);                                                  \

@@ -33,9 +33,11 @@ struct KernelChoice {
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY);
 
+// The prologue's per-block partials: 4 doubles per block, prologue_blocks(maxN) blocks per image.
+int prologue_blocks(int64_t maxN);
 template <class T>
 void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
-                     hipStream_t s);
+                     double *part, int64_t maxN, hipStream_t s);
 void launch_reset_counts(int *counts, hipStream_t s);
 void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list, int *count, int nimg,
                          hipStream_t s);

@@ -54,7 +54,11 @@ def test_warm_start_round_trip(cuda):
     x = torch.tensor(image((30, 40), 8), device=cuda)
     z1, i1 = tv_iso(x, 0.2, tol=1e-5, max_iter=100000, return_info=True)
     z2, i2 = tv_iso(x, 0.2, tol=1e-5, max_iter=100000, warm_start=i1.dual, return_info=True)
-    assert i2.iters == 0 and torch.equal(z1, z2)
+    # The seed is projected again (tv_iso.cpp:456-477), as the reference does.
+    r = oracle.tv_iso(x.cpu().numpy(), 0.2, tol=1e-5, max_iter=100000, warm=i1.dual.cpu().numpy())
+    assert i2.iters == r.iters == 0
+    np.testing.assert_array_equal(z2.cpu().numpy(), r.z)
+    assert (z1 - z2).abs().max() < 1e-12
     z3, i3 = tv_aniso(x, 0.2, tol=1e-5, max_iter=100000, return_info=True)
     z4, i4 = tv_aniso(x, 0.2, tol=1e-5, max_iter=100000, warm_start=i3.dual, return_info=True)
     assert i4.iters <= 4
