@@ -39,6 +39,12 @@ def lib():
         L.or_iso_gap.argtypes = [C.c_int, i64p, dp, dp, C.c_double, dp, dp]
         L.or_aniso_gap.restype = C.c_double
         L.or_aniso_gap.argtypes = [C.c_int, i64p, dp, dp, C.c_double, dp, dp]
+        L.or_iso_deriv.restype = C.c_int
+        L.or_iso_deriv.argtypes = [C.c_int, i64p, dp, dp, C.c_double, dp, dp, dp, C.c_double, C.c_int, C.c_int,
+                                   dp, C.c_double, dp, C.c_double, C.c_int, dp, dp, dp, ip, i64p]
+        L.or_aniso_deriv.restype = C.c_int
+        L.or_aniso_deriv.argtypes = [C.c_int, i64p, dp, dp, C.c_double, dp, dp, dp, C.c_double, C.c_double,
+                                     C.c_int, dp, C.c_double, dp, dp, dp, dp, i64p]
         L.or_zerr.restype = C.c_double
         L.or_zerr.argtypes = [C.c_double, dp, C.c_int64, C.c_double, C.c_double, C.c_int, C.c_double]
         _lib = L
@@ -129,3 +135,67 @@ def zerr(gap, x, dtype, scale_exp=0, scaled_gap=0.0):
     u = 0.5 * float(f.eps)
     eta = 0.5 * float(f.smallest_subnormal)
     return lib().or_zerr(float(gap), _p(x), x.size, u, eta, int(scale_exp), float(scaled_gap))
+
+
+class Deriv:
+    """A derivative's result: out (gx or dz), the parameter gradients (vjp),
+    the CG residual and iterations, and the number of regions."""
+
+    def __init__(self, out, gw=None, gspatial=None, ggamma=None, residual=0.0, iters=0, regions=0):
+        self.out, self.gw, self.gspatial, self.ggamma = out, gw, gspatial, ggamma
+        self.residual, self.iters, self.regions = residual, iters, regions
+
+
+def iso_deriv(x, z, u, gap, scale_exp, gamma, w=None, alpha=None, *, g=None, dx=None, dgamma=0.0, dparams=None,
+              deriv_tol=0.0, deriv_max_iter=10000):
+    """The isotropic derivative at a solve's output (z, dual u, gap, scale
+    exponent) of one image x: with g, the VJP (out = J^T g; gw, gspatial,
+    ggamma the parameter gradients); else the JVP along (dx, dgamma,
+    dparams)."""
+    x, z, u = _c(x), _c(z), _c(u)
+    k = x.ndim
+    w = np.ones(k) if w is None else _c(np.broadcast_to(np.asarray(w, np.float64), (k,)))
+    alpha = _c(alpha)
+    shape = (C.c_int64 * k)(*x.shape)
+    out = np.empty_like(x)
+    P = k + (x.size if alpha is not None else 0)
+    acc = np.zeros(P + 1)
+    res, its, nr = C.c_double(), C.c_int(), C.c_int64()
+    mode = 0 if g is not None else 1
+    inp = _c(g) if g is not None else _c(dx)
+    dp = None if dparams is None else _c(np.asarray(dparams, np.float64).reshape(-1))
+    s = lib().or_iso_deriv(k, shape, _p(w), _p(alpha), float(gamma), _p(x), _p(z), _p(u), float(gap),
+                           int(scale_exp), mode, _p(inp), float(dgamma), _p(dp), float(deriv_tol),
+                           int(deriv_max_iter), _p(out), _p(acc), C.byref(res), C.byref(its), C.byref(nr))
+    if s:
+        raise MemoryError
+    if mode == 0:
+        return Deriv(out, acc[:k].copy(), acc[k:P].reshape(x.shape).copy() if alpha is not None else None,
+                     float(acc[P]), res.value, its.value, nr.value)
+    return Deriv(out, residual=res.value, iters=its.value, regions=nr.value)
+
+
+def aniso_deriv(x, z, u, gamma, w=None, beta=None, dtype=np.float64, *, g=None, dx=None, dgamma=0.0, dw=None,
+                dbeta=None):
+    """The device method's anisotropic derivative at (z, per-edge dual u (k,
+    *img)) of one image x: with g, the VJP; else the JVP."""
+    x, z, u = _c(x), _c(z), _c(u)
+    k = x.ndim
+    w = np.ones(k) if w is None else _c(np.broadcast_to(np.asarray(w, np.float64), (k,)))
+    beta = _c(beta)
+    eps = float(np.finfo(dtype).eps)
+    eta = max(1e-9, 64 * eps)
+    shape = (C.c_int64 * k)(*x.shape)
+    out = np.empty_like(x)
+    acc = np.zeros(k + 1 + (k * x.size if beta is not None else 0))
+    nr = C.c_int64()
+    mode = 0 if g is not None else 1
+    inp = _c(g) if g is not None else _c(dx)
+    s = lib().or_aniso_deriv(k, shape, _p(w), _p(beta), float(gamma), _p(x), _p(z), _p(u), eta, eps, mode,
+                             _p(inp), float(dgamma), _p(_c(dw)), _p(_c(dbeta)), _p(out), _p(acc), C.byref(nr))
+    if s:
+        raise MemoryError
+    if mode == 0:
+        return Deriv(out, acc[:k].copy(), acc[k + 1:].reshape((k,) + x.shape).copy() if beta is not None else None,
+                     float(acc[k]), regions=nr.value)
+    return Deriv(out, regions=nr.value)

@@ -1,5 +1,7 @@
 """Writes tests/golden/golden_tv.npz: inputs and the reference library's
-outputs for the image-TV path, and tests/golden/cvxpy_images.npz: the
+outputs for the image-TV path, tests/golden/golden_deriv.npz: the reference
+library's isotropic JVP and VJP at kept solves, and
+tests/golden/cvxpy_images.npz: the
 reference's own CVXPY+Clarabel image fixtures (tv_iso and tv_aniso cases of
 tests/fixtures/reference_images.npz, copied as data).
 
@@ -13,6 +15,7 @@ from __future__ import annotations
 
 import pathlib
 import sys
+import zlib
 
 import numpy as np
 
@@ -74,7 +77,49 @@ def cases():
     return c
 
 
+def deriv_cases():
+    """(name, x, gamma, w, pixel weights, tol): isotropic derivatives."""
+    rng = np.random.default_rng(40)
+    yy, xx = np.mgrid[0:64, 0:64] / 64
+    audit = ((xx - 0.5) ** 2 + (yy - 0.5) ** 2 < 0.1) + 0.5 * (xx > 0.7)
+    audit = audit + 0.1 * np.random.default_rng(0).standard_normal((64, 64))
+    return [
+        ("d_audit", audit, 0.2, [1.0, 1.0], None, 1e-9),
+        ("d_audit_w", audit, 0.2, [1.0, 0.6], np.random.default_rng(2).uniform(0.5, 1.5, audit.shape), 1e-9),
+        ("d_72x70", image((72, 70), 41), 0.15, [1.0, 1.0], None, 1e-8),
+        ("d_3d", image((5, 16, 14), 42), 0.3, [1.0, 0.7, 1.2], None, 1e-8),
+        ("d_pixel", image((40, 44), 43), 0.1, [0.8, 1.0], rng.uniform(0.5, 1.5, (40, 44)), 1e-7),
+        ("d_loose", image((48, 40), 44), 0.1, [1.0, 1.0], None, 1e-5),
+        ("d_scaled", image((30, 31), 45) * 2.0 ** 80, 0.2 * 2.0 ** 80, [1.0, 1.0], None, 1e-9),
+        ("d_oneaxis", image((17, 23), 46), 0.3, [1.0, 0.0], None, 1e-9),
+    ]
+
+
+def write_deriv():
+    out = {}
+    for name, x, gamma, w, al, tol in deriv_cases():
+        k = x.ndim
+        op = refbind.Op(True, k, w, spatial=al)
+        z, dual, gaps, its, zerrs, status, ws = op.solve(x, gamma, tol=tol, max_iter=10 ** 6, keep=True)
+        rng = np.random.default_rng(zlib.crc32(name.encode()))
+        P = k + (x.size if al is not None else 0)
+        g, dx, dp = rng.standard_normal(x.shape), rng.standard_normal(x.shape), rng.standard_normal(P)
+        gx, gg, gp = op.vjp(ws, g)
+        jv = op.jvp(ws, dx, 0.3, dp)
+        refbind.Op.free_ws(ws)
+        for key, val in (("x", x), ("gamma", np.array(gamma)), ("w", np.asarray(w, np.float64)), ("tol", np.array(tol)),
+                         ("z", z), ("dual", dual), ("gap", np.array(gaps[0])), ("iters", np.array(its[0])),
+                         ("g", g), ("dx", dx), ("dgamma", np.array(0.3)), ("dparams", dp), ("gx", gx),
+                         ("ggamma", np.array(gg)), ("gparams", gp), ("jvp", jv)):
+            out[f"{name}/{key}"] = val
+        if al is not None:
+            out[f"{name}/alpha"] = al
+        print(f"{name}: iters {its[0]} gap {gaps[0]:.3e}")
+    np.savez_compressed(HERE / "golden_deriv.npz", **out)
+
+
 def main():
+    write_deriv()
     out = {}
     for name, iso, x, gamma, w, sp, tol, max_iter, warm_tol in cases():
         k = len(w)
