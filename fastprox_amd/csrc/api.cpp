@@ -23,6 +23,7 @@
 #include "autotune.hpp"
 #include "deriv.hpp"
 #include "engine.hpp"
+#include "ws.hpp"
 
 #ifndef FASTPROX_ROCM_BUILD
 #define FASTPROX_ROCM_BUILD "1.0.6"
@@ -32,51 +33,6 @@
 // Operators and workspaces
 // ---------------------------------------------------------------------------
 
-struct fp_op {
-    bool iso = true;
-    int k = 2;
-    double w[3] = {1.0, 1.0, 1.0};
-    std::vector<double> spatial;  // iso: alpha (N); aniso: beta (k N); empty for none
-    int64_t sshape[3] = {1, 1, 1};
-    double spatial_max = 1.0;
-    // device copies of the spatial weights, per (device, dtype)
-    mutable std::mutex mu;
-    mutable std::vector<std::pair<int, void *>> dev_f32, dev_f64;
-    ~fp_op() {
-        for (auto &p : dev_f32) (void)hipFree(p.second);
-        for (auto &p : dev_f64) (void)hipFree(p.second);
-    }
-    int64_t nparams() const { return k + (int64_t)spatial.size(); }
-    double step_scale() const {
-        double m = 0.0;
-        for (int a = 0; a < 3; ++a) m = std::max(m, a < k ? w[a] : 0.0);
-        return m * (spatial.empty() ? 1.0 : spatial_max);
-    }
-    // The spatial weights on device dev in dtype (4 or 8 bytes); null without.
-    void *device_weights(int dev, int tsize, hipError_t &err) const {
-        err = hipSuccess;
-        if (spatial.empty()) return nullptr;
-        std::lock_guard<std::mutex> lock(mu);
-        auto &vec = tsize == 4 ? dev_f32 : dev_f64;
-        for (auto &p : vec)
-            if (p.first == dev) return p.second;
-        void *ptr = nullptr;
-        const size_t n = spatial.size();
-        if ((err = hipMalloc(&ptr, n * (size_t)tsize))) return nullptr;
-        if (tsize == 4) {
-            std::vector<float> f(spatial.begin(), spatial.end());
-            err = hipMemcpy(ptr, f.data(), n * 4, hipMemcpyHostToDevice);
-        } else {
-            err = hipMemcpy(ptr, spatial.data(), n * 8, hipMemcpyHostToDevice);
-        }
-        if (err) {
-            (void)hipFree(ptr);
-            return nullptr;
-        }
-        vec.emplace_back(dev, ptr);
-        return ptr;
-    }
-};
 
 namespace {
 
@@ -301,48 +257,6 @@ fp_status make_tv(bool iso, int ndim, const fp_array *axis_weights, const fp_arr
 
 }  // namespace
 
-struct fp_workspace {
-    const fp_op *op = nullptr;
-    int dtype = FP_F64;
-    int ndim = 0;
-    int64_t shape[FP_MAX_NDIM] = {};
-    int64_t numel = 0;
-    int k = 2;
-    int64_t B = 0, N = 0;
-    int64_t img[3] = {1, 1, 1};
-    int device = 0;
-    void *stream = nullptr;
-    bool keep = false;
-    size_t tsize = 8;
-    fpr::KernelChoice kc;
-    std::unique_ptr<fpr::Engine> eng;
-    void *dstage = nullptr, *zstage = nullptr, *q = nullptr;
-    void *xsave = nullptr, *zsave = nullptr;  // keep_structure: the point (x, z) of the last kept call
-    std::vector<char> host_stage;             // host arrays, packed
-    std::vector<double> gammas;
-    std::vector<fpr::ImagePtrs> ptrs;
-    // results of the last call
-    bool dual_valid = false, warm = false, seeded = false;
-    bool info_valid = false;
-    fp_status info_status = FP_OK;
-    fp_info info{0.0, 0, 0};
-    std::vector<double> gaps, zerrs, smooth;
-    std::vector<int> iters;
-    std::vector<char> exact;
-    // derivatives
-    bool valid = false;      // a kept point exists
-    bool steps_on = false;
-    double gamma = 0.0;
-    double deriv_tol = 0.0;
-    int deriv_max_iter = 10000;
-    double deriv_res = 0.0;
-    int deriv_its = 0;
-    std::unique_ptr<fpr::Deriv> deriv;
-    ~fp_workspace() {
-        for (void *p : {dstage, zstage, q, xsave, zsave})
-            if (p) (void)hipFree(p);
-    }
-};
 
 namespace {
 
@@ -564,6 +478,9 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     if (w->keep) {
         if (fp_status s = alloc(&w->xsave, bytes)) return s;
         if (fp_status s = alloc(&w->zsave, bytes)) return s;
+        if (!op->iso)
+            if (fp_status s = alloc(&w->usave, bytes * (size_t)op->k)) return s;
+        w->exps.assign((size_t)B, 0);
     }
     hipError_t err;
     (void)op->device_weights(w->device, (int)w->tsize, err);
@@ -719,7 +636,33 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
         const size_t bytes = (size_t)ws->numel * ws->tsize;
         if (hipMemcpyAsync(ws->xsave, d, bytes, hipMemcpyDeviceToDevice, s)) return FP_EDEVICE;
         if (hipMemcpyAsync(ws->zsave, z, bytes, hipMemcpyDeviceToDevice, s)) return FP_EDEVICE;
+        if (!op->iso && ws->eng->copy_dual_iterate(s, ws->usave, (int64_t)op->k * ws->N)) return FP_EDEVICE;
         if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        // The smoothing of the derivative system (tv_iso.cpp:566-568): mu =
+        // 0.1 w_max sqrt(2 gap / N) at the solved scale when the structure is
+        // read off the dual (two or more active axes), back at the input's.
+        for (int64_t b = 0; b < B; ++b) {
+            const fpr::ImgState &t = S[b];
+            ws->exps[(size_t)b] = t.e;
+            double mu = 0.0;
+            if (op->iso) {
+                double wmax = 0.0, wL = 0.0;
+                int na = 0;
+                for (int a = 0; a < k; ++a)
+                    if (ws->img[3 - k + a] > 1) wL = std::max(wL, op->w[a]);
+                for (int a = 0; a < k; ++a)
+                    if (ws->gammas[(size_t)b] > 0.0 && op->w[a] > 0.0 && ws->img[3 - k + a] > 1 &&
+                        op->w[a] >= 1e-100 * wL) {
+                        wmax = std::max(wmax, op->w[a]);
+                        ++na;
+                    }
+                const double gs = std::ldexp(ws->gaps[(size_t)b], -2 * t.e);
+                if (na >= 2 && gs == gs)
+                    mu = 0.1 * wmax * std::sqrt(2.0 * std::max(gs, 0.0) / (double)ws->N);
+            }
+            ws->smooth[(size_t)b] = std::ldexp(mu, t.e);
+            ws->exact[(size_t)b] = !(mu > 0.0);
+        }
         ws->valid = true;
         ws->steps_on = sa != nullptr;
         ws->gamma = step;
@@ -831,6 +774,222 @@ fp_status op_check(const fp_op *op, int *row_parallel_only) {
     return FP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Derivatives at a kept solve (the reference's jvp_impl / vjp_impl,
+// src/api/api.cpp:584-760, over the device derivatives of deriv.hip)
+// ---------------------------------------------------------------------------
+
+struct DevGuard {  // the workspace's device current for the call
+    int dev, was;
+    explicit DevGuard(int d) : dev(d), was(d) {
+        (void)hipGetDevice(&was);
+        if (was != dev) (void)hipSetDevice(dev);
+    }
+    ~DevGuard() {
+        if (was != dev) (void)hipSetDevice(was);
+    }
+};
+
+fp_status check_like(const fp_workspace &ws, const Arr &a) {
+    if (a.dtype != ws.dtype) return FP_EDTYPE;
+    if (a.ndim != ws.ndim) return FP_ESHAPE;
+    for (int d = 0; d < a.ndim; ++d)
+        if (a.shape[d] != ws.shape[d]) return FP_ESHAPE;
+    return FP_OK;
+}
+
+// Writes n doubles to a float array (host or device, any strides), in C order.
+fp_status write_doubles(const Arr &x, const double *v) {
+    const int64_t n = x.numel();
+    if (n == 0) return FP_OK;
+    intptr_t lo, hi;
+    x.span(lo, hi);
+    const size_t bytes = (size_t)(hi - lo + 1);
+    std::vector<char> host;
+    char *base = nullptr;
+    if (x.device == FP_DEVICE_ROCM) {
+        host.resize(bytes);
+        if (hipMemcpy(host.data(), (const void *)lo, bytes, hipMemcpyDeviceToHost)) return FP_EDEVICE;
+        base = host.data() + ((intptr_t)x.data - lo);
+    } else {
+        base = x.data;
+    }
+    const int64_t item = (int64_t)x.itemsize();
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t r = i, off = 0;
+        for (int d = x.ndim - 1; d >= 0; --d) {
+            const int64_t c = r % x.shape[d];
+            r /= x.shape[d];
+            off += c * x.strides[d];
+        }
+        char *p = base + off * item;
+        if (x.dtype == FP_F32) *(float *)p = (float)v[i];
+        else *(double *)p = v[i];
+    }
+    if (x.device == FP_DEVICE_ROCM && hipMemcpy((void *)lo, host.data(), bytes, hipMemcpyHostToDevice))
+        return FP_EDEVICE;
+    return FP_OK;
+}
+
+fp_status scratch(fp_workspace *ws, size_t n, double **out) {
+    if (ws->dscr_n < n) {
+        if (ws->dscr) (void)hipFree(ws->dscr);
+        ws->dscr = nullptr;
+        ws->dscr_n = 0;
+        if (hipMalloc((void **)&ws->dscr, std::max<size_t>(n, 16) * sizeof(double))) return FP_ENOMEM;
+        ws->dscr_n = n;
+    }
+    *out = ws->dscr;
+    return FP_OK;
+}
+
+// The structure of the kept solve, built on first use.
+fp_status ensure_deriv(const fp_op *op, fp_workspace *ws, hipStream_t s) {
+    if (ws->deriv) return FP_OK;
+    fpr::DerivSpec sp;
+    sp.iso = op->iso;
+    sp.k = op->k;
+    for (int a = 0; a < 3; ++a) sp.L[a] = ws->img[a];
+    sp.B = ws->B;
+    sp.tsize = (int)ws->tsize;
+    for (int a = 0; a < op->k; ++a) sp.w[a] = op->w[a];
+    sp.x = ws->xsave;
+    sp.z = ws->zsave;
+    if (op->iso) {
+        sp.u = ws->q;  // (k, B, N): the solve's dual
+        sp.u_img = ws->N;
+        sp.u_plane = ws->B * ws->N;
+    } else {
+        sp.u = ws->usave;  // (B, k, N): the per-edge dual
+        sp.u_img = (int64_t)op->k * ws->N;
+        sp.u_plane = ws->N;
+    }
+    hipError_t err;
+    sp.sw = op->device_weights(ws->device, (int)ws->tsize, err);
+    if (err) return hip_status(err);
+    sp.sw_img = 0;
+    sp.sw_plane = ws->N;
+    sp.gamma = ws->gammas.data();
+    sp.gap = ws->gaps.data();
+    sp.e = ws->exps.data();
+    fpr::Deriv *d = nullptr;
+    if (int r = fpr::deriv_create(sp, s, &d)) return (fp_status)r;
+    ws->deriv.reset(d);
+    return FP_OK;
+}
+
+fp_status jvp_impl(const fp_op *op, fp_workspace *ws, const fp_array *dxa, double dgamma, const fp_array *dpa,
+                   fp_array *oa) {
+    if (!op || !ws || !oa) return FP_EINVAL;
+    if (ws->op != op || !ws->valid) return FP_EINVAL;
+    if (!std::isfinite(dgamma)) return FP_EPARAM;
+    Arr dx, out;
+    if (fp_status s = to_arr(oa, out)) return s;
+    if (fp_status s = check_like(*ws, out)) return s;
+    if (dxa) {
+        if (fp_status s = to_arr(dxa, dx)) return s;
+        if (fp_status s = check_like(*ws, dx)) return s;
+    }
+    if (fp_status s = check_output(dxa ? &dx : nullptr, out)) return s;
+    const int64_t P = op->nparams();
+    std::vector<double> dp;
+    if (dpa && P > 0) {
+        Arr t;
+        if (fp_status s = to_arr(dpa, t)) return s;
+        if (dpa->ndim > 1 || t.numel() != P) return FP_ESHAPE;
+        if (fp_status s = read_doubles(t, dp)) return s;
+    }
+    if (ws->numel == 0) return FP_OK;
+    DevGuard guard(ws->device);
+    hipStream_t s = (hipStream_t)ws->stream;
+    if (fp_status st = ensure_deriv(op, ws, s)) return st;
+    const int64_t B = ws->B;
+    const int64_t ns = (int64_t)op->spatial.size();
+    const void *dxd = nullptr;
+    if (dxa) {
+        if (dx.device == FP_DEVICE_ROCM && dx.contiguous()) {
+            dxd = dx.data;
+        } else {
+            if (fp_status st = stage_in(*ws, dx, ws->dstage, s)) return st;
+            dxd = ws->dstage;
+        }
+    }
+    const bool o_direct = out.device == FP_DEVICE_ROCM && out.contiguous();
+    void *dz = o_direct ? (void *)out.data : ws->zstage;
+    double *scr = nullptr;
+    if (fp_status st = scratch(ws, (size_t)(B + ns), &scr)) return st;
+    std::vector<double> up((size_t)(B + ns), dgamma);
+    const bool sp_on = !dp.empty() && ns > 0;
+    if (sp_on)
+        for (int64_t i = 0; i < ns; ++i) up[(size_t)(B + i)] = dp[(size_t)(op->k + i)];
+    if (hipMemcpyAsync(scr, up.data(), sizeof(double) * (size_t)(B + (sp_on ? ns : 0)), hipMemcpyHostToDevice, s))
+        return FP_EDEVICE;
+    double dw[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < op->k && !dp.empty(); ++a) dw[a] = dp[(size_t)a];
+    int r = fpr::deriv_jvp(ws->deriv.get(), s, dxd, scr, dp.empty() ? nullptr : dw, sp_on ? scr + B : nullptr, dz,
+                           ws->deriv_tol, ws->deriv_max_iter);
+    if (r) return (fp_status)r;
+    if (!o_direct)
+        if (fp_status st = stage_out(*ws, dz, out, s)) return st;
+    if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    fpr::deriv_result(ws->deriv.get(), &ws->deriv_res, &ws->deriv_its, nullptr);
+    return FP_OK;
+}
+
+fp_status vjp_impl(const fp_op *op, fp_workspace *ws, const fp_array *ga, fp_array *gxa, double *ggamma,
+                   const fp_array *gga, fp_array *gpa) {
+    if (!op || !ws || !ga || !gxa) return FP_EINVAL;
+    if (ws->op != op || !ws->valid) return FP_EINVAL;
+    Arr g, gx, gp, gg;
+    if (fp_status s = to_arr(ga, g)) return s;
+    if (fp_status s = to_arr(gxa, gx)) return s;
+    if (fp_status s = check_like(*ws, g)) return s;
+    if (fp_status s = check_like(*ws, gx)) return s;
+    if (fp_status s = check_output(&g, gx)) return s;
+    const int64_t P = op->nparams();
+    if (gpa) {  // validate before any work
+        if (fp_status s = to_arr(gpa, gp)) return s;
+        if (gpa->ndim > 1 || gp.numel() != P) return FP_ESHAPE;
+    }
+    if (gga) {
+        if (fp_status s = to_arr(gga, gg)) return s;
+        if (gga->ndim != 0) return FP_ESHAPE;
+    }
+    const int64_t B = ws->B;
+    const int64_t ns = (int64_t)op->spatial.size();
+    std::vector<double> h((size_t)(B + op->k + ns), 0.0);
+    if (ws->numel > 0) {
+        DevGuard guard(ws->device);
+        hipStream_t s = (hipStream_t)ws->stream;
+        if (fp_status st = ensure_deriv(op, ws, s)) return st;
+        const void *gd = g.data;
+        if (!(g.device == FP_DEVICE_ROCM && g.contiguous())) {
+            if (fp_status st = stage_in(*ws, g, ws->dstage, s)) return st;
+            gd = ws->dstage;
+        }
+        const bool o_direct = gx.device == FP_DEVICE_ROCM && gx.contiguous() && gx.data != gd;
+        void *gxd = o_direct ? (void *)gx.data : ws->zstage;
+        double *scr = nullptr;
+        if (fp_status st = scratch(ws, h.size(), &scr)) return st;
+        int r = fpr::deriv_vjp(ws->deriv.get(), s, gd, gxd, scr, scr + B, ns ? scr + B + op->k : nullptr,
+                               ws->deriv_tol, ws->deriv_max_iter);
+        if (r) return (fp_status)r;
+        if (!o_direct)
+            if (fp_status st = stage_out(*ws, gxd, gx, s)) return st;
+        if (hipMemcpyAsync(h.data(), scr, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        fpr::deriv_result(ws->deriv.get(), &ws->deriv_res, &ws->deriv_its, nullptr);
+    }
+    double tot = 0.0;  // the gamma gradient summed over the images
+    for (int64_t b = 0; b < B; ++b) tot += h[(size_t)b];
+    if (ggamma) *ggamma = tot;
+    if (gga)
+        if (fp_status st = write_doubles(gg, &tot)) return st;
+    if (gpa && P > 0)
+        if (fp_status st = write_doubles(gp, h.data() + B)) return st;
+    return FP_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -850,6 +1009,18 @@ struct fpr_batch {
     std::vector<double> gammas, gaps, zerrs;
     std::vector<int> iters, conv;
     bool valid = false;
+};
+
+struct fpr_deriv {
+    std::unique_ptr<fpr::Deriv, fpr::DerivDeleter> d;
+    int device = 0;
+    int64_t B = 0;
+    int k = 2;
+    int64_t ns = 0;          // spatial entries (per image or shared), 0 without
+    double *scr = nullptr;   // device: the axis weights' gradient
+    ~fpr_deriv() {
+        if (scr) (void)hipFree(scr);
+    }
 };
 
 namespace {
@@ -970,20 +1141,20 @@ FP_API fp_status fp_prox_steps(const fp_op *op, const fp_array *gamma, const fp_
 
 FP_API fp_status fp_jvp(const fp_op *op, const fp_workspace *ws, const fp_array *dx, double dgamma,
                         const fp_array *dparams, fp_array *dout) {
-    return guard([&] { return fpr::jvp_impl(op, const_cast<fp_workspace *>(ws), dx, dgamma, dparams, dout); });
+    return guard([&] { return jvp_impl(op, const_cast<fp_workspace *>(ws), dx, dgamma, dparams, dout); });
 }
 
 FP_API fp_status fp_vjp(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx, double *ggamma,
                         fp_array *gparams) {
     return guard([&] {
-        return fpr::vjp_impl(op, const_cast<fp_workspace *>(ws), g, gx, ggamma, nullptr, gparams);
+        return vjp_impl(op, const_cast<fp_workspace *>(ws), g, gx, ggamma, nullptr, gparams);
     });
 }
 
 FP_API fp_status fp_vjp_device(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx,
                                fp_array *ggamma, fp_array *gparams) {
     return guard([&] {
-        return fpr::vjp_impl(op, const_cast<fp_workspace *>(ws), g, gx, nullptr, ggamma, gparams);
+        return vjp_impl(op, const_cast<fp_workspace *>(ws), g, gx, nullptr, ggamma, gparams);
     });
 }
 
@@ -1242,6 +1413,129 @@ FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt,
     return FP_OK;
 }
 
+FP_API fp_status fpr_batch_scale_exps(const fpr_batch *b, int *exps, int64_t count) {
+    if (!b || !exps || !b->valid) return FP_EINVAL;
+    if (count != (int64_t)b->specs.size()) return FP_ESHAPE;
+    const fpr::ImgState *S = b->eng->state();
+    for (int64_t i = 0; i < count; ++i) exps[i] = S[i].e;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_batch_copy_dual_iterate(const fpr_batch *b, void *dst, void *stream) {
+    return guard([&] {
+        if (!b || !dst || !b->valid || !b->eng) return FP_EINVAL;
+        for (const fpr::ImageSpec &sp : b->specs)
+            for (int a = 0; a < 3; ++a)
+                if (sp.L[a] != b->specs[0].L[a]) return FP_ESHAPE;
+        const int64_t N = b->specs.empty() ? 0 : b->specs[0].L[0] * b->specs[0].L[1] * b->specs[0].L[2];
+        DevGuard dg(b->device);
+        return hip_status(b->eng->copy_dual_iterate((hipStream_t)stream, dst, (int64_t)b->op->k * N));
+    });
+}
+
+FP_API fp_status fpr_deriv_create(const fp_op *op, fp_dtype dtype, int64_t count, const int64_t *img_shape,
+                                  const void *x, const void *z, const void *dual, const void *spatial,
+                                  int spatial_per_image, const double *gammas, const double *gaps,
+                                  const int *scale_exps, void *stream, fpr_deriv **out) {
+    return guard([&] {
+        if (!out) return FP_EINVAL;
+        *out = nullptr;
+        if (!op || !img_shape || count <= 0 || !x || !z || !dual || !gammas || (op->iso && !gaps)) return FP_EINVAL;
+        const int dt = enum_bits(dtype);
+        if (dt != FP_F32 && dt != FP_F64) return FP_EDTYPE;
+        if (!op->spatial.empty() && spatial) return FP_EINVAL;  // the op's own map, or the caller's
+        std::unique_ptr<fpr_deriv> d(new fpr_deriv());
+        fpr::DerivSpec sp;
+        sp.iso = op->iso;
+        sp.k = op->k;
+        int64_t N = 1;
+        for (int a = 0; a < op->k; ++a) {
+            if (img_shape[a] < 1) return FP_ESHAPE;
+            sp.L[3 - op->k + a] = img_shape[a];
+            N *= img_shape[a];
+        }
+        sp.B = count;
+        sp.tsize = dt == FP_F32 ? 4 : 8;
+        for (int a = 0; a < op->k; ++a) sp.w[a] = op->w[a];
+        sp.x = x;
+        sp.z = z;
+        sp.u = dual;
+        sp.u_img = (int64_t)op->k * N;
+        sp.u_plane = N;
+        if (!op->spatial.empty()) {
+            hipError_t err;
+            int dev = 0;
+            (void)hipGetDevice(&dev);
+            sp.sw = op->device_weights(dev, sp.tsize, err);
+            if (err) return hip_status(err);
+            sp.sw_img = 0;
+        } else {
+            sp.sw = spatial;
+            sp.sw_img = spatial && spatial_per_image ? (op->iso ? N : (int64_t)op->k * N) : 0;
+        }
+        sp.sw_plane = N;
+        sp.gamma = gammas;
+        sp.gap = gaps;
+        sp.e = scale_exps;
+        d->B = count;
+        d->k = op->k;
+        d->ns = sp.sw ? (op->iso ? N : (int64_t)op->k * N) * (sp.sw_img ? count : 1) : 0;
+        (void)hipGetDevice(&d->device);
+        if (hipMalloc((void **)&d->scr, 8 * (size_t)(count + 4))) return FP_ENOMEM;
+        fpr::Deriv *dv = nullptr;
+        if (int r = fpr::deriv_create(sp, (hipStream_t)stream, &dv)) return (fp_status)r;
+        d->d.reset(dv);
+        *out = d.release();
+        return FP_OK;
+    });
+}
+
+FP_API void fpr_deriv_free(fpr_deriv *d) { delete d; }
+
+FP_API fp_status fpr_deriv_vjp(fpr_deriv *d, const void *g, void *gx, double *ggamma, double *gw, double *gspatial,
+                               double tol, int max_iter, void *stream) {
+    return guard([&] {
+        if (!d || !g || !gx || !(tol >= 0.0) || max_iter < 1) return FP_EINVAL;
+        if (gspatial && !d->ns) return FP_EINVAL;
+        DevGuard dg(d->device);
+        hipStream_t s = (hipStream_t)stream;
+        if (int r = fpr::deriv_vjp(d->d.get(), s, g, gx, ggamma, d->scr, gspatial, tol, max_iter))
+            return (fp_status)r;
+        double h[3] = {0.0, 0.0, 0.0};
+        if (hipMemcpyAsync(h, d->scr, 8 * (size_t)d->k, hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        if (gw)
+            for (int a = 0; a < d->k; ++a) gw[a] = h[a];
+        return FP_OK;
+    });
+}
+
+FP_API fp_status fpr_deriv_jvp(fpr_deriv *d, const void *dx, const double *dgamma, const double *dw,
+                               const double *dspatial, void *dz, double tol, int max_iter, void *stream) {
+    return guard([&] {
+        if (!d || !dz || !(tol >= 0.0) || max_iter < 1) return FP_EINVAL;
+        if (dspatial && !d->ns) return FP_EINVAL;
+        DevGuard dg(d->device);
+        hipStream_t s = (hipStream_t)stream;
+        if (int r = fpr::deriv_jvp(d->d.get(), s, dx, dgamma, dw, dspatial, dz, tol, max_iter)) return (fp_status)r;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        return FP_OK;
+    });
+}
+
+FP_API fp_status fpr_deriv_info(const fpr_deriv *d, double *residual, int *iters, int64_t *regions) {
+    if (!d) return FP_EINVAL;
+    fpr::deriv_result(d->d.get(), residual, iters, regions);
+    return FP_OK;
+}
+
+FP_API fp_status fpr_deriv_image_info(const fpr_deriv *d, double *residuals, int *iters, int64_t count) {
+    if (!d) return FP_EINVAL;
+    if (count != d->B) return FP_ESHAPE;
+    fpr::deriv_image_result(d->d.get(), residuals, iters);
+    return FP_OK;
+}
+
 FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs) {
     if (!b) return FP_EINVAL;
     if (launches) *launches = b->eng ? b->eng->stats.launches : 0;
@@ -1272,13 +1566,3 @@ FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int 
 
 }  // extern "C"
 
-// Derivatives: not yet on the device.
-namespace fpr {
-struct Deriv {};
-fp_status jvp_impl(const fp_op *, fp_workspace *, const fp_array *, double, const fp_array *, fp_array *) {
-    return FP_EINVAL;
-}
-fp_status vjp_impl(const fp_op *, fp_workspace *, const fp_array *, fp_array *, double *, fp_array *, fp_array *) {
-    return FP_EINVAL;
-}
-}  // namespace fpr

@@ -1,0 +1,1416 @@
+// Derivatives of image TV at a solution, on the device.
+//
+// Isotropic TV (src/ops/tv_iso.cpp:551-631 save, :672-1067 Sys, cg, jvp_img,
+// vjp_img): z - d + G^T u = 0 with G = W D. Pixels with the dual strictly
+// inside the ball (|u_p| < r_p (1 - 1e-9)), or a gradient at the rounding
+// level of the data (|g_p| <= w_max 64 2.2e-16 max|d|), are flat: z is
+// constant across their forward edges, so dz lives on the connected regions
+// of those edges (one unknown per region). On the others the dual moves on
+// the sphere, du_p = H_p (G dz)_p with H_p = r_p / max(|g_p|, mu) (I - n n^T)
+// and mu = 0.1 w_max sqrt(2 gap / N), which gives the SPD system
+// M^T (I + G^T H G) M c = M^T rhs in region space, solved by conjugate
+// gradients with the Jacobi preconditioner.
+//
+// Anisotropic TV, the device method's derivative
+// (bindings/python/fastprox/_tv_torch.py:346-362, :528-585): flat edges
+// (|u_e| < r_e (1 - eta), or a jump at the rounding level with r_e > 0) join
+// their pixels; dz is the region mean of dx - sum_a dw_a D_a^T u_a -
+// G^T (dr n), n = sign(u) on the other edges.
+//
+// Layout: a uniform batch of B images, pixel gp = b N + p. The structure:
+//   labels: union-find over the joined edges with atomic hooks of the larger
+//     root under the smaller (the component's smallest index ends as its
+//     root whatever the order), full compression, an exclusive scan of the
+//     roots (global region ids, ordered by image, then by first pixel);
+//   members: a stable radix sort of (label, pixel), so each region's pixels
+//     are contiguous and in index order; rstart marks each region's start.
+// Every sum is in a fixed order (no floating-point atomics), so results
+// repeat bit for bit run to run:
+//   region sums: blocks of 4096 member positions (256 threads x 16), each
+//     thread summing its runs in order, a block-wide segmented scan joining
+//     runs across threads, and the pieces of regions that cross blocks added
+//     in block order by the region's consumer;
+//   per-image dot products in region space: blocks of 1024 regions of one
+//     image, a fixed tree in each, the block partials added in order;
+//   per-image pixel sums: blocks of 4096 pixels, the same way.
+// CG runs every image of the batch in lockstep, each with its own scalars;
+// an image that meets its stopping rule is frozen where the reference's
+// per-image loop would stop (tv_iso.cpp:919-942), so iteration counts and
+// residuals are per image. The host checks for a live image every 8
+// iterations.
+//
+// SPDX-License-Identifier: MIT
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/fastprox_rocm.h"
+#include "deriv.hpp"
+
+namespace fpr {
+namespace {
+
+constexpr double kDualEta = 1e-9;     // tv_iso.cpp:109
+constexpr double kCurvRms = 0.1;      // tv_iso.cpp:113
+constexpr double kDerivFloor = 1e-10; // tv_iso.cpp:115
+constexpr int kThr = 256;
+constexpr int kPerThr = 16;
+constexpr int kChunk = kThr * kPerThr;  // member positions per region-sum block
+constexpr int kRegPer = 4;
+constexpr int kRB = kThr * kRegPer;     // regions per region-space block
+constexpr int kPixBlk = kThr * 16;      // pixels per block of the per-image pixel sums
+constexpr int kCheck = 8;               // CG iterations between host checks
+constexpr int kMaxQ = 7;                // per-image pixel sums (2k + 1)
+
+struct Geo {
+    int k;
+    int64_t L[3], S[3], N, BN;
+};
+
+struct DImg {
+    double gamma_s;   // the step at the solved scale
+    double sc;        // 2^e
+    double gap_s;     // the gap at the solved scale
+    double thr, rel, mu;
+    double wsave[3];  // iso: the structure's weights (the solve's active axes)
+    double wa[3];     // iso: the system's weights; aniso: w
+    double floor_, eta;  // aniso flat test
+    int e, nan, from_dual, use_dual;
+    int rbase, rcount, bbase, bcount;
+};
+
+struct CgState {
+    double bn, rz, res, tol;
+    int it, active, max_iter, pad_;
+};
+
+struct RBlock {
+    int img, c0, c1, pad_;
+};
+
+__device__ __forceinline__ void coords(const Geo &G, int64_t p, int64_t c[3]) {
+    c[2] = p % G.L[2];
+    const int64_t t = p / G.L[2];
+    c[1] = t % G.L[1];
+    c[0] = t / G.L[1];
+}
+
+template <class T>
+__device__ __forceinline__ double ld(const T *a, int64_t i) {
+    return (double)a[i];
+}
+
+__device__ __forceinline__ double dmaxd(double a, double b) { return a < b ? b : a; }
+
+// ---------------------------------------------------------------------------
+// Fixed-order block reductions
+// ---------------------------------------------------------------------------
+
+__device__ double block_sum(double v, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int d = kThr / 2; d > 0; d >>= 1) {
+        if (t < d) sh[t] = sh[t] + sh[t + d];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int d = kThr / 2; d > 0; d >>= 1) {
+        if (t < d) sh[t] = dmaxd(sh[t], sh[t + d]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Structure
+// ---------------------------------------------------------------------------
+
+// Per image: NaN, max |d_s| and |d_s|^2 (d_s = 2^-e x), then the flat
+// threshold, the certified relative error and the curvature cap
+// (tv_iso.cpp:557-569); aniso: the rounding floor of the jumps
+// (_tv_torch.py:348-350).
+template <class T>
+__global__ __launch_bounds__(kThr) void prep_kernel(Geo G, int iso, double eps, const T *x, DImg *di) {
+    __shared__ double sh[kThr];
+    const int b = blockIdx.x;
+    DImg &D = di[b];
+    const T *xb = x + (int64_t)b * G.N;
+    double m = 0.0, ss = 0.0, nanv = 0.0;
+    for (int64_t i = threadIdx.x; i < G.N; i += kThr) {
+        const double v = ldexp(ld(xb, i), -D.e);
+        if (v != v) nanv = 1.0;
+        m = dmaxd(m, fabs(v));
+    }
+    m = block_max(m, sh);
+    nanv = block_max(nanv, sh);
+    for (int64_t i = threadIdx.x; i < G.N; i += kThr) {
+        const double v = ldexp(ld(xb, i), -D.e);
+        ss += v * v;
+    }
+    ss = block_sum(ss, sh);
+    if (threadIdx.x == 0) {
+        D.nan = nanv != 0.0;
+        if (iso) {
+            double wm = 0.0;
+            for (int a = 0; a < G.k; ++a) wm = dmaxd(wm, D.wsave[a]);
+            D.from_dual = D.from_dual && !D.nan;
+            D.use_dual = D.use_dual && !D.nan;
+            const double gap = D.nan ? 0.0 : D.gap_s;
+            D.thr = D.from_dual ? wm * 64.0 * 2.2e-16 * m : 0.0;
+            D.rel = ss > 0.0 ? sqrt(2.0 * dmaxd(gap, 0.0) / ss) : 0.0;
+            D.mu = D.from_dual ? kCurvRms * wm * sqrt(2.0 * dmaxd(gap, 0.0) / (double)G.N) : 0.0;
+        } else {
+            double wm = 0.0;
+            for (int a = 0; a < G.k; ++a) wm = dmaxd(wm, D.wa[a]);
+            D.floor_ = wm * 64.0 * eps * m;
+            D.eta = dmaxd(1e-9, 64.0 * eps);
+            D.rel = 0.0;
+            D.mu = 0.0;
+        }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ double radius(const DImg &D, const T *sw, int64_t sw_img, int b, int64_t p) {
+    if (!sw) return D.gamma_s;
+    const double a = ld(sw, b * sw_img + p);
+    if (a == 0.0) return 0.0;
+    return D.gamma_s * a;
+}
+
+// Iso: gz = W D z_s (the system's weights), the flat test (the structure's
+// weights), and the union-find parents.
+template <class T>
+__global__ __launch_bounds__(kThr) void flat_iso_kernel(Geo G, const T *x, const T *z, const T *u, int64_t u_img,
+                                                        int64_t u_plane, const T *sw, int64_t sw_img, const DImg *di,
+                                                        double *gz, uint8_t *flat, int *parent) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    const T *zsrc = D.nan ? x : z;
+    int64_t c[3];
+    coords(G, p, c);
+    const double z0 = ldexp(ld(zsrc, gp), -D.e);
+    double nn = 0.0;
+    bool any = false;
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        const bool nx = c[ax] + 1 < G.L[ax];
+        const double z1 = nx ? ldexp(ld(zsrc, gp + G.S[ax]), -D.e) : 0.0;
+        gz[a * G.BN + gp] = (D.wa[a] != 0.0 && nx) ? D.wa[a] * (z1 - z0) : 0.0;
+        if (D.wsave[a] != 0.0 && nx) {
+            const double t = D.wsave[a] * (z1 - z0);
+            nn += t * t;
+            any = true;
+        }
+    }
+    bool fl = !any || sqrt(nn) <= D.thr;
+    if (D.from_dual && !fl) {
+        double uu = 0.0;
+        for (int a = 0; a < G.k; ++a) {
+            const double ua = ldexp(ld(u, b * u_img + a * u_plane + p), -D.e);
+            uu += ua * ua;
+        }
+        fl = sqrt(uu) < radius(D, sw, sw_img, b, p) * (1.0 - kDualEta);
+    }
+    flat[gp] = (uint8_t)((fl ? 1 : 0) | (any ? 2 : 0));
+    parent[gp] = (int)gp;
+}
+
+// Aniso: the flat edges (bit a of flat) and the signs n on the others.
+template <class T>
+__global__ __launch_bounds__(kThr) void flat_aniso_kernel(Geo G, const T *x, const T *z, const T *u, int64_t u_img,
+                                                          int64_t u_plane, const T *sw, int64_t sw_img,
+                                                          int64_t sw_plane, const DImg *di, int8_t *nsg, uint8_t *flat,
+                                                          int *parent) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    int64_t c[3];
+    coords(G, p, c);
+    const double z0 = ld(z, gp);
+    uint8_t bits = 0;
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        const bool nx = c[ax] + 1 < G.L[ax];
+        const double be = sw ? ld(sw, b * sw_img + a * sw_plane + p) : 1.0;
+        const double r = sw ? D.gamma_s * be : D.gamma_s;
+        const double inside = r * (1.0 - D.eta);
+        const double gj = nx ? D.wa[a] * (ld(z, gp + G.S[ax]) - z0) : 0.0;
+        const double ua = ld(u, b * u_img + a * u_plane + p);
+        const bool fl = fabs(ua) < inside || (fabs(gj) <= D.floor_ && inside > 0.0);
+        double s = ua > 0.0 ? 1.0 : (ua < 0.0 ? -1.0 : 0.0);
+        if (sw && be == 0.0) s = gj > 0.0 ? 1.0 : (gj < 0.0 ? -1.0 : 0.0);
+        nsg[a * G.BN + gp] = (int8_t)(fl ? 0 : (int)s);
+        if (fl && nx && D.wa[a] != 0.0) bits |= (uint8_t)(1u << a);
+    }
+    flat[gp] = bits;
+    parent[gp] = (int)gp;
+}
+
+__device__ int uf_find(int *par, int x) {
+    while (true) {
+        const int p = __atomic_load_n(&par[x], __ATOMIC_RELAXED);
+        if (p == x) return x;
+        const int q = __atomic_load_n(&par[p], __ATOMIC_RELAXED);
+        if (q == p) return p;
+        __atomic_store_n(&par[x], q, __ATOMIC_RELAXED);  // path halving: q is an ancestor of x
+        x = q;
+    }
+}
+
+// Hooks the larger root under the smaller until a and b share a root.
+__device__ void uf_unite(int *par, int a, int b) {
+    while (true) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        if (a > b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(&par[b], b, a) == b) return;
+    }
+}
+
+// Joins: iso, a flat pixel with every forward neighbour along the
+// structure's active axes (tv_iso.cpp:598-605); aniso, the flat edges.
+__global__ __launch_bounds__(kThr) void union_kernel(Geo G, int iso, const DImg *di, const uint8_t *flat, int *par) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const uint8_t f = flat[gp];
+    if (iso && f != 3) return;  // flat and with a neighbour
+    if (!iso && f == 0) return;
+    const int b = (int)(gp / G.N);
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    int64_t c[3];
+    coords(G, p, c);
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        const bool join = iso ? (D.wsave[a] != 0.0 && c[ax] + 1 < G.L[ax]) : ((f >> a) & 1);
+        if (join) uf_unite(par, (int)gp, (int)(gp + G.S[ax]));
+    }
+}
+
+__global__ __launch_bounds__(kThr) void compress_kernel(int64_t BN, int *par, int *isroot) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= BN) return;
+    const int r = uf_find(par, (int)gp);
+    isroot[gp] = r == (int)gp ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kThr) void finish_roots_kernel(int64_t BN, int *par) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= BN) return;
+    par[gp] = uf_find(par, (int)gp);
+}
+
+// label = the rank of the root; iota for the sort's values; per-image
+// region bases (the first pixel of an image is a root) and the total.
+__global__ __launch_bounds__(kThr) void label_kernel(Geo G, const int *par, const int *rank, const int *isroot,
+                                                     int *label, int *iota, int *rbase) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    label[gp] = rank[par[gp]];
+    iota[gp] = (int)gp;
+    if (gp % G.N == 0) rbase[gp / G.N] = rank[gp];
+    if (gp == G.BN - 1) rbase[G.BN / G.N] = rank[gp] + isroot[gp];
+}
+
+__global__ __launch_bounds__(kThr) void rstart_kernel(int64_t BN, const int *skey, int *rstart) {
+    const int64_t m = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (m >= BN) return;
+    if (m == 0 || skey[m] != skey[m - 1]) rstart[skey[m]] = (int)m;
+    if (m == BN - 1) rstart[skey[m] + 1] = (int)BN;
+}
+
+__global__ __launch_bounds__(kThr) void inv_kernel(int nc, const int *rstart, double *inv) {
+    const int C = blockIdx.x * kThr + threadIdx.x;
+    if (C >= nc) return;
+    inv[C] = 1.0 / (double)(rstart[C + 1] - rstart[C]);  // the count of tv_iso.cpp:617-618, exactly
+}
+
+// ---------------------------------------------------------------------------
+// Region sums
+// ---------------------------------------------------------------------------
+
+// out[C] = the sum of f(p, C) over C's members for every region inside one
+// block of kChunk member positions; head[j] / tail[j]: the parts of the
+// regions that cross into block j from before / out of it after.
+template <class F>
+__global__ __launch_bounds__(kThr) void rsum_kernel(F f, const int *skey, const int *members, const int *rstart,
+                                                    int64_t M, double *out, double *head, double *tail) {
+    __shared__ double sv[kThr];
+    __shared__ int sf[kThr];
+    const int64_t base = (int64_t)blockIdx.x * kChunk;
+    const int t = threadIdx.x;
+    const int64_t m0 = base + (int64_t)t * kPerThr;
+    const int64_t m1 = m0 + kPerThr < M ? m0 + kPerThr : M;
+    double open = 0.0;
+    int reset = 1;
+    int has_head = 0, head_key = -1, last_key = -1, last_after = 0;
+    double head_sum = 0.0;
+    if (m0 < m1) {
+        const int kprev = m0 > 0 ? skey[m0 - 1] : -1;
+        const int knext = m1 < M ? skey[m1] : -1;
+        int cur = skey[m0];
+        double s = 0.0;
+        bool first = true;
+        for (int64_t m = m0; m < m1; ++m) {
+            const int C = skey[m];
+            if (C != cur) {
+                if (first && cur == kprev) {
+                    has_head = 1;
+                    head_sum = s;
+                    head_key = cur;
+                } else {
+                    out[cur] = s;
+                }
+                first = false;
+                cur = C;
+                s = 0.0;
+            }
+            s += f(members[m], C);
+        }
+        const bool before = first && cur == kprev;
+        const bool after = cur == knext;
+        if (!before && !after) {
+            out[cur] = s;
+        } else if (before && !after) {
+            has_head = 1;
+            head_sum = s;
+            head_key = cur;
+        } else {
+            open = s;
+            reset = before ? 0 : 1;
+        }
+        last_key = cur;
+        last_after = after ? 1 : 0;
+    }
+    // Inclusive segmented scan of the open runs over the threads, in a fixed
+    // tree: sv[t] = the sum of the run open at thread t's end, from its start
+    // (or the block's).
+    sv[t] = open;
+    sf[t] = reset;
+    __syncthreads();
+    for (int d = 1; d < kThr; d <<= 1) {
+        double v = sv[t];
+        int fl = sf[t];
+        if (t >= d && !fl) {
+            v = sv[t - d] + v;
+            fl = sf[t - d];
+        }
+        __syncthreads();
+        sv[t] = v;
+        sf[t] = fl;
+        __syncthreads();
+    }
+    if (has_head) {
+        const double tot = (t > 0 ? sv[t - 1] : 0.0) + head_sum;
+        if (rstart[head_key] >= base) out[head_key] = tot;
+        else head[blockIdx.x] = tot;
+    }
+    const int64_t cend = base + kChunk < M ? base + kChunk : M;
+    const int tl = (int)((cend - base - 1) / kPerThr);
+    if (t == tl && last_after) {
+        if (rstart[last_key] >= base) tail[blockIdx.x] = sv[t];
+        else head[blockIdx.x] = sv[t];  // the region covers the whole block
+    }
+}
+
+// A region's sum: its block's, or its pieces added in block order.
+__device__ __forceinline__ double region_value(int C, const double *out, const int *rstart, const double *head,
+                                               const double *tail) {
+    const int64_t j0 = rstart[C] / kChunk, j1 = (rstart[C + 1] - 1) / kChunk;
+    if (j0 == j1) return out[C];
+    double t = tail[j0];
+    for (int64_t j = j0 + 1; j <= j1; ++j) t += head[j];
+    return t;
+}
+
+template <class T>
+struct FPix {  // a pixel array of the dtype
+    const T *v;
+    __device__ double operator()(int p, int) const { return ld(v, p); }
+};
+
+struct FPixD {  // a double pixel array
+    const double *v;
+    __device__ double operator()(int p, int) const { return v[p]; }
+};
+
+// The apply's pixel values (tv_iso.cpp:881-887): pv_C - sum_a w_a (h_a[p] -
+// h_a[p - e_a]), zero for frozen images.
+struct FApply {
+    Geo G;
+    const double *pv, *h;
+    const DImg *di;
+    const CgState *cg;
+    __device__ double operator()(int gp, int C) const {
+        const int b = (int)(gp / G.N);
+        if (!cg[b].active) return 0.0;
+        const DImg &D = di[b];
+        int64_t c[3];
+        coords(G, gp - (int64_t)b * G.N, c);
+        double q = -pv[C];
+        for (int a = 0; a < G.k; ++a) {
+            if (D.wa[a] == 0.0) continue;
+            const int ax = 3 - G.k + a;
+            const double hp = h[a * G.BN + gp];
+            q += c[ax] > 0 ? D.wa[a] * (hp - h[a * G.BN + gp - G.S[ax]]) : D.wa[a] * hp;
+        }
+        return -q;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Iso system pieces (per pixel)
+// ---------------------------------------------------------------------------
+
+// H_p y (tv_iso.cpp:810-828) in place on y (k values), for a non-flat pixel.
+__device__ __forceinline__ void hess_apply(int k, const double *g, double h, double *y) {
+    double nn = 0.0, dot = 0.0;
+    for (int a = 0; a < k; ++a) nn += g[a] * g[a];
+    const double nr = sqrt(nn);
+    for (int a = 0; a < k; ++a) dot += g[a] * y[a];
+    for (int a = 0; a < k; ++a) {
+        const double na = g[a] / nr;
+        y[a] = h * (y[a] - na * dot / nr);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ double curv(const Geo &G, const DImg &D, const double *gzp, const T *sw, int64_t sw_img,
+                                       int b, int64_t p, double *nr_out) {
+    double nn = 0.0;
+    for (int a = 0; a < G.k; ++a) nn += gzp[a] * gzp[a];
+    const double nr = sqrt(nn);
+    *nr_out = nr;
+    return radius(D, sw, sw_img, b, p) / dmaxd(nr, D.mu);
+}
+
+// h = H G (M pv) at every pixel of a live image (the apply's first half).
+template <class T>
+__global__ __launch_bounds__(kThr) void hgv_kernel(Geo G, const DImg *di, const CgState *cg, const double *gz,
+                                                   const uint8_t *flat, const int *label, const double *pv,
+                                                   const T *sw, int64_t sw_img, double *h) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    if (!cg[b].active) return;
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    if (flat[gp] & 1) {
+        for (int a = 0; a < G.k; ++a) h[a * G.BN + gp] = 0.0;
+        return;
+    }
+    int64_t c[3];
+    coords(G, p, c);
+    const double x0 = pv[label[gp]];
+    double y[3], g[3];
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        y[a] = (D.wa[a] != 0.0 && c[ax] + 1 < G.L[ax]) ? D.wa[a] * (pv[label[gp + G.S[ax]]] - x0) : 0.0;
+        g[a] = gz[a * G.BN + gp];
+    }
+    double nr;
+    const double hs = curv(G, D, g, sw, sw_img, b, p, &nr);
+    hess_apply(G.k, g, hs, y);
+    for (int a = 0; a < G.k; ++a) h[a * G.BN + gp] = y[a];
+}
+
+// The Jacobi diagonal's terms moved onto pixels (tv_iso.cpp:833-870, in the
+// gathered form of _tv_torch.py:446-485): own[q] = the term of q's region
+// from q itself and from each backward neighbour p = q - e_a whose forward
+// edge reaches q's region first.
+template <class T>
+__device__ double jac_term(const Geo &G, const DImg &D, const double *gz, const int *label, int64_t gp,
+                           const int64_t *c, const T *sw, int64_t sw_img, int b, int64_t p, int C) {
+    double g[3];
+    for (int a = 0; a < G.k; ++a) g[a] = gz[a * G.BN + gp];
+    double nr;
+    const double hs = curv(G, D, g, sw, sw_img, b, p, &nr);
+    const int l0 = label[gp];
+    double hh = 0.0, nh = 0.0;
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        if (!(D.wa[a] != 0.0 && c[ax] + 1 < G.L[ax])) continue;
+        const int la = label[gp + G.S[ax]];
+        const double hv = D.wa[a] * ((la == C ? 1.0 : 0.0) - (l0 == C ? 1.0 : 0.0));
+        hh += hv * hv;
+        nh += hv * g[a] / nr;
+    }
+    return hs * (hh - nh * nh);
+}
+
+template <class T>
+__global__ __launch_bounds__(kThr) void jacobi_own_kernel(Geo G, const DImg *di, const double *gz,
+                                                          const uint8_t *flat, const int *label, const T *sw,
+                                                          int64_t sw_img, double *own) {
+    const int64_t gq = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gq >= G.BN) return;
+    const int b = (int)(gq / G.N);
+    const int64_t q = gq - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    int64_t cq[3];
+    coords(G, q, cq);
+    const int C = label[gq];
+    double s = 0.0;
+    if (!(flat[gq] & 1)) s += jac_term(G, D, gz, label, gq, cq, sw, sw_img, b, q, C);
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        if (cq[ax] == 0 || D.wa[a] == 0.0) continue;
+        const int64_t gp = gq - G.S[ax];
+        if (flat[gp] & 1) continue;
+        if (label[gp] == C) continue;
+        int64_t cp[3] = {cq[0], cq[1], cq[2]};
+        cp[ax] -= 1;
+        bool seen = false;
+        for (int a2 = 0; a2 < a; ++a2) {
+            const int ax2 = 3 - G.k + a2;
+            if (D.wa[a2] != 0.0 && cp[ax2] + 1 < G.L[ax2] && label[gp + G.S[ax2]] == C) seen = true;
+        }
+        if (seen) continue;
+        s += jac_term(G, D, gz, label, gp, cp, sw, sw_img, b, q - G.S[ax], C);
+    }
+    own[gq] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Region-space kernels (blocks of kRB regions of one image)
+// ---------------------------------------------------------------------------
+
+// dinv = 1 / (|C| + the summed terms).
+__global__ __launch_bounds__(kThr) void dinv_kernel(const RBlock *rb, const int *rstart, const double *osum,
+                                                    const double *head, const double *tail, const double *inv,
+                                                    double *dinv) {
+    const RBlock B = rb[blockIdx.x];
+    for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr)
+        dinv[C] = 1.0 / (1.0 / inv[C] + region_value(C, osum, rstart, head, tail));
+}
+
+// Region means (aniso): out[C] = sum / |C|.
+__global__ __launch_bounds__(kThr) void mean_kernel(const RBlock *rb, const int *rstart, const double *rs,
+                                                    const double *head, const double *tail, const double *inv,
+                                                    double *out) {
+    const RBlock B = rb[blockIdx.x];
+    for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr) out[C] = region_value(C, rs, rstart, head, tail) * inv[C];
+}
+
+// CG set-up (tv_iso.cpp:904-915): b finalised, x = 0, r = b, pv = r dinv;
+// partials of b^2 inv and r pv.
+__global__ __launch_bounds__(kThr) void cg_init_kernel(const RBlock *rb, const int *rstart, double *bv,
+                                                       const double *head, const double *tail, const double *inv,
+                                                       const double *dinv, double *x, double *r, double *pv,
+                                                       double *part) {
+    __shared__ double sh[kThr];
+    const RBlock B = rb[blockIdx.x];
+    double t0 = 0.0, t1 = 0.0;
+    for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr) {
+        const double v = region_value(C, bv, rstart, head, tail);
+        bv[C] = v;
+        t0 += v * v * inv[C];
+        x[C] = 0.0;
+        r[C] = v;
+        pv[C] = v * dinv[C];
+        t1 += v * pv[C];
+    }
+    t0 = block_sum(t0, sh);
+    t1 = block_sum(t1, sh);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = t0;
+        part[3 * blockIdx.x + 1] = t1;
+    }
+}
+
+__device__ __forceinline__ double image_total(const double *part, int j0, int n, int q) {
+    double t = 0.0;
+    for (int j = j0; j < j0 + n; ++j) t += part[3 * j + q];
+    return t;
+}
+
+__global__ void cg_start_kernel(int B, const DImg *di, const double *part, double tol, int max_iter, CgState *cg) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const DImg &D = di[b];
+    CgState S;
+    S.bn = image_total(part, D.bbase, D.bcount, 0);
+    S.rz = image_total(part, D.bbase, D.bcount, 1);
+    S.tol = tol > 0.0 ? tol : dmaxd(kDerivFloor, D.rel);
+    S.it = 0;
+    S.max_iter = max_iter;
+    S.pad_ = 0;
+    if (S.bn == 0.0) {
+        S.res = 0.0;
+        S.active = 0;
+    } else {
+        S.res = 1.0;
+        S.active = (S.res > S.tol && max_iter > 0) ? 1 : 0;
+    }
+    cg[b] = S;
+}
+
+// pq partials, with the apply's region sums finalised into qv.
+__global__ __launch_bounds__(kThr) void cg_pq_kernel(const RBlock *rb, const CgState *cg, const int *rstart,
+                                                     double *qv, const double *head, const double *tail,
+                                                     const double *pv, double *part) {
+    __shared__ double sh[kThr];
+    const RBlock B = rb[blockIdx.x];
+    if (!cg[B.img].active) return;
+    double t = 0.0;
+    for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr) {
+        const double q = region_value(C, qv, rstart, head, tail);
+        qv[C] = q;
+        t += pv[C] * q;
+    }
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) part[3 * blockIdx.x] = t;
+}
+
+// alpha = rz / pq; x += alpha pv; r -= alpha qv; partials of r^2 dinv, r^2 inv.
+__global__ __launch_bounds__(kThr) void cg_update_kernel(const RBlock *rb, const DImg *di, const CgState *cg,
+                                                         const double *pv, const double *qv, const double *dinv,
+                                                         const double *inv, double *x, double *r, double *part) {
+    __shared__ double sh[kThr];
+    const RBlock B = rb[blockIdx.x];
+    const CgState S = cg[B.img];
+    if (!S.active) return;
+    const DImg &D = di[B.img];
+    const double pq = image_total(part, D.bbase, D.bcount, 0);
+    if (!(pq > 0.0)) return;
+    const double alpha = S.rz / pq;
+    double t0 = 0.0, t1 = 0.0;
+    for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr) {
+        x[C] += alpha * pv[C];
+        r[C] -= alpha * qv[C];
+        t0 += r[C] * r[C] * dinv[C];
+        t1 += r[C] * r[C] * inv[C];
+    }
+    t0 = block_sum(t0, sh);
+    t1 = block_sum(t1, sh);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 1] = t0;
+        part[3 * blockIdx.x + 2] = t1;
+    }
+}
+
+// beta = rzn / rz; pv = r dinv + beta pv; the image's first block writes the
+// next state (tv_iso.cpp:935-941; a non-positive pq stops the image).
+__global__ __launch_bounds__(kThr) void cg_dir_kernel(const RBlock *rb, const DImg *di, const CgState *cur,
+                                                      CgState *nxt, const double *r, const double *dinv, double *pv,
+                                                      const double *part) {
+    const RBlock B = rb[blockIdx.x];
+    const DImg &D = di[B.img];
+    const bool lead = (int)blockIdx.x == D.bbase;
+    CgState S = cur[B.img];
+    if (!S.active) {
+        if (lead && threadIdx.x == 0) nxt[B.img] = S;
+        return;
+    }
+    const double pq = image_total(part, D.bbase, D.bcount, 0);
+    if (!(pq > 0.0)) {
+        if (lead && threadIdx.x == 0) {
+            S.active = 0;
+            nxt[B.img] = S;
+        }
+        return;
+    }
+    const double rzn = image_total(part, D.bbase, D.bcount, 1);
+    const double rn = image_total(part, D.bbase, D.bcount, 2);
+    const double beta = rzn / S.rz;
+    for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr) pv[C] = r[C] * dinv[C] + beta * pv[C];
+    if (lead && threadIdx.x == 0) {
+        S.rz = rzn;
+        S.it += 1;
+        S.res = sqrt(rn / S.bn);
+        S.active = (S.res > S.tol && S.it < S.max_iter) ? 1 : 0;
+        nxt[B.img] = S;
+    }
+}
+
+__global__ void count_active_kernel(int B, const CgState *cg, int *out) {
+    __shared__ int sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    int n = 0;
+    for (int b = threadIdx.x; b < B; b += blockDim.x) n += cg[b].active;
+    if (n) atomicAdd(&sh, n);
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = sh;
+}
+
+// Region values back to the pixels, in the dtype.
+template <class T>
+__global__ __launch_bounds__(kThr) void spread_kernel(int64_t BN, const int *label, const double *xr, T *out) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= BN) return;
+    out[gp] = (T)xr[label[gp]];
+}
+
+// ---------------------------------------------------------------------------
+// JVP right-hand sides (per pixel)
+// ---------------------------------------------------------------------------
+
+// c_A at pixel q (tv_iso.cpp:969-985): H (dW D z) + dr n on non-flat pixels.
+template <class T>
+__device__ void jvp_c(const Geo &G, const DImg &D, const double *gz, const uint8_t *flat, int64_t gq, int b,
+                      int64_t q, const double *dw, double dgam, const T *sw, int64_t sw_img, const double *dsw,
+                      int64_t dsw_img, double *out) {
+    for (int a = 0; a < G.k; ++a) out[a] = 0.0;
+    if (flat[gq] & 1) return;
+    double g[3];
+    for (int a = 0; a < G.k; ++a) {
+        g[a] = gz[a * G.BN + gq];
+        const double wd = D.wa[a] != 0.0 ? g[a] / D.wa[a] : 0.0;
+        out[a] = (dw && D.wa[a] != 0.0) ? D.sc * dw[a] * wd : 0.0;
+    }
+    double nr;
+    const double hs = curv(G, D, g, sw, sw_img, b, q, &nr);
+    hess_apply(G.k, g, hs, out);
+    double dr = dgam;
+    if (sw) dr = dgam * ld(sw, b * sw_img + q) + (dsw ? D.gamma_s * (D.sc * dsw[b * dsw_img + q]) : 0.0);
+    for (int a = 0; a < G.k; ++a) out[a] += dr * g[a] / nr;
+}
+
+struct DwArg {
+    double v[3];
+    int on;
+};
+
+template <class T>
+__global__ __launch_bounds__(kThr) void jvp_rhs_iso_kernel(Geo G, const DImg *di, const double *gz,
+                                                           const uint8_t *flat, const T *dx, const T *u,
+                                                           int64_t u_img, int64_t u_plane, DwArg dw,
+                                                           const double *dgamma, const T *sw, int64_t sw_img,
+                                                           const double *dsw, int64_t dsw_img, double *rhs) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    int64_t c[3];
+    coords(G, p, c);
+    double v = dx ? ld(dx, gp) : 0.0;
+    const double *dwp = dw.on ? dw.v : nullptr;
+    if (dwp)
+        for (int a = 0; a < G.k; ++a) {
+            const int ax = 3 - G.k + a;
+            if (!(dw.v[a] != 0.0 && G.L[ax] > 1)) continue;
+            const double cc = D.sc * dw.v[a];
+            double up = D.use_dual ? ldexp(ld(u, b * u_img + a * u_plane + p), -D.e) : 0.0;
+            if (c[ax] > 0) {
+                const double um = D.use_dual ? ldexp(ld(u, b * u_img + a * u_plane + p - G.S[ax]), -D.e) : 0.0;
+                v += cc * (up - um);
+            } else {
+                v += cc * up;
+            }
+        }
+    const double dg = dgamma ? dgamma[b] : 0.0;
+    double c0[3];
+    jvp_c(G, D, gz, flat, gp, b, p, dwp, dg, sw, sw_img, dsw, dsw_img, c0);
+    for (int a = 0; a < G.k; ++a) {
+        if (D.wa[a] == 0.0) continue;
+        const int ax = 3 - G.k + a;
+        if (c[ax] > 0) {
+            double cm[3];
+            jvp_c(G, D, gz, flat, gp - G.S[ax], b, p - G.S[ax], dwp, dg, sw, sw_img, dsw, dsw_img, cm);
+            v += D.wa[a] * (c0[a] - cm[a]);
+        } else {
+            v += D.wa[a] * c0[a];
+        }
+    }
+    rhs[gp] = v;
+}
+
+// Aniso: r = dx - sum_a dw_a D_a^T u_a - G^T (dr n) (_tv_torch.py:528-547).
+template <class T>
+__global__ __launch_bounds__(kThr) void jvp_rhs_aniso_kernel(Geo G, const DImg *di, const int8_t *nsg, const T *dx,
+                                                             const T *u, int64_t u_img, int64_t u_plane, DwArg dw,
+                                                             const double *dgamma, const T *sw, int64_t sw_img,
+                                                             int64_t sw_plane, const double *dsw, int64_t dsw_img,
+                                                             double *rhs) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    int64_t c[3];
+    coords(G, p, c);
+    const double dg = dgamma ? dgamma[b] : 0.0;
+    auto edge = [&](int a, int64_t q) {  // dw_a u_a[q] + w_a dr_q n_q on the edge (q, q + e_a)
+        double t = dw.on ? dw.v[a] * ld(u, b * u_img + a * u_plane + q) : 0.0;
+        double dr = dg;
+        if (sw) dr = dg * ld(sw, b * sw_img + a * sw_plane + q) +
+                     (dsw ? D.gamma_s * dsw[b * dsw_img + a * G.N + q] : 0.0);
+        return t + D.wa[a] * (dr * (double)nsg[a * G.BN + (int64_t)b * G.N + q]);
+    };
+    double v = dx ? ld(dx, gp) : 0.0;
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        if (c[ax] + 1 < G.L[ax]) v += edge(a, p);
+        if (c[ax] > 0) v -= edge(a, p - G.S[ax]);
+    }
+    rhs[gp] = v;
+}
+
+// ---------------------------------------------------------------------------
+// VJP parameter gradients (per image sums over pixel blocks)
+// ---------------------------------------------------------------------------
+
+// Iso (tv_iso.cpp:1021-1066): q[a] = sum (v[p+e_a] - v[p]) u_a[p] (axes with
+// L > 1), q[k] = the gamma term, q[k+1+a] = sum [H G v]_a (D_a z); the
+// alpha gradient per pixel to gsw_img.
+template <class T>
+__global__ __launch_bounds__(kThr) void vjp_params_iso_kernel(Geo G, const DImg *di, const double *gz,
+                                                              const uint8_t *flat, const int *label, const double *xr,
+                                                              const T *u, int64_t u_img, int64_t u_plane, const T *sw,
+                                                              int64_t sw_img, double *gsw_img, int nblk,
+                                                              double *part) {
+    __shared__ double sh[kThr];
+    const int b = blockIdx.y;
+    const DImg &D = di[b];
+    double q[kMaxQ];
+    for (int i = 0; i < kMaxQ; ++i) q[i] = 0.0;
+    const int64_t p0 = (int64_t)blockIdx.x * kPixBlk;
+    for (int64_t p = p0 + threadIdx.x; p < p0 + kPixBlk && p < G.N; p += kThr) {
+        const int64_t gp = (int64_t)b * G.N + p;
+        int64_t c[3];
+        coords(G, p, c);
+        const double v0 = xr[label[gp]];
+        double gv[3], g[3];
+        for (int a = 0; a < G.k; ++a) {
+            const int ax = 3 - G.k + a;
+            const bool nx = c[ax] + 1 < G.L[ax];
+            const double v1 = nx ? xr[label[gp + G.S[ax]]] : 0.0;
+            if (nx && D.use_dual) q[a] += (v1 - v0) * ldexp(ld(u, b * u_img + a * u_plane + p), -D.e);
+            gv[a] = (D.wa[a] != 0.0 && nx) ? D.wa[a] * (v1 - v0) : 0.0;
+            g[a] = gz[a * G.BN + gp];
+        }
+        if (flat[gp] & 1) {
+            if (gsw_img) gsw_img[gp] = 0.0;
+            continue;
+        }
+        double nr;
+        const double hs = curv(G, D, g, sw, sw_img, b, p, &nr);
+        double t = 0.0;
+        for (int a = 0; a < G.k; ++a) t += gv[a] * g[a] / nr;
+        if (sw) {
+            q[G.k] += ld(sw, b * sw_img + p) * t;
+            if (gsw_img) gsw_img[gp] = -(D.sc * (D.gamma_s * t));
+        } else {
+            q[G.k] += t;
+        }
+        hess_apply(G.k, g, hs, gv);
+        for (int a = 0; a < G.k; ++a)
+            if (D.wa[a] != 0.0) q[G.k + 1 + a] += gv[a] * (g[a] / D.wa[a]);
+    }
+    const int nq = 2 * G.k + 1;
+    for (int i = 0; i < nq; ++i) {
+        const double s = block_sum(q[i], sh);
+        if (threadIdx.x == 0) part[((int64_t)b * nblk + blockIdx.x) * kMaxQ + i] = s;
+    }
+}
+
+// Aniso (_tv_torch.py:563-585): q[a] = sum D_a(h) u_a, q[k] = sum sw w_a D_a(h) n;
+// the edge-weight gradient per edge to gsw_img (k planes per image).
+template <class T>
+__global__ __launch_bounds__(kThr) void vjp_params_aniso_kernel(Geo G, const DImg *di, const int8_t *nsg,
+                                                                const int *label, const double *xr, const T *u,
+                                                                int64_t u_img, int64_t u_plane, const T *sw,
+                                                                int64_t sw_img, int64_t sw_plane, double *gsw_img,
+                                                                int nblk, double *part) {
+    __shared__ double sh[kThr];
+    const int b = blockIdx.y;
+    const DImg &D = di[b];
+    double q[kMaxQ];
+    for (int i = 0; i < kMaxQ; ++i) q[i] = 0.0;
+    const int64_t p0 = (int64_t)blockIdx.x * kPixBlk;
+    for (int64_t p = p0 + threadIdx.x; p < p0 + kPixBlk && p < G.N; p += kThr) {
+        const int64_t gp = (int64_t)b * G.N + p;
+        int64_t c[3];
+        coords(G, p, c);
+        const double v0 = xr[label[gp]];
+        for (int a = 0; a < G.k; ++a) {
+            const int ax = 3 - G.k + a;
+            const bool nx = c[ax] + 1 < G.L[ax];
+            const double dh = nx ? xr[label[gp + G.S[ax]]] - v0 : 0.0;
+            q[a] += dh * ld(u, b * u_img + a * u_plane + p);
+            const double t = D.wa[a] * dh * (double)nsg[a * G.BN + gp];
+            const double swv = sw ? ld(sw, b * sw_img + a * sw_plane + p) : 1.0;
+            q[G.k] += swv * t;
+            if (gsw_img) gsw_img[(int64_t)b * G.k * G.N + a * G.N + p] = -(D.gamma_s * t);
+        }
+    }
+    const int nq = G.k + 1;
+    for (int i = 0; i < nq; ++i) {
+        const double s = block_sum(q[i], sh);
+        if (threadIdx.x == 0) part[((int64_t)b * nblk + blockIdx.x) * kMaxQ + i] = s;
+    }
+}
+
+// Per image: the gamma gradient; the axis weights' gradient summed over the
+// images in order (one thread).
+__global__ void vjp_combine_kernel(int B, int k, int iso, const DImg *di, const double *part, int nblk,
+                                   double *ggamma, double *gw) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double tw[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < B; ++b) {
+        const DImg &D = di[b];
+        double q[kMaxQ];
+        for (int i = 0; i < kMaxQ; ++i) q[i] = 0.0;
+        for (int j = 0; j < nblk; ++j)
+            for (int i = 0; i < kMaxQ; ++i) q[i] += part[((int64_t)b * nblk + j) * kMaxQ + i];
+        if (ggamma) ggamma[b] = -q[k];
+        for (int a = 0; a < k; ++a) {
+            double acc = 0.0;
+            if (iso) {
+                acc -= D.sc * q[a];
+                if (D.wa[a] != 0.0) acc -= D.sc * q[k + 1 + a];
+            } else {
+                acc = -q[a];
+            }
+            tw[a] += acc;
+        }
+    }
+    if (gw)
+        for (int a = 0; a < k; ++a) gw[a] = tw[a];
+}
+
+// A shared map's gradient: the per-image gradients summed in image order.
+__global__ __launch_bounds__(kThr) void sum_images_kernel(int64_t n, int B, const double *src, double *dst) {
+    const int64_t i = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (i >= n) return;
+    double t = 0.0;
+    for (int b = 0; b < B; ++b) t += src[(int64_t)b * n + i];
+    dst[i] = t;
+}
+
+
+int grid_of(int64_t n) { return (int)((n + kThr - 1) / kThr); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct Deriv {
+    DerivSpec sp;
+    Geo G{};
+    int nc = 0;
+    int nrb = 0;   // region blocks
+    int nch = 0;   // region-sum blocks
+    int npb = 0;   // pixel blocks per image
+    std::vector<DImg> hdi;
+    std::vector<void *> bufs;
+    DImg *di = nullptr;
+    RBlock *rb = nullptr;
+    double *gz = nullptr, *hbuf = nullptr, *pix = nullptr;
+    uint8_t *flat = nullptr;
+    int8_t *nsg = nullptr;
+    int *parent = nullptr, *isroot = nullptr, *rank = nullptr, *label = nullptr, *iota = nullptr;
+    int *skey = nullptr, *members = nullptr, *rstart = nullptr, *rbase = nullptr, *cnt = nullptr;
+    double *inv = nullptr, *dinv = nullptr, *cb = nullptr, *xr = nullptr, *rr = nullptr, *pv = nullptr,
+           *qv = nullptr;
+    double *head = nullptr, *tail = nullptr, *part = nullptr, *ppart = nullptr, *gsw_img = nullptr,
+           *dsw_full = nullptr;
+    CgState *cg[2] = {nullptr, nullptr};
+    int *hcount = nullptr;  // pinned
+    CgState *hcg = nullptr; // pinned
+    void *cub = nullptr;
+    size_t cub_bytes = 0;
+    int cur = 0;
+    bool has_result = false;
+
+    ~Deriv() {
+        for (void *p : bufs) (void)hipFree(p);
+        if (hcount) (void)hipHostFree(hcount);
+        if (hcg) (void)hipHostFree(hcg);
+    }
+    template <class P>
+    hipError_t alloc(P **p, size_t bytes) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(bytes, 16));
+        if (e) return e;
+        bufs.push_back(q);
+        *p = (P *)q;
+        return hipSuccess;
+    }
+};
+
+void deriv_free(Deriv *d) { delete d; }
+
+namespace {
+
+int64_t spatial_len(const Deriv &d) { return d.sp.iso ? d.G.N : d.G.k * d.G.N; }
+
+template <class T>
+int build(Deriv &d, hipStream_t s) {
+    const DerivSpec &sp = d.sp;
+    const Geo G = d.G;
+    const int64_t BN = G.BN;
+    const int B = (int)sp.B;
+    const T *x = (const T *)sp.x, *z = (const T *)sp.z, *u = (const T *)sp.u, *sw = (const T *)sp.sw;
+    hipError_t e = hipSuccess;
+#define FPR_A(ptr, bytes) \
+    if ((e = d.alloc(&(ptr), (bytes)))) return FP_ENOMEM
+    FPR_A(d.di, sizeof(DImg) * (size_t)B);
+    FPR_A(d.flat, (size_t)BN);
+    FPR_A(d.parent, 4 * (size_t)BN);
+    FPR_A(d.isroot, 4 * (size_t)BN);
+    FPR_A(d.rank, 4 * (size_t)BN);
+    FPR_A(d.label, 4 * (size_t)BN);
+    FPR_A(d.iota, 4 * (size_t)BN);
+    FPR_A(d.skey, 4 * (size_t)BN);
+    FPR_A(d.members, 4 * (size_t)BN);
+    FPR_A(d.rstart, 4 * ((size_t)BN + 1));
+    FPR_A(d.rbase, 4 * ((size_t)B + 1));
+    FPR_A(d.cnt, 16);
+    FPR_A(d.inv, 8 * (size_t)BN);
+    FPR_A(d.cb, 8 * (size_t)BN);
+    FPR_A(d.xr, 8 * (size_t)BN);
+    FPR_A(d.pix, 8 * (size_t)BN);
+    d.nch = (int)((BN + kChunk - 1) / kChunk);
+    FPR_A(d.head, 8 * (size_t)d.nch);
+    FPR_A(d.tail, 8 * (size_t)d.nch);
+    d.npb = (int)((G.N + kPixBlk - 1) / kPixBlk);
+    FPR_A(d.ppart, 8 * (size_t)kMaxQ * (size_t)B * (size_t)d.npb);
+    if (sp.iso) {
+        FPR_A(d.gz, 8 * (size_t)G.k * (size_t)BN);
+        FPR_A(d.hbuf, 8 * (size_t)G.k * (size_t)BN);
+        FPR_A(d.dinv, 8 * (size_t)BN);
+        FPR_A(d.rr, 8 * (size_t)BN);
+        FPR_A(d.pv, 8 * (size_t)BN);
+        FPR_A(d.qv, 8 * (size_t)BN);
+        FPR_A(d.cg[0], sizeof(CgState) * (size_t)B);
+        FPR_A(d.cg[1], sizeof(CgState) * (size_t)B);
+    } else {
+        FPR_A(d.nsg, (size_t)G.k * (size_t)BN);
+    }
+    if (sp.sw) {
+        FPR_A(d.gsw_img, 8 * (size_t)B * (size_t)spatial_len(d));
+        if (sp.sw_img == 0) FPR_A(d.dsw_full, 8 * (size_t)B * (size_t)spatial_len(d));
+    }
+#undef FPR_A
+    if (hipHostMalloc((void **)&d.hcount, 16) || hipHostMalloc((void **)&d.hcg, sizeof(CgState) * (size_t)B))
+        return FP_ENOMEM;
+    // Per-image constants on the host (tv_iso.cpp:342-402 for the structure's
+    // view of the solve).
+    d.hdi.assign((size_t)B, DImg{});
+    for (int b = 0; b < B; ++b) {
+        DImg &D = d.hdi[(size_t)b];
+        const double gamma = sp.gamma[b];
+        D.e = sp.iso && sp.e ? sp.e[b] : 0;
+        D.sc = ldexp(1.0, D.e);
+        D.gamma_s = ldexp(gamma, -D.e);
+        D.gap_s = sp.iso && sp.gap ? ldexp(sp.gap[b], -2 * D.e) : 0.0;
+        double wmax = 0.0;
+        for (int a = 0; a < G.k; ++a)
+            if (G.L[3 - G.k + a] > 1) wmax = std::max(wmax, sp.w[a]);
+        int na = 0;
+        for (int a = 0; a < 3; ++a) {
+            D.wsave[a] = 0.0;
+            D.wa[a] = 0.0;
+        }
+        for (int a = 0; a < G.k; ++a) {
+            const int64_t La = G.L[3 - G.k + a];
+            if (sp.iso) {
+                if (gamma > 0.0 && sp.w[a] > 0.0 && La > 1 && sp.w[a] >= 1e-100 * wmax) {
+                    D.wsave[a] = sp.w[a];
+                    ++na;
+                }
+                D.wa[a] = (D.gamma_s > 0.0 && sp.w[a] > 0.0 && La > 1) ? sp.w[a] : 0.0;
+            } else {
+                D.wa[a] = sp.w[a];
+            }
+        }
+        D.from_dual = na >= 2;
+        D.use_dual = na >= 1;
+    }
+    if (hipMemcpyAsync(d.di, d.hdi.data(), sizeof(DImg) * (size_t)B, hipMemcpyHostToDevice, s)) return FP_EDEVICE;
+    const double eps = sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
+    hipLaunchKernelGGL(prep_kernel<T>, dim3(B), dim3(kThr), 0, s, G, (int)sp.iso, eps, x, d.di);
+    const int gpx = grid_of(BN);
+    if (sp.iso)
+        hipLaunchKernelGGL(flat_iso_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, x, z, u, sp.u_img, sp.u_plane, sw,
+                           sp.sw_img, d.di, d.gz, d.flat, d.parent);
+    else
+        hipLaunchKernelGGL(flat_aniso_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, x, z, u, sp.u_img, sp.u_plane, sw,
+                           sp.sw_img, sp.sw_plane, d.di, d.nsg, d.flat, d.parent);
+    hipLaunchKernelGGL(union_kernel, dim3(gpx), dim3(kThr), 0, s, G, (int)sp.iso, d.di, d.flat, d.parent);
+    hipLaunchKernelGGL(compress_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.parent, d.isroot);
+    hipLaunchKernelGGL(finish_roots_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.parent);
+    // scratch for the scan and the sort
+    size_t b1 = 0, b2 = 0;
+    void *scan_tmp = nullptr;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.isroot, d.rank, (int)BN, s)) return FP_EDEVICE;
+    if ((e = d.alloc(&scan_tmp, b1))) return FP_ENOMEM;
+    if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, b1, d.isroot, d.rank, (int)BN, s)) return FP_EDEVICE;
+    hipLaunchKernelGGL(label_kernel, dim3(gpx), dim3(kThr), 0, s, G, d.parent, d.rank, d.isroot, d.label, d.iota,
+                       d.rbase);
+    std::vector<int> rb((size_t)B + 1);
+    if (hipMemcpyAsync(rb.data(), d.rbase, 4 * ((size_t)B + 1), hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+    if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    d.nc = rb[(size_t)B];
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) <= (long long)d.nc) ++bits;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, b2, d.label, d.skey, d.iota, d.members, (int)BN, 0, bits, s))
+        return FP_EDEVICE;
+    d.cub_bytes = b2;
+    if ((e = d.alloc(&d.cub, d.cub_bytes))) return FP_ENOMEM;
+    if (hipcub::DeviceRadixSort::SortPairs(d.cub, b2, d.label, d.skey, d.iota, d.members, (int)BN, 0, bits, s))
+        return FP_EDEVICE;
+    hipLaunchKernelGGL(rstart_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.skey, d.rstart);
+    hipLaunchKernelGGL(inv_kernel, dim3(grid_of(d.nc)), dim3(kThr), 0, s, d.nc, d.rstart, d.inv);
+    // region blocks per image
+    std::vector<RBlock> blocks;
+    for (int b = 0; b < B; ++b) {
+        DImg &D = d.hdi[(size_t)b];
+        D.rbase = rb[(size_t)b];
+        D.rcount = rb[(size_t)b + 1] - rb[(size_t)b];
+        D.bbase = (int)blocks.size();
+        for (int c0 = D.rbase; c0 < D.rbase + D.rcount; c0 += kRB)
+            blocks.push_back(RBlock{b, c0, std::min(c0 + kRB, D.rbase + D.rcount), 0});
+        D.bcount = (int)blocks.size() - D.bbase;
+    }
+    d.nrb = (int)blocks.size();
+    if ((e = d.alloc(&d.rb, sizeof(RBlock) * blocks.size()))) return FP_ENOMEM;
+    if ((e = d.alloc(&d.part, 8 * 3 * blocks.size()))) return FP_ENOMEM;
+    if (hipMemcpyAsync(d.rb, blocks.data(), sizeof(RBlock) * blocks.size(), hipMemcpyHostToDevice, s))
+        return FP_EDEVICE;
+    // the region fields of the device copy (after prep's fields are final)
+    std::vector<DImg> dev((size_t)B);
+    if (hipMemcpyAsync(dev.data(), d.di, sizeof(DImg) * (size_t)B, hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+    if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    for (int b = 0; b < B; ++b) {
+        DImg &D = dev[(size_t)b];
+        D.rbase = d.hdi[(size_t)b].rbase;
+        D.rcount = d.hdi[(size_t)b].rcount;
+        D.bbase = d.hdi[(size_t)b].bbase;
+        D.bcount = d.hdi[(size_t)b].bcount;
+    }
+    d.hdi = dev;
+    if (hipMemcpyAsync(d.di, d.hdi.data(), sizeof(DImg) * (size_t)B, hipMemcpyHostToDevice, s)) return FP_EDEVICE;
+    if (sp.iso) {
+        // the Jacobi diagonal, once per structure
+        hipLaunchKernelGGL(jacobi_own_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, d.di, d.gz, d.flat, d.label, sw,
+                           sp.sw_img, d.pix);
+        hipLaunchKernelGGL(rsum_kernel<FPixD>, dim3(d.nch), dim3(kThr), 0, s, FPixD{d.pix}, d.skey, d.members,
+                           d.rstart, BN, d.cb, d.head, d.tail);
+        hipLaunchKernelGGL(dinv_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.rstart, d.cb, d.head, d.tail, d.inv,
+                           d.dinv);
+    }
+    if (hipGetLastError()) return FP_EDEVICE;
+    return FP_OK;
+}
+
+// Solves S c = cb (region space) for every image (tv_iso.cpp:899-943).
+int run_cg(Deriv &d, hipStream_t s, double tol, int max_iter) {
+    const Geo G = d.G;
+    const int B = (int)d.sp.B;
+    hipLaunchKernelGGL(cg_init_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.rstart, d.cb, d.head, d.tail, d.inv,
+                       d.dinv, d.xr, d.rr, d.pv, d.part);
+    hipLaunchKernelGGL(cg_start_kernel, dim3((B + 63) / 64), dim3(64), 0, s, B, d.di, d.part, tol, max_iter,
+                       d.cg[0]);
+    d.cur = 0;
+    const int gpx = grid_of(G.BN);
+    for (int done = 0; done < max_iter;) {
+        for (int j = 0; j < kCheck && done < max_iter; ++j, ++done) {
+            CgState *c0 = d.cg[d.cur], *c1 = d.cg[d.cur ^ 1];
+            if (d.sp.tsize == 4)
+                hipLaunchKernelGGL(hgv_kernel<float>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
+                                   d.label, d.pv, (const float *)d.sp.sw, d.sp.sw_img, d.hbuf);
+            else
+                hipLaunchKernelGGL(hgv_kernel<double>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
+                                   d.label, d.pv, (const double *)d.sp.sw, d.sp.sw_img, d.hbuf);
+            hipLaunchKernelGGL(rsum_kernel<FApply>, dim3(d.nch), dim3(kThr), 0, s, FApply{G, d.pv, d.hbuf, d.di, c0},
+                               d.skey, d.members, d.rstart, G.BN, d.qv, d.head, d.tail);
+            hipLaunchKernelGGL(cg_pq_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, c0, d.rstart, d.qv, d.head, d.tail,
+                               d.pv, d.part);
+            hipLaunchKernelGGL(cg_update_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.di, c0, d.pv, d.qv, d.dinv,
+                               d.inv, d.xr, d.rr, d.part);
+            hipLaunchKernelGGL(cg_dir_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.di, c0, c1, d.rr, d.dinv, d.pv,
+                               d.part);
+            d.cur ^= 1;
+        }
+        hipLaunchKernelGGL(count_active_kernel, dim3(1), dim3(256), 0, s, B, d.cg[d.cur], d.cnt);
+        if (hipMemcpyAsync(d.hcount, d.cnt, 4, hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+        if (*d.hcount == 0) break;
+    }
+    if (hipMemcpyAsync(d.hcg, d.cg[d.cur], sizeof(CgState) * (size_t)B, hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
+    if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    d.has_result = true;
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+template <class T>
+int vjp_t(Deriv &d, hipStream_t s, const void *gin, void *gxo, double *ggamma, double *gw, double *gsw, double tol,
+          int max_iter) {
+    const DerivSpec &sp = d.sp;
+    const Geo G = d.G;
+    const int B = (int)sp.B;
+    const T *u = (const T *)sp.u, *sw = (const T *)sp.sw;
+    hipLaunchKernelGGL(rsum_kernel<FPix<T>>, dim3(d.nch), dim3(kThr), 0, s, FPix<T>{(const T *)gin}, d.skey,
+                       d.members, d.rstart, G.BN, d.cb, d.head, d.tail);
+    if (sp.iso) {
+        if (int r = run_cg(d, s, tol, max_iter)) return r;
+    } else {
+        hipLaunchKernelGGL(mean_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.rstart, d.cb, d.head, d.tail, d.inv,
+                           d.xr);
+        d.has_result = false;
+    }
+    hipLaunchKernelGGL(spread_kernel<T>, dim3(grid_of(G.BN)), dim3(kThr), 0, s, G.BN, d.label, d.xr, (T *)gxo);
+    double *gsi = (gsw && sp.sw) ? d.gsw_img : nullptr;
+    if (sp.iso)
+        hipLaunchKernelGGL(vjp_params_iso_kernel<T>, dim3(d.npb, B), dim3(kThr), 0, s, G, d.di, d.gz, d.flat,
+                           d.label, d.xr, u, sp.u_img, sp.u_plane, sw, sp.sw_img, gsi, d.npb, d.ppart);
+    else
+        hipLaunchKernelGGL(vjp_params_aniso_kernel<T>, dim3(d.npb, B), dim3(kThr), 0, s, G, d.di, d.nsg, d.label,
+                           d.xr, u, sp.u_img, sp.u_plane, sw, sp.sw_img, sp.sw_plane, gsi, d.npb, d.ppart);
+    hipLaunchKernelGGL(vjp_combine_kernel, dim3(1), dim3(64), 0, s, B, G.k, (int)sp.iso, d.di, d.ppart, d.npb,
+                       ggamma, gw);
+    if (gsi) {
+        const int64_t n = spatial_len(d);
+        if (sp.sw_img == 0)
+            hipLaunchKernelGGL(sum_images_kernel, dim3(grid_of(n)), dim3(kThr), 0, s, n, B, gsi, gsw);
+        else if (hipMemcpyAsync(gsw, gsi, 8 * (size_t)B * (size_t)n, hipMemcpyDeviceToDevice, s))
+            return FP_EDEVICE;
+    } else if (gsw && !sp.sw) {
+        return FP_EINVAL;
+    }
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+__global__ __launch_bounds__(kThr) void broadcast_kernel(int64_t n, int B, const double *src, double *dst) {
+    const int64_t i = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (i >= n * B) return;
+    dst[i] = src[i % n];
+}
+
+template <class T>
+int jvp_t(Deriv &d, hipStream_t s, const void *dx, const double *dgamma, const double *dw, const double *dsw,
+          void *dz, double tol, int max_iter) {
+    const DerivSpec &sp = d.sp;
+    const Geo G = d.G;
+    const int B = (int)sp.B;
+    const T *u = (const T *)sp.u, *sw = (const T *)sp.sw;
+    DwArg dwa{{0.0, 0.0, 0.0}, dw ? 1 : 0};
+    if (dw)
+        for (int a = 0; a < G.k; ++a) dwa.v[a] = dw[a];
+    const int64_t n = spatial_len(d);
+    const double *dsw_i = dsw;
+    int64_t dsw_img = n;
+    if (dsw && sp.sw && sp.sw_img == 0) {  // a shared map's tangent, the same for every image
+        hipLaunchKernelGGL(broadcast_kernel, dim3(grid_of(n * B)), dim3(kThr), 0, s, n, B, dsw, d.dsw_full);
+        dsw_i = d.dsw_full;
+    }
+    if (dsw && !sp.sw) return FP_EINVAL;
+    if (sp.iso)
+        hipLaunchKernelGGL(jvp_rhs_iso_kernel<T>, dim3(grid_of(G.BN)), dim3(kThr), 0, s, G, d.di, d.gz, d.flat,
+                           (const T *)dx, u, sp.u_img, sp.u_plane, dwa, dgamma, sw, sp.sw_img, dsw_i, dsw_img, d.pix);
+    else
+        hipLaunchKernelGGL(jvp_rhs_aniso_kernel<T>, dim3(grid_of(G.BN)), dim3(kThr), 0, s, G, d.di, d.nsg,
+                           (const T *)dx, u, sp.u_img, sp.u_plane, dwa, dgamma, sw, sp.sw_img, sp.sw_plane, dsw_i,
+                           dsw_img, d.pix);
+    hipLaunchKernelGGL(rsum_kernel<FPixD>, dim3(d.nch), dim3(kThr), 0, s, FPixD{d.pix}, d.skey, d.members, d.rstart,
+                       G.BN, d.cb, d.head, d.tail);
+    if (sp.iso) {
+        if (int r = run_cg(d, s, tol, max_iter)) return r;
+    } else {
+        hipLaunchKernelGGL(mean_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.rstart, d.cb, d.head, d.tail, d.inv,
+                           d.xr);
+        d.has_result = false;
+    }
+    hipLaunchKernelGGL(spread_kernel<T>, dim3(grid_of(G.BN)), dim3(kThr), 0, s, G.BN, d.label, d.xr, (T *)dz);
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+}  // namespace
+
+int deriv_create(const DerivSpec &spec, hipStream_t s, Deriv **out) {
+    *out = nullptr;
+    if (spec.B <= 0 || (spec.k != 2 && spec.k != 3) || !spec.x || !spec.z || !spec.gamma) return FP_EINVAL;
+    if (spec.iso && !spec.gap) return FP_EINVAL;
+    if (!spec.u) return FP_EINVAL;
+    Deriv *d = new (std::nothrow) Deriv();
+    if (!d) return FP_ENOMEM;
+    d->sp = spec;
+    Geo &G = d->G;
+    G.k = spec.k;
+    for (int a = 0; a < 3; ++a) G.L[a] = spec.L[a];
+    G.S[2] = 1;
+    G.S[1] = G.L[2];
+    G.S[0] = G.L[1] * G.L[2];
+    G.N = G.L[0] * G.L[1] * G.L[2];
+    G.BN = G.N * spec.B;
+    if (G.BN >= (int64_t)1 << 31 || G.N <= 0) {
+        delete d;
+        return FP_ESHAPE;
+    }
+    const int r = spec.tsize == 4 ? build<float>(*d, s) : build<double>(*d, s);
+    if (r) {
+        delete d;
+        return r;
+    }
+    *out = d;
+    return FP_OK;
+}
+
+int deriv_vjp(Deriv *d, hipStream_t s, const void *g, void *gx, double *ggamma, double *gw, double *gsw, double tol,
+              int max_iter) {
+    if (!d || !g || !gx) return FP_EINVAL;
+    return d->sp.tsize == 4 ? vjp_t<float>(*d, s, g, gx, ggamma, gw, gsw, tol, max_iter)
+                            : vjp_t<double>(*d, s, g, gx, ggamma, gw, gsw, tol, max_iter);
+}
+
+int deriv_jvp(Deriv *d, hipStream_t s, const void *dx, const double *dgamma, const double *dw, const double *dsw,
+              void *dz, double tol, int max_iter) {
+    if (!d || !dz) return FP_EINVAL;
+    return d->sp.tsize == 4 ? jvp_t<float>(*d, s, dx, dgamma, dw, dsw, dz, tol, max_iter)
+                            : jvp_t<double>(*d, s, dx, dgamma, dw, dsw, dz, tol, max_iter);
+}
+
+void deriv_image_result(const Deriv *d, double *residual, int *iters) {
+    const int B = (int)d->sp.B;
+    for (int b = 0; b < B; ++b) {
+        if (residual) residual[b] = d->has_result ? d->hcg[b].res : 0.0;
+        if (iters) iters[b] = d->has_result ? d->hcg[b].it : 0;
+    }
+}
+
+void deriv_result(const Deriv *d, double *residual, int *iters, int64_t *regions) {
+    double r = 0.0;
+    int it = 0;
+    if (d->has_result)
+        for (int b = 0; b < (int)d->sp.B; ++b) {
+            r = std::max(r, d->hcg[b].res);
+            it = std::max(it, d->hcg[b].it);
+        }
+    if (residual) *residual = r;
+    if (iters) *iters = it;
+    if (regions) *regions = d->nc;
+}
+
+}  // namespace fpr

@@ -1,17 +1,66 @@
-// Implicit-differentiation derivatives of image TV on the device (deriv.cpp).
+// Implicit-differentiation derivatives of image TV on the device (deriv.hip):
+// the structure at a solution (flat pixels or edges, their connected regions)
+// and the JVP and VJP of a uniform batch of images.
 //
 // SPDX-License-Identifier: MIT
 #pragma once
 
-#include "../../include/fastprox_rocm.h"
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 namespace fpr {
 
 struct Deriv;
 
-fp_status jvp_impl(const fp_op *op, fp_workspace *ws, const fp_array *dx, double dgamma, const fp_array *dparams,
-                   fp_array *dout);
-fp_status vjp_impl(const fp_op *op, fp_workspace *ws, const fp_array *g, fp_array *gx, double *ggamma_host,
-                   fp_array *ggamma_dev, fp_array *gparams);
+// A solved batch of B images of one shape, as the derivatives read it. Every
+// pointer is device memory of the solve's dtype (tsize bytes) unless noted.
+struct DerivSpec {
+    bool iso = true;
+    int k = 2;
+    int64_t L[3] = {1, 1, 1};   // extents, padded in front with 1 (L[2] contiguous)
+    int64_t B = 0;
+    int tsize = 8;
+    double w[3] = {1.0, 1.0, 1.0};  // the operator's axis weights (first k)
+    const void *x = nullptr;    // the inputs, B contiguous images
+    const void *z = nullptr;    // the outputs
+    // The dual: iso, u at the input's scale (|u_p| <= gamma alpha_p); aniso,
+    // the per-edge dual (|u_e| <= gamma beta_e). Image b's plane a at
+    // u + b u_img + a u_plane.
+    const void *u = nullptr;
+    int64_t u_img = 0, u_plane = 0;
+    // Spatial weights (iso alpha: one plane; aniso beta: k planes) or null;
+    // sw_img = 0: one map shared by every image.
+    const void *sw = nullptr;
+    int64_t sw_img = 0, sw_plane = 0;
+    const double *gamma = nullptr;  // host, B: the steps as given
+    const double *gap = nullptr;    // host, B: the gaps at the input's scale (iso)
+    const int *e = nullptr;         // host, B: the solves' scale exponents (iso), or null for 0
+};
+
+// Builds the structure (one host synchronisation); an fp_status.
+int deriv_create(const DerivSpec &spec, hipStream_t s, Deriv **out);
+void deriv_free(Deriv *d);
+
+// The VJP of the cotangent g (B images): gx (B images); per-image gradients
+// in gamma (device, B doubles) or null; the axis weights' gradient summed over
+// the images (device, k doubles) or null; the spatial weights' gradient
+// (device doubles shaped like the weights: per image, or summed over the
+// images for a shared map) or null. tol 0: the default tolerance.
+int deriv_vjp(Deriv *d, hipStream_t s, const void *g, void *gx, double *ggamma, double *gw, double *gsw, double tol,
+              int max_iter);
+
+// The JVP along dx (B images, or null for 0), per-image dgamma (device, B
+// doubles, or null), dw (host, k doubles, or null) and dsw (device doubles
+// shaped like the spatial weights, or null): dz (B images).
+int deriv_jvp(Deriv *d, hipStream_t s, const void *dx, const double *dgamma, const double *dw, const double *dsw,
+              void *dz, double tol, int max_iter);
+
+// The last derivative's largest CG residual and iteration count over the
+// images, and the number of regions of the structure. Waits for the stream.
+void deriv_result(const Deriv *d, double *residual, int *iters, int64_t *regions);
+
+// Per-image results of the last derivative (host arrays of B).
+void deriv_image_result(const Deriv *d, double *residual, int *iters);
 
 }  // namespace fpr

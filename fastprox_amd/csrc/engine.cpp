@@ -245,4 +245,14 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
     return hipStreamSynchronize(s);
 }
 
+int Engine::copy_dual_iterate(hipStream_t s, void *dst, int64_t img_stride) const {
+    if (!arena_) return hipErrorInvalidValue;
+    if (tsize_ == 4)
+        launch_copy_u<float>((const float *)arena_, desc_, state_, nimg(), P_.k, maxN_, (float *)dst, img_stride, s);
+    else
+        launch_copy_u<double>((const double *)arena_, desc_, state_, nimg(), P_.k, maxN_, (double *)dst, img_stride,
+                              s);
+    return hipGetLastError();
+}
+
 }  // namespace fpr

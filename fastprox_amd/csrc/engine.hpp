@@ -53,6 +53,10 @@ public:
     // arrays of nimg()). Blocks until done; results in state().
     int solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
 
+    // Copies every image's dual iterate u (k planes of N entries, the solver's
+    // own scaling) to dst + i * img_stride elements, after a solve.
+    int copy_dual_iterate(hipStream_t s, void *dst, int64_t img_stride) const;
+
     const ImgState *state() const { return hstate_; }
     ImgState *device_state() const { return state_; }
     int nimg() const { return (int)specs_.size(); }

@@ -1693,6 +1693,30 @@ bool kernel_available(int dtype_bytes, bool stream, int TT, int RY) {
     return TT == 2 && RY == 16;
 }
 
+// The dual iterate u (k planes) of every image out of the state arena, to
+// dst + i * img_stride (zero for an image whose state was never written).
+template <class T>
+__global__ __launch_bounds__(256) void copy_u_kernel(const T *arena, const ImgDesc *desc, const ImgState *state,
+                                                     int k, T *dst, int64_t img_stride) {
+    const int img = blockIdx.y;
+    const ImgDesc &D = desc[img];
+    const ImgState &S = state[img];
+    const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
+    const T *U = arena + D.st_off + (int64_t)S.cur * 2 * k * N;
+    T *out = dst + img * img_stride;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)k * N;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = S.fresh ? (T)0 : U[i];
+}
+
+template <class T>
+void launch_copy_u(const T *arena, const ImgDesc *desc, const ImgState *state, int nimg, int k, int64_t maxN, T *dst,
+                   int64_t img_stride, hipStream_t s) {
+    if (nimg <= 0) return;
+    hipLaunchKernelGGL(copy_u_kernel<T>, dim3(grid_for(k * maxN, 256, 1024), nimg), dim3(256), 0, s, arena, desc,
+                       state, k, dst, img_stride);
+}
+
 template <class T>
 void launch_gather(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s) {
     hipLaunchKernelGGL(strided_copy_kernel<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n, nd, sh, 1);
@@ -1714,7 +1738,9 @@ void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
                                  hipStream_t);                                                  \
     template void launch_step<T>(const KernelChoice &, bool, bool, const StepArgs<T> &, int, hipStream_t);       \
     template void launch_gather<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);            \
-    template void launch_scatter<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);
+    template void launch_scatter<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);            \
+    template void launch_copy_u<T>(const T *, const ImgDesc *, const ImgState *, int, int, int64_t, T *, int64_t,  \
+                                   hipStream_t);
 FPR_INST(float)
 FPR_INST(double)
 #undef FPR_INST

@@ -55,6 +55,9 @@ template <class T>
 void launch_gather(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s);
 template <class T>
 void launch_scatter(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s);
+template <class T>
+void launch_copy_u(const T *arena, const ImgDesc *desc, const ImgState *state, int nimg, int k, int64_t maxN, T *dst,
+                   int64_t img_stride, hipStream_t s);
 template <class S, class D>
 void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s);
 

@@ -253,6 +253,10 @@ def _declare_batch(L):
     L.fpr_batch_set_timing.argtypes = [P, C.c_int]
     L.fpr_batch_timing.restype = C.c_int
     L.fpr_batch_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.fpr_batch_scale_exps.restype = C.c_int
+    L.fpr_batch_scale_exps.argtypes = [P, C.POINTER(C.c_int), C.c_int64]
+    L.fpr_batch_copy_dual_iterate.restype = C.c_int
+    L.fpr_batch_copy_dual_iterate.argtypes = [P, P, P]
     L._batch_declared = True
 
 
@@ -310,6 +314,20 @@ class Batch:
         _check(lib().fpr_batch_stats(self.h, C.byref(a), C.byref(b)), "fpr_batch_stats")
         return a.value, b.value
 
+    def scale_exps(self):
+        e = np.zeros(len(self.shapes), np.int32)
+        _check(lib().fpr_batch_scale_exps(self.h, e.ctypes.data_as(C.POINTER(C.c_int)), len(self.shapes)),
+               "fpr_batch_scale_exps")
+        return e
+
+    def copy_dual_iterate(self, dst, stream=None):
+        """The per-edge (or per-pixel) dual iterate of every image into dst,
+        a contiguous (B, k, *img) device tensor of the batch's dtype."""
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(lib().fpr_batch_copy_dual_iterate(self.h, C.c_void_p(dst.data_ptr()), C.c_void_p(stream or None)),
+               "fpr_batch_copy_dual_iterate")
+
     def set_timing(self, on):
         _check(lib().fpr_batch_set_timing(self.h, int(on)), "fpr_batch_set_timing")
 
@@ -347,10 +365,12 @@ def _batch_for(iso, k, w, dtype, device, B, img, weighted):
         return hit
 
 
-def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None):
+def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None, keep=False):
     """(z, dual (k, B, *img) in the C form, gaps, iters, zerrs, converged)
     for x (B, *img) on the GPU; gamma (B,); warm: a dual (k, B, *img) or None;
-    sw: pixel weights (B, *img) or edge weights (k, B, *img)."""
+    sw: pixel weights (B, *img) or edge weights (k, B, *img). keep: also the
+    scale exponents and, anisotropic, the per-edge dual (B, k, *img) that the
+    derivatives read."""
     k = len(w)
     B = x.shape[0]
     img = tuple(x.shape[1:])
@@ -370,7 +390,15 @@ def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None):
         bt.solve(list(xb), list(z), list(dual), g, weights=wlist, tol=tol, max_iter=max_iter,
                  warm=warm is not None)
     gaps, its, zerrs, conv = bt.image_info()
-    return z, dual.transpose(0, 1), gaps, its, zerrs, conv
+    if not keep:
+        return z, dual.transpose(0, 1), gaps, its, zerrs, conv
+    exps = bt.scale_exps()
+    uedge = None
+    if not iso:
+        uedge = torch.empty((B, k) + img, dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            bt.copy_dual_iterate(uedge)
+    return z, dual.transpose(0, 1), gaps, its, zerrs, conv, exps, uedge
 
 
 class _DeviceTV(torch.autograd.Function):
@@ -383,13 +411,14 @@ class _DeviceTV(torch.autograd.Function):
         img = tuple(x.shape[-k:])
         xb = x.detach().reshape((-1,) + img)
         wb = None if warm is None else warm.reshape((k, -1) + img)
-        z, dual, gaps, its, zerrs, conv = solve(xb, gamma.detach(), cfg.w, cfg.iso, cfg.tol, cfg.max_iter, wb,
-                                                None if sw is None else sw.detach())
+        z, dual, gaps, its, zerrs, conv, exps, uedge = solve(xb, gamma.detach(), cfg.w, cfg.iso, cfg.tol,
+                                                             cfg.max_iter, wb, None if sw is None else sw.detach(),
+                                                             keep=True)
         cfg.info = IterInfo(float(np.sum(gaps)), int(its.max()) if its.size else 0, 0, bool(conv.all()),
                             dual.reshape((k,) + tuple(x.shape)), float(np.max(zerrs)) if zerrs.size else 0.0,
                             gaps, its)
         ctx.cfg, ctx.xb, ctx.z, ctx.dual, ctx.gam = cfg, xb, z, dual, gamma.detach()
-        ctx.gaps = gaps
+        ctx.gaps, ctx.exps, ctx.uedge = gaps, exps, uedge
         ctx.shape = tuple(x.shape)
         ctx.wt = wt
         ctx.sw = None if sw is None else sw.detach()
@@ -401,7 +430,8 @@ class _DeviceTV(torch.autograd.Function):
         if ctx.S is None:
             from .deriv import Structure
             cfg = ctx.cfg
-            ctx.S = Structure(ctx.xb, ctx.z, ctx.dual, ctx.gaps, ctx.gam, cfg.w, cfg.iso, ctx.sw)
+            ctx.S = Structure(ctx.xb, ctx.z, ctx.dual if cfg.iso else ctx.uedge, ctx.gaps, ctx.exps, ctx.gam, cfg.w,
+                              cfg.iso, ctx.sw)
         return ctx.S
 
     @staticmethod

@@ -223,6 +223,52 @@ FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_sy
  * step launches timed. */
 FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on);
 FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches);
+/* The last solve's power-of-two scale exponent per image (isotropic solves
+ * far from unit scale run on 2^-e x; src/ops/tv_iso.cpp:367-377). */
+FP_API fp_status fpr_batch_scale_exps(const fpr_batch *b, int *exps, int64_t count);
+/* Copies the last solve's dual iterate u of every image to dst (device, the
+ * batch's dtype): image i's ndim planes at dst + i ndim N. Anisotropic
+ * batches: the per-edge dual |u_e| <= gamma beta_e that the derivatives read
+ * (the C form of fpr_image.dual sums it along each axis). Uniform batches only
+ * (FP_ESHAPE otherwise). Enqueued on stream; does not wait. */
+FP_API fp_status fpr_batch_copy_dual_iterate(const fpr_batch *b, void *dst, void *stream);
+
+/* Derivatives of a solved uniform batch, the device counterpart of the
+ * keep_structure workspace for callers that hold the solution themselves
+ * (the PyTorch wrapper's autograd; _tv_torch.py:286-585 Structure):
+ *   x, z: the batch's inputs and outputs, B contiguous images of img_shape;
+ *   dual: (B, ndim, *img) contiguous: isotropic, fpr_image.dual of the solve;
+ *         anisotropic, fpr_batch_copy_dual_iterate's per-edge dual;
+ *   spatial: pixel weights (B, *img), or edge weights (B, ndim, *img), or one
+ *         map shared by every image (spatial_per_image = 0), or NULL;
+ *   gammas, gaps (host, B): the solve's steps and gaps; scale_exps (host, B,
+ *         or NULL for 0): fpr_batch_scale_exps.
+ * The device arrays must outlive the handle. Creating it builds the structure
+ * (the flat pixels or edges and their regions) on stream and waits. */
+typedef struct fpr_deriv fpr_deriv;
+FP_API fp_status fpr_deriv_create(const fp_op *op, fp_dtype dtype, int64_t count, const int64_t *img_shape,
+                                  const void *x, const void *z, const void *dual, const void *spatial,
+                                  int spatial_per_image, const double *gammas, const double *gaps,
+                                  const int *scale_exps, void *stream, fpr_deriv **out);
+FP_API void      fpr_deriv_free(fpr_deriv *d);
+/* VJP of the cotangent g (device, B images): gx (device, B images); ggamma
+ * (device, B doubles: one per image) or NULL; gw (host, ndim doubles: the axis
+ * weights' gradient summed over the images) or NULL; gspatial (device
+ * doubles shaped like the spatial weights; summed over the images for a
+ * shared map) or NULL. tol 0: the default tolerance (the relative error of z
+ * that each image's gap certifies, at least 1e-10). Waits for the stream. */
+FP_API fp_status fpr_deriv_vjp(fpr_deriv *d, const void *g, void *gx, double *ggamma, double *gw, double *gspatial,
+                               double tol, int max_iter, void *stream);
+/* JVP along dx (device, B images, or NULL), dgamma (device, B doubles, or
+ * NULL), dw (host, ndim doubles, or NULL), dspatial (device doubles shaped
+ * like the spatial weights, or NULL): dz (device, B images). Waits. */
+FP_API fp_status fpr_deriv_jvp(fpr_deriv *d, const void *dx, const double *dgamma, const double *dw,
+                               const double *dspatial, void *dz, double tol, int max_iter, void *stream);
+/* The last derivative's largest CG residual and iterations over the images
+ * (0 for anisotropic TV: the region means are exact), and the number of
+ * regions of the structure. */
+FP_API fp_status fpr_deriv_info(const fpr_deriv *d, double *residual, int *iters, int64_t *regions);
+FP_API fp_status fpr_deriv_image_info(const fpr_deriv *d, double *residuals, int *iters, int64_t count);
 
 #ifdef __cplusplus
 }
