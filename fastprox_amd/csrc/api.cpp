@@ -24,6 +24,7 @@
 #include "deriv.hpp"
 #include "engine.hpp"
 #include "ws.hpp"
+#include "serve.hpp"
 
 #ifndef FASTPROX_ROCM_BUILD
 #define FASTPROX_ROCM_BUILD "1.0.6"
@@ -1011,6 +1012,12 @@ struct fpr_batch {
     bool valid = false;
 };
 
+struct fpr_server {
+    std::unique_ptr<fpr::Server> s;
+    int device = 0;
+    int tsize = 4;
+};
+
 struct fpr_deriv {
     std::unique_ptr<fpr::Deriv, fpr::DerivDeleter> d;
     int device = 0;
@@ -1533,6 +1540,104 @@ FP_API fp_status fpr_deriv_image_info(const fpr_deriv *d, double *residuals, int
     if (!d) return FP_EINVAL;
     if (count != d->B) return FP_ESHAPE;
     fpr::deriv_image_result(d->d.get(), residuals, iters);
+    return FP_OK;
+}
+
+FP_API fp_status fpr_server_create(const fp_op *op2d, const fp_op *op3d, fp_dtype dtype, size_t pool_bytes,
+                                   int slots, fpr_server **out) {
+    return guard([&] {
+        if (!out) return FP_EINVAL;
+        *out = nullptr;
+        if ((!op2d && !op3d) || slots < 1 || pool_bytes < ((size_t)1 << 20)) return FP_EINVAL;
+        if ((op2d && op2d->k != 2) || (op3d && op3d->k != 3)) return FP_EINVAL;
+        if ((op2d && !op2d->spatial.empty()) || (op3d && !op3d->spatial.empty())) return FP_EINVAL;
+        const int dt = enum_bits(dtype);
+        if (dt != FP_F32 && dt != FP_F64) return FP_EDTYPE;
+        const int tsize = dt == FP_F32 ? 4 : 8;
+        fpr::Problem probs[2] = {};
+        fpr::KernelChoice kcs[2];
+        const fp_op *ops[2] = {op2d, op3d};
+        for (int c = 0; c < 2; ++c) {
+            const fp_op *op = ops[c];
+            if (!op) continue;
+            fpr::Problem &P = probs[c];
+            P.k = op->k;
+            P.iso = op->iso ? 1 : 0;
+            P.sw = 0;
+            P.sw_per_image = 0;
+            for (int a = 0; a < 3; ++a) P.w[a] = op->w[a];
+            P.sched = op->iso ? fpr::kSchedC : fpr::kSchedDevice;
+            P.wmax_norm = op->iso ? 1 : 0;
+            P.alpha_max = 1.0;
+            const int64_t img[3] = {op->k == 3 ? 128 : 1, 512, 512};
+            kcs[c] = fpr::default_kernel(op->k, tsize, img, op->iso, false, 64);
+        }
+        std::unique_ptr<fpr_server> sv(new fpr_server());
+        sv->tsize = tsize;
+        if (hipGetDevice(&sv->device)) return FP_EDEVICE;
+        sv->s.reset(new fpr::Server());
+        if (int e = sv->s->setup(probs, kcs, tsize, slots, pool_bytes)) return hip_status(e);
+        *out = sv.release();
+        return FP_OK;
+    });
+}
+
+FP_API void fpr_server_free(fpr_server *s) { delete s; }
+
+FP_API fp_status fpr_server_submit(fpr_server *s, const fpr_request *r, double t_submit, int64_t *id) {
+    return guard([&] {
+        if (!s || !r) return FP_EINVAL;
+        fpr::ServeRequest q;
+        q.k = r->ndim;
+        if (q.k != 2 && q.k != 3) return FP_EINVAL;
+        for (int a = 0; a < q.k; ++a) {
+            q.L[3 - q.k + a] = r->shape[a];
+            q.xs[3 - q.k + a] = r->strides ? r->strides[a] : 0;
+        }
+        if (r->strides && q.k == 2) q.xs[0] = 0;
+        q.x = r->x;
+        q.z = r->z;
+        q.gamma = r->gamma;
+        q.tol = r->tol;
+        q.max_iter = r->max_iter;
+        q.keep_dual = r->keep_dual;
+        q.warm_from = r->warm_from;
+        q.t_submit = t_submit;
+        DevGuard dg(s->device);
+        return (fp_status)s->s->submit(q, id);
+    });
+}
+
+FP_API fp_status fpr_server_step(fpr_server *s, int *busy, int *queued) {
+    return guard([&] {
+        if (!s) return FP_EINVAL;
+        DevGuard dg(s->device);
+        const int e = s->s->step(busy, queued);
+        return e ? hip_status(e) : FP_OK;
+    });
+}
+
+FP_API int64_t fpr_server_poll(fpr_server *s, fpr_result *out, int64_t max) {
+    if (!s || !out || max <= 0) return 0;
+    std::vector<fpr::ServeResult> tmp((size_t)max);
+    const int64_t n = s->s->poll(tmp.data(), max);
+    for (int64_t i = 0; i < n; ++i) {
+        const fpr::ServeResult &R = tmp[(size_t)i];
+        out[i] = fpr_result{R.id, R.status, R.converged, R.iters, R.gap, R.zerr, R.t_submit, R.t_admit, R.t_done};
+    }
+    return n;
+}
+
+FP_API double fpr_server_clock(const fpr_server *s) { return s ? s->s->now() : 0.0; }
+
+FP_API fp_status fpr_server_release(fpr_server *s, int64_t id) {
+    if (!s) return FP_EINVAL;
+    return s->s->release_dual(id) ? FP_OK : FP_EINVAL;
+}
+
+FP_API fp_status fpr_server_stats(const fpr_server *s, int64_t *launches) {
+    if (!s) return FP_EINVAL;
+    if (launches) *launches = s->s->launches();
     return FP_OK;
 }
 

@@ -1,0 +1,459 @@
+// A serving loop for image-TV requests: continuous batching on one GPU.
+//
+// Every request class (2D, 3D) owns a fixed table of slots, one device pool
+// of the dtype for the requests' buffers, a pool of tiles for the per-tile
+// partial sums, and a stream. The step and epilogue kernels of the solve
+// engine run over the whole slot table: an empty or finished slot is in mode
+// kDone and gets no tiles, so one launch serves every live request of the
+// class whatever its size, step, tolerance or age. A round of the loop
+//   1. admits queued requests, first come first served, while a slot, the
+//      request's memory (x, z and the state: (2 + 4k) N entries; the dual:
+//      k N) and its tiles fit, evicting kept duals (least recently used)
+//      when memory is short: the set-up (input copy, the prologue, a warm
+//      start) is enqueued on the class's stream and the request's tiles are
+//      appended to the list that the next step launch reads;
+//   2. enqueues a chunk of step/epilogue launch pairs and a snapshot of the
+//      slot states, keeping two chunks in flight so the GPU never waits on
+//      the host;
+//   3. waits for the older chunk and retires the requests its snapshot shows
+//      done: their output is copied out, their memory and tiles return to
+//      the pools, their dual stays resident when asked (a warm start for a
+//      later request of the same shape).
+// A request's result depends only on its own data: the kernels compute each
+// image's tiles and fixed-order reductions independently of the others, so
+// it is the same alone and in any mix.
+//
+// SPDX-License-Identifier: MIT
+#include "serve.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/fastprox_rocm.h"
+
+namespace fpr {
+
+// ---------------------------------------------------------------------------
+// RangeAlloc
+// ---------------------------------------------------------------------------
+
+void RangeAlloc::reset(int64_t capacity) {
+    free_.clear();
+    if (capacity > 0) free_[0] = capacity;
+    free_total_ = capacity;
+}
+
+int64_t RangeAlloc::alloc(int64_t n) {
+    if (n <= 0) return -1;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < n) continue;
+        const int64_t off = it->first, rest = it->second - n;
+        free_.erase(it);
+        if (rest) free_[off + n] = rest;
+        free_total_ -= n;
+        return off;
+    }
+    return -1;
+}
+
+void RangeAlloc::free(int64_t off, int64_t n) {
+    if (off < 0 || n <= 0) return;
+    auto it = free_.emplace(off, n).first;
+    free_total_ += n;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// SlotPool
+// ---------------------------------------------------------------------------
+
+SlotPool::~SlotPool() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : {pool_, (void *)desc_, (void *)state_, (void *)part_, (void *)gam_, (void *)propart_,
+                    (void *)list_[0], (void *)list_[1], (void *)counts_})
+        if (p) (void)hipFree(p);
+    for (void *p : {(void *)hdesc_, (void *)hgam_, (void *)hsnap_[0], (void *)hsnap_[1]})
+        if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int SlotPool::setup(const Problem &P, const KernelChoice &kc, int tsize, int slots, int64_t pool_elems,
+                    int64_t tile_cap, int64_t max_pixels) {
+    P_ = P;
+    kc_ = kc;
+    P_.TT = kc.iters();
+    tsize_ = tsize;
+    nslots_ = slots;
+    pool_elems_ = pool_elems;
+    tile_cap_ = tile_cap;
+    max_pixels_ = max_pixels;
+    slots_.assign((size_t)slots, Slot{});
+    mem_.reset(pool_elems);
+    tiles_.reset(tile_cap);
+    hipError_t e;
+    if ((e = hipStreamCreate(&stream))) return e;
+    if ((e = hipMalloc(&pool_, (size_t)pool_elems * (size_t)tsize))) return e;
+    if ((e = hipMalloc((void **)&desc_, sizeof(ImgDesc) * (size_t)slots))) return e;
+    if ((e = hipMalloc((void **)&state_, sizeof(ImgState) * (size_t)slots))) return e;
+    if ((e = hipMalloc((void **)&part_, sizeof(double) * kSlots * (size_t)tile_cap))) return e;
+    if ((e = hipMalloc((void **)&gam_, sizeof(double) * (size_t)slots))) return e;
+    if ((e = hipMalloc((void **)&propart_, sizeof(double) * 4 * (size_t)prologue_blocks(max_pixels)))) return e;
+    for (int i = 0; i < 2; ++i)
+        if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)tile_cap))) return e;
+    if ((e = hipMalloc((void **)&counts_, 64))) return e;
+    if ((e = hipHostMalloc((void **)&hdesc_, sizeof(ImgDesc) * (size_t)slots, 0))) return e;
+    if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)slots, 0))) return e;
+    for (int i = 0; i < 2; ++i)
+        if ((e = hipHostMalloc((void **)&hsnap_[i], sizeof(ImgState) * (size_t)slots, 0))) return e;
+    // every slot starts finished (mode kDone = 0); both lists empty
+    if ((e = hipMemsetAsync(state_, 0, sizeof(ImgState) * (size_t)slots, stream))) return e;
+    if ((e = hipMemsetAsync(desc_, 0, sizeof(ImgDesc) * (size_t)slots, stream))) return e;
+    launch_reset_counts(counts_, stream);
+    events_.resize(4);
+    for (auto &ev : events_)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return e;
+    return hipStreamSynchronize(stream);
+}
+
+int64_t SlotPool::footprint(const ServeRequest &r) const {
+    const int64_t N = r.L[0] * r.L[1] * r.L[2];
+    return (2 + 5 * (int64_t)P_.k) * N;
+}
+
+bool SlotPool::fits_ever(const ServeRequest &r) const {
+    const int64_t N = r.L[0] * r.L[1] * r.L[2];
+    return N <= max_pixels_ && footprint(r) <= pool_elems_;
+}
+
+bool SlotPool::release_dual(int64_t id) {
+    auto it = kept_.find(id);
+    if (it == kept_.end()) return false;
+    mem_.free(it->second.off, it->second.n);
+    kept_.erase(it);
+    return true;
+}
+
+bool SlotPool::evict_one_dual() {
+    if (kept_.empty()) return false;
+    auto old = kept_.begin();
+    for (auto it = kept_.begin(); it != kept_.end(); ++it)
+        if (it->second.stamp < old->second.stamp) old = it;
+    mem_.free(old->second.off, old->second.n);
+    kept_.erase(old);
+    return true;
+}
+
+template <class T>
+static void launch_setup(const Problem &Pi, void *pool, ImgDesc *desc, ImgState *state, double *gam, double *propart,
+                         int64_t N, bool warm, bool tv1d, hipStream_t s) {
+    const double u_round = 0.5 * (sizeof(T) == 4 ? (double)FLT_EPSILON : DBL_EPSILON);
+    const double eta = 0.5 * (sizeof(T) == 4 ? (double)FLT_TRUE_MIN : DBL_TRUE_MIN);
+    launch_prologue<T>(Pi, desc, state, gam, 1, propart, N, s);
+    if (warm) launch_warm<T>(Pi, (T *)pool, desc, state, 1, N, s);
+    if (tv1d) launch_tv1d<T>(Pi, desc, state, 1, N, u_round, eta, s);
+}
+
+bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
+    *err = 0;
+    int i = -1;
+    for (int j = 0; j < nslots_; ++j)
+        if (!slots_[(size_t)j].used) {
+            i = j;
+            break;
+        }
+    if (i < 0) return false;
+    const int k = P_.k;
+    const int64_t N = r.L[0] * r.L[1] * r.L[2];
+    ImgDesc D{};
+    for (int a = 0; a < 3; ++a) D.L[a] = (int32_t)r.L[a];
+    D.ntx = (int32_t)((r.L[2] + kc_.tile_x() - 1) / kc_.tile_x());
+    D.nty = (int32_t)((r.L[1] + kc_.tile_y() - 1) / kc_.tile_y());
+    D.ntz = (int32_t)(kc_.generic ? r.L[0] : 1);
+    D.ntiles = D.ntx * D.nty * D.ntz;
+    const int64_t n_main = (2 + 4 * (int64_t)k) * N, n_dual = (int64_t)k * N;
+    const int64_t off_main = mem_.alloc(n_main);
+    if (off_main < 0) return false;
+    const int64_t off_dual = mem_.alloc(n_dual);
+    if (off_dual < 0) {
+        mem_.free(off_main, n_main);
+        return false;
+    }
+    const int64_t tile_off = tiles_.alloc(D.ntiles);
+    if (tile_off < 0) {
+        mem_.free(off_main, n_main);
+        mem_.free(off_dual, n_dual);
+        return false;
+    }
+    hipStream_t s = stream;
+    char *base = (char *)pool_;
+    const size_t ts = (size_t)tsize_;
+    D.d = base + (size_t)off_main * ts;
+    D.z = base + (size_t)(off_main + N) * ts;
+    D.st_off = off_main + 2 * N;
+    D.q = base + (size_t)off_dual * ts;
+    D.sw = nullptr;
+    D.q_cstride = N;
+    D.sw_cstride = N;
+    D.tile_base = (int32_t)tile_off;
+    // the input, gathered from its strides
+    StridedShape sh{};
+    int64_t st[3] = {r.xs[0], r.xs[1], r.xs[2]};
+    if (!st[0] && !st[1] && !st[2]) {
+        st[2] = 1;
+        st[1] = r.L[2];
+        st[0] = r.L[1] * r.L[2];
+    }
+    for (int a = 0; a < 3; ++a) {
+        sh.shape[a] = r.L[a];
+        sh.stride[a] = st[a];
+    }
+    if (tsize_ == 4) launch_gather<float>((const float *)r.x, (float *)D.d, N, 3, sh, s);
+    else launch_gather<double>((const double *)r.x, (double *)D.d, N, 3, sh, s);
+    // a warm start from a kept dual of the same shape
+    bool warm = false;
+    if (r.warm_from >= 0) {
+        auto it = kept_.find(r.warm_from);
+        if (it != kept_.end() && it->second.L[0] == r.L[0] && it->second.L[1] == r.L[1] &&
+            it->second.L[2] == r.L[2]) {
+            if ((*err = hipMemcpyAsync(D.q, base + (size_t)it->second.off * ts, (size_t)n_dual * ts,
+                                       hipMemcpyDeviceToDevice, s)))
+                return false;
+            it->second.stamp = ++stamp_;
+            warm = true;
+        }
+    }
+    hdesc_[i] = D;
+    hgam_[i] = r.gamma;
+    if ((*err = hipMemcpyAsync(desc_ + i, hdesc_ + i, sizeof(ImgDesc), hipMemcpyHostToDevice, s))) return false;
+    if ((*err = hipMemcpyAsync(gam_ + i, hgam_ + i, sizeof(double), hipMemcpyHostToDevice, s))) return false;
+    Problem Pi = P_;
+    Pi.tol = r.tol;
+    Pi.max_iter = r.max_iter;
+    Pi.warm = warm ? 1 : 0;
+    int active = 0;
+    for (int a = 0; a < k; ++a)
+        if (r.L[3 - k + a] > 1 && P_.w[a] > 0.0) ++active;
+    const bool tv1d = P_.iso && !P_.sw && active == 1;
+    if (tsize_ == 4) launch_setup<float>(Pi, pool_, desc_ + i, state_ + i, gam_ + i, propart_, N, warm, tv1d, s);
+    else launch_setup<double>(Pi, pool_, desc_ + i, state_ + i, gam_ + i, propart_, N, warm, tv1d, s);
+    launch_initial_list(desc_ + i, state_ + i, list_[p_], counts_ + p_, 1, s, i);
+    if ((*err = hipGetLastError())) return false;
+    Slot &S = slots_[(size_t)i];
+    S.used = true;
+    S.done = false;
+    S.req = r;
+    S.t_admit = now;
+    S.off_main = off_main;
+    S.n_main = n_main;
+    S.off_dual = off_dual;
+    S.n_dual = n_dual;
+    S.tile_off = tile_off;
+    S.ntiles = D.ntiles;
+    S.N = N;
+    S.admit_seq = seq_;
+    ++live_;
+    live_tiles_ += D.ntiles;
+    top_ = std::max(top_, i + 1);
+    return true;
+}
+
+int SlotPool::enqueue_chunk(bool *enqueued) {
+    *enqueued = false;
+    if (live_ == 0 || flight_.size() >= 2) return 0;
+    hipStream_t s = stream;
+    const int TT = kc_.iters();
+    const double u_round = 0.5 * (tsize_ == 4 ? (double)FLT_EPSILON : DBL_EPSILON);
+    const double eta = 0.5 * (tsize_ == 4 ? (double)FLT_TRUE_MIN : DBL_TRUE_MIN);
+    const int64_t tl = std::max<int64_t>(live_tiles_, 1);
+    int grid;
+    if (kc_.generic) grid = (int)std::max<int64_t>(1, std::min<int64_t>(tl, 8192));
+    else grid = (int)std::max<int64_t>(8, std::min<int64_t>(((tl + 3) / 4 + 7) / 8 * 8, 2048));
+    for (int j = 0; j < steps_per_sync; ++j) {
+        if (tsize_ == 4) {
+            StepArgs<float> A{(float *)pool_, desc_, state_, part_, list_[p_], counts_ + p_, counts_ + (p_ ^ 1),
+                              P_.k};
+            launch_step<float>(kc_, P_.iso != 0, P_.sw != 0, A, grid, s);
+        } else {
+            StepArgs<double> A{(double *)pool_, desc_, state_, part_, list_[p_], counts_ + p_, counts_ + (p_ ^ 1),
+                               P_.k};
+            launch_step<double>(kc_, P_.iso != 0, P_.sw != 0, A, grid, s);
+        }
+        launch_epilogue(desc_, state_, part_, list_[p_ ^ 1], counts_ + (p_ ^ 1), TT, u_round, eta, top_, s);
+        p_ ^= 1;
+        ++launches;
+    }
+    const int buf = flight_.empty() ? 0 : (flight_.back().buf ^ 1);
+    hipError_t e = hipMemcpyAsync(hsnap_[buf], state_, sizeof(ImgState) * (size_t)top_, hipMemcpyDeviceToHost, s);
+    if (e) return e;
+    ++seq_;
+    hipEvent_t ev = events_[(size_t)(seq_ % (int64_t)events_.size())];
+    if ((e = hipEventRecord(ev, s))) return e;
+    flight_.push_back(Flight{ev, buf, top_, seq_});
+    *enqueued = true;
+    return 0;
+}
+
+int SlotPool::finish_slot(int i, const ImgState &St, std::vector<ServeResult> &out, double now) {
+    Slot &S = slots_[(size_t)i];
+    ServeResult R{};
+    R.id = S.req.id;
+    R.status = St.status ? FP_EINVAL : 0;
+    R.converged = St.converged;
+    R.iters = St.it;
+    double g = St.gp;
+    if (P_.iso && g == g) g = g < 0.0 ? 0.0 : g;
+    if (P_.iso && St.e) g = std::ldexp(g, 2 * St.e);
+    R.gap = g;
+    R.zerr = St.zerr;
+    R.t_submit = S.req.t_submit;
+    R.t_admit = S.t_admit;
+    R.t_done = now;
+    if (S.req.z) {
+        const char *zsrc = (const char *)pool_ + (size_t)(S.off_main + S.N) * (size_t)tsize_;
+        if (hipError_t e = hipMemcpyAsync(S.req.z, zsrc, (size_t)S.N * (size_t)tsize_, hipMemcpyDeviceToDevice,
+                                          stream))
+            return e;
+    }
+    mem_.free(S.off_main, S.n_main);
+    tiles_.free(S.tile_off, S.ntiles);
+    if (S.req.keep_dual && !R.status) {
+        Kept K{S.off_dual, S.n_dual, {S.req.L[0], S.req.L[1], S.req.L[2]}, ++stamp_};
+        kept_[S.req.id] = K;
+    } else {
+        mem_.free(S.off_dual, S.n_dual);
+    }
+    --live_;
+    live_tiles_ -= S.ntiles;
+    S = Slot{};
+    out.push_back(R);
+    return 0;
+}
+
+int SlotPool::retire(std::vector<ServeResult> &out, double (*clock)(const void *), const void *ctx) {
+    if (flight_.empty()) return 0;
+    const Flight f = flight_.front();
+    flight_.pop_front();
+    if (hipError_t e = hipEventSynchronize(f.ev)) return e;
+    const double now = clock(ctx);
+    const ImgState *snap = hsnap_[f.buf];
+    for (int i = 0; i < f.top; ++i) {
+        Slot &S = slots_[(size_t)i];
+        // a slot admitted after the chunk was enqueued is not in its snapshot
+        if (!S.used || S.admit_seq >= f.seq) continue;
+        if (snap[i].mode != kDone) continue;
+        if (int e = finish_slot(i, snap[i], out, now)) return e;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Server
+// ---------------------------------------------------------------------------
+
+Server::~Server() = default;
+
+int Server::setup(const Problem *probs, const KernelChoice *kcs, int tsize, int slots, size_t pool_bytes) {
+    for (int c = 0; c < 2; ++c) {
+        if (probs[c].k == 0) continue;
+        pools_[c].reset(new SlotPool());
+        const int64_t elems = (int64_t)(pool_bytes / (size_t)tsize);
+        const int k = probs[c].k;
+        const int64_t max_pixels = std::min<int64_t>(elems / (2 + 5 * k), (int64_t)1 << 28);
+        const int64_t tile_cap = std::max<int64_t>(elems / 1024, 4096);
+        if (int e = pools_[c]->setup(probs[c], kcs[c], tsize, slots, elems, tile_cap, max_pixels)) return e;
+    }
+    return 0;
+}
+
+double Server::now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+static double server_clock(const void *ctx) { return ((const Server *)ctx)->now(); }
+
+int Server::submit(ServeRequest r, int64_t *id) {
+    const int c = r.k == 3 ? 1 : 0;
+    if ((r.k != 2 && r.k != 3) || !pools_[c]) return FP_EINVAL;
+    for (int a = 0; a < 3; ++a)
+        if (r.L[a] < 1) return FP_ESHAPE;
+    if (!r.x) return FP_EINVAL;
+    if (!(r.gamma >= 0.0) || !std::isfinite(r.gamma) || !(r.tol >= 0.0) || !std::isfinite(r.tol) || r.max_iter < 1)
+        return FP_EPARAM;
+    if (!pools_[c]->fits_ever(r)) return FP_ENOMEM;
+    r.id = next_id_++;
+    if (r.t_submit < 0.0) r.t_submit = now();
+    queue_[c].push_back(r);
+    if (id) *id = r.id;
+    return FP_OK;
+}
+
+int Server::step(int *live, int *queued) {
+    for (int c = 0; c < 2; ++c) {
+        SlotPool *P = pools_[c].get();
+        if (!P) continue;
+        // admissions, first come first served
+        while (!queue_[c].empty()) {
+            int err = 0;
+            if (P->try_admit(queue_[c].front(), now(), &err)) {
+                queue_[c].pop_front();
+                continue;
+            }
+            if (err) return err;
+            if (P->evict_one_dual()) continue;
+            break;
+        }
+        bool enq = false;
+        if (int e = P->enqueue_chunk(&enq)) return e;
+    }
+    for (int c = 0; c < 2; ++c) {
+        SlotPool *P = pools_[c].get();
+        if (!P) continue;
+        // keep one chunk in flight while the older one is retired
+        if (P->in_flight() >= 2 || (P->in_flight() > 0 && P->live() == 0))
+            if (int e = P->retire(done_, server_clock, this)) return e;
+    }
+    int l = 0, q = 0;
+    for (int c = 0; c < 2; ++c) {
+        if (pools_[c]) l += pools_[c]->live() + pools_[c]->in_flight();
+        q += (int)queue_[c].size();
+    }
+    if (live) *live = l;
+    if (queued) *queued = q;
+    return 0;
+}
+
+int64_t Server::poll(ServeResult *out, int64_t max) {
+    const int64_t n = std::min<int64_t>(max, (int64_t)done_.size());
+    for (int64_t i = 0; i < n; ++i) out[i] = done_[(size_t)i];
+    done_.erase(done_.begin(), done_.begin() + n);
+    return n;
+}
+
+bool Server::release_dual(int64_t id) {
+    for (auto &p : pools_)
+        if (p && p->release_dual(id)) return true;
+    return false;
+}
+
+int64_t Server::launches() const {
+    int64_t n = 0;
+    for (auto &p : pools_)
+        if (p) n += p->launches;
+    return n;
+}
+
+}  // namespace fpr

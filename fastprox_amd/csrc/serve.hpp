@@ -1,0 +1,156 @@
+// A serving loop for image-TV requests on one GPU: continuous batching over
+// the solve engine's kernels (serve.cpp).
+//
+// SPDX-License-Identifier: MIT
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "tv_common.hpp"
+#include "tv_kernels.hpp"
+
+namespace fpr {
+
+// A first-fit allocator of [0, capacity) with coalescing frees, in units.
+class RangeAlloc {
+public:
+    void reset(int64_t capacity);
+    int64_t alloc(int64_t n);  // offset, or -1
+    void free(int64_t off, int64_t n);
+    int64_t free_total() const { return free_total_; }
+
+private:
+    std::map<int64_t, int64_t> free_;  // offset -> size
+    int64_t free_total_ = 0;
+};
+
+struct ServeRequest {
+    int64_t id = -1;
+    int k = 2;
+    int64_t L[3] = {1, 1, 1};    // extents, padded in front with 1
+    const void *x = nullptr;     // device, the server's dtype
+    int64_t xs[3] = {0, 0, 0};   // element strides of x (padded like L)
+    void *z = nullptr;           // device output (contiguous) or null
+    double gamma = 0.0, tol = 1e-6;
+    int max_iter = 100000;
+    int keep_dual = 0;
+    int64_t warm_from = -1;
+    double t_submit = 0.0;       // seconds on the server clock
+};
+
+struct ServeResult {
+    int64_t id;
+    int status;     // 0, or an fp_status
+    int converged;
+    int iters;
+    double gap;     // at the input's scale
+    double zerr;
+    double t_submit, t_admit, t_done;
+};
+
+// One class of requests (one k and one step kernel) with its own stream,
+// slots, memory pool and tile pool.
+class SlotPool {
+public:
+    ~SlotPool();
+    int setup(const Problem &P, const KernelChoice &kc, int tsize, int slots, int64_t pool_elems,
+              int64_t tile_cap, int64_t max_pixels);
+    // Admits r into a free slot if a slot, memory and tiles are free (and
+    // enqueues its set-up); false otherwise.
+    bool try_admit(const ServeRequest &r, double now, int *err);
+    // Enqueues a chunk of step/epilogue launches and a state snapshot when
+    // any slot is live; returns whether one was enqueued.
+    int enqueue_chunk(bool *enqueued);
+    // Waits for the oldest chunk in flight and finishes the requests it shows
+    // done (results appended to out).
+    int retire(std::vector<ServeResult> &out, double (*clock)(const void *), const void *ctx);
+    int in_flight() const { return (int)flight_.size(); }
+    int live() const { return live_; }
+    bool fits_ever(const ServeRequest &r) const;
+    int64_t footprint(const ServeRequest &r) const;
+    bool release_dual(int64_t id);
+    bool evict_one_dual();
+    int steps_per_sync = 8;
+    hipStream_t stream = nullptr;
+    int64_t launches = 0;
+
+private:
+    struct Slot {
+        bool used = false;
+        bool done = false;
+        ServeRequest req;
+        double t_admit = 0.0;
+        int64_t off_main = -1, n_main = 0;  // x, z, state
+        int64_t off_dual = -1, n_dual = 0;  // q
+        int64_t tile_off = -1, ntiles = 0;
+        int64_t N = 0;
+        int64_t admit_seq = 0;  // chunks enqueued before the request's set-up
+    };
+    struct Kept {
+        int64_t off, n, L[3];
+        uint64_t stamp;
+    };
+    int finish_slot(int i, const ImgState &S, std::vector<ServeResult> &out, double now);
+    Problem P_{};
+    KernelChoice kc_{};
+    int tsize_ = 4;
+    int nslots_ = 0;
+    std::vector<Slot> slots_;
+    RangeAlloc mem_, tiles_;
+    int64_t pool_elems_ = 0, tile_cap_ = 0, max_pixels_ = 0;
+    void *pool_ = nullptr;
+    ImgDesc *desc_ = nullptr;
+    ImgState *state_ = nullptr;
+    double *part_ = nullptr, *gam_ = nullptr, *propart_ = nullptr;
+    Tile *list_[2] = {nullptr, nullptr};
+    int *counts_ = nullptr;
+    ImgDesc *hdesc_ = nullptr;     // pinned, per slot
+    double *hgam_ = nullptr;       // pinned, per slot
+    ImgState *hsnap_[2] = {nullptr, nullptr};  // pinned snapshots of the chunks in flight
+    struct Flight {
+        hipEvent_t ev;
+        int buf;
+        int top;      // slots covered by the snapshot
+        int64_t seq;  // the chunk's number (from 1)
+    };
+    int64_t seq_ = 0;
+    std::deque<Flight> flight_;
+    std::vector<hipEvent_t> events_;
+    int p_ = 0;       // the list the next step reads
+    int live_ = 0;
+    int top_ = 0;     // 1 + the highest slot ever used
+    int64_t live_tiles_ = 0;
+    std::map<int64_t, Kept> kept_;
+    uint64_t stamp_ = 0;
+};
+
+class Server {
+public:
+    ~Server();
+    // ops[0]: 2D, ops[1]: 3D problems (k = 0 for a class not served).
+    int setup(const Problem *probs, const KernelChoice *kcs, int tsize, int slots, size_t pool_bytes);
+    int submit(ServeRequest r, int64_t *id);
+    // One round: admissions, a chunk of launches per busy class, and the
+    // retirement of the oldest chunk in flight. live/queued after the round.
+    int step(int *live, int *queued);
+    int64_t poll(ServeResult *out, int64_t max);
+    double now() const;
+    bool release_dual(int64_t id);
+    int64_t launches() const;
+
+private:
+    std::unique_ptr<SlotPool> pools_[2];
+    std::deque<ServeRequest> queue_[2];
+    std::vector<ServeResult> done_;
+    int64_t next_id_ = 0;
+    std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
+};
+
+}  // namespace fpr

@@ -1538,7 +1538,7 @@ __global__ void reset_counts_kernel(int *counts) {
 }
 
 __global__ __launch_bounds__(64) void initial_list_kernel(const ImgDesc *desc, const ImgState *state, Tile *list,
-                                                          int *count, int nimg) {
+                                                          int *count, int nimg, int img0) {
     const int img = blockIdx.x;
     if (img >= nimg) return;
     const ImgState &S = state[img];
@@ -1551,7 +1551,7 @@ __global__ __launch_bounds__(64) void initial_list_kernel(const ImgDesc *desc, c
     const int nxy = D.ntx * D.nty;
     for (int i = threadIdx.x; i < D.ntiles; i += 64) {
         Tile t;
-        t.img = img;
+        t.img = img0 + img;
         t.tz = i / nxy;
         const int r = i - t.tz * nxy;
         t.ty = (int16_t)(r / D.ntx);
@@ -1613,8 +1613,8 @@ void launch_reset_counts(int *counts, hipStream_t s) {
 }
 
 void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list, int *count, int nimg,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(initial_list_kernel, dim3(nimg), dim3(64), 0, s, desc, state, list, count, nimg);
+                         hipStream_t s, int img0) {
+    hipLaunchKernelGGL(initial_list_kernel, dim3(nimg), dim3(64), 0, s, desc, state, list, count, nimg, img0);
 }
 
 template <class T>

@@ -39,8 +39,10 @@ template <class T>
 void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
                      double *part, int64_t maxN, hipStream_t s);
 void launch_reset_counts(int *counts, hipStream_t s);
+// desc and state may point at image img0 of the arrays the step kernels
+// read: the tiles carry img0 + i.
 void launch_initial_list(const ImgDesc *desc, const ImgState *state, Tile *list, int *count, int nimg,
-                         hipStream_t s);
+                         hipStream_t s, int img0 = 0);
 template <class T>
 void launch_warm(const Problem &P, T *arena, const ImgDesc *desc, const ImgState *state, int nimg, int64_t maxN,
                  hipStream_t s);

@@ -270,6 +270,54 @@ FP_API fp_status fpr_deriv_jvp(fpr_deriv *d, const void *dx, const double *dgamm
 FP_API fp_status fpr_deriv_info(const fpr_deriv *d, double *residual, int *iters, int64_t *regions);
 FP_API fp_status fpr_deriv_image_info(const fpr_deriv *d, double *residuals, int *iters, int64_t count);
 
+/* A serving loop: requests of different sizes, steps and tolerances solved
+ * by continuous batching on one GPU. Each class (2D, 3D) has a table of
+ * `slots` slots, a device pool of pool_bytes for the requests' buffers and its
+ * own stream; the step kernels run over every live request of a class at
+ * once, and a request is admitted as soon as a slot, its memory and its tiles
+ * free up (first come, first served; kept duals are evicted, least recently
+ * used first, when memory is short). A request's result is the same as the
+ * plain solve's (fp_prox on that image alone). No counterpart in the
+ * reference, which has no server; the solve per request is tv_iso.cpp's. */
+typedef struct fpr_server fpr_server;
+typedef struct {
+    int ndim;                 /* 2 or 3: the class */
+    int64_t shape[3];         /* the first ndim entries */
+    const void *x;            /* device, the server's dtype; read at admission */
+    const int64_t *strides;   /* element strides of x, or NULL for C order */
+    void *z;                  /* device output (contiguous), or NULL */
+    double gamma;
+    double tol;               /* stop at gap <= tol |x|^2 / 2 */
+    int max_iter;
+    int keep_dual;            /* keep the dual resident after completion */
+    int64_t warm_from;        /* start from the kept dual of that request
+                                 (same shape), or -1 */
+} fpr_request;
+typedef struct {
+    int64_t id;
+    int status;               /* FP_OK, or FP_EINVAL for an infinite entry */
+    int converged;
+    int iters;
+    double gap;
+    double zerr;              /* certified relative error of z */
+    double t_submit, t_admit, t_done;  /* seconds on the server's clock */
+} fpr_result;
+/* op2d / op3d: the operators of the two classes (NULL: class not served;
+ * spatial weights are not served). */
+FP_API fp_status fpr_server_create(const fp_op *op2d, const fp_op *op3d, fp_dtype dtype, size_t pool_bytes,
+                                   int slots, fpr_server **out);
+FP_API void      fpr_server_free(fpr_server *s);
+/* Queues r; t_submit: its arrival on the server clock (< 0: now). */
+FP_API fp_status fpr_server_submit(fpr_server *s, const fpr_request *r, double t_submit, int64_t *id);
+/* One round of the loop (admissions, launches, retirements); busy: live
+ * requests plus chunks in flight, queued: requests waiting. */
+FP_API fp_status fpr_server_step(fpr_server *s, int *busy, int *queued);
+/* Moves up to max finished requests' results to out; returns the count. */
+FP_API int64_t   fpr_server_poll(fpr_server *s, fpr_result *out, int64_t max);
+FP_API double    fpr_server_clock(const fpr_server *s);
+FP_API fp_status fpr_server_release(fpr_server *s, int64_t id);
+FP_API fp_status fpr_server_stats(const fpr_server *s, int64_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,141 @@
+"""The serving loop (serve.cpp) on the GPU: every request's result is the
+plain solve's (fp_prox on that image alone, same kernel) to the bit, whatever
+else runs beside it; admission waits for memory and slots; kept duals warm
+later requests; edge cases (gamma 0, NaN, Inf) come back as the plain solve
+reports them."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from fastprox_amd import Op, Workspace
+from fastprox_amd.serve import Server, latency_stats, replay
+from fastprox_amd.synth import crop_of, serving_sources, serving_trace
+
+pytestmark = pytest.mark.gpu
+
+KERNEL = {2: (0, 4, 32), 3: (1, 1, 4)}  # the server's choices: streaming TT=4 SEG=32; generic
+
+
+def _alone(x, gamma, tol, warm=None):
+    k = x.dim()
+    xc = x.contiguous()
+    op = Op(True, k)
+    ws = Workspace(op, xc)
+    ws.set_kernel(*KERNEL[k])
+    if warm is not None:
+        ws.set_dual(warm)
+        ws.set_warm(True)
+    out = torch.empty_like(xc)
+    st = ws.prox(xc, gamma, out, tol=tol, max_iter=100000)
+    gaps, its, _ = ws.image_info(1)
+    dual = torch.empty((k,) + tuple(xc.shape), dtype=xc.dtype, device=xc.device)
+    ws.get_dual(dual)
+    return out, int(its[0]), float(gaps[0]), st, dual
+
+
+def _sources():
+    a, b = serving_sources(n2d=3, n3d=2, side2d=320, side3d=40)
+    return torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda")
+
+
+def test_serve_same_alone_and_in_the_mix(cuda):
+    src2d, src3d = _sources()
+    trace = serving_trace(seconds=1.0, rate=60, seed=3, side2d=(24, 320), side3d=(8, 40))
+    assert len(trace) >= 30
+    srv = Server(Op(True, 2), Op(True, 3), pool_bytes=64 << 20, slots=6)
+    outs, reqs = {}, {}
+    half = len(trace) // 2
+    for j, r in enumerate(trace):
+        x = crop_of(r, src2d, src3d)
+        out = torch.empty(x.shape, dtype=x.dtype, device="cuda")
+        rid = srv.submit(x, r.gamma, tol=r.tol, out=out)
+        outs[rid], reqs[rid] = out, (x, r)
+        if j == half:  # the second half arrives while the first is on the GPU
+            for _ in range(3):
+                srv.step()
+    res = srv.drain()
+    torch.cuda.synchronize()
+    assert len(res) == len(trace)
+    admitted_late = 0
+    first_done = min(d["t_done"] for d in res)
+    for d in res:
+        x, r = reqs[d["id"]]
+        z, its, gap, st, _ = _alone(x, r.gamma, r.tol)
+        print(f"request {d['id']} k={r.k} side={r.side} iters {d['iters']} vs {its}", flush=True)
+        assert d["status"] == 0
+        assert d["iters"] == its
+        assert d["gap"] == gap
+        assert torch.equal(outs[d["id"]], z)
+        admitted_late += d["t_admit"] >= first_done
+    assert admitted_late > 0  # continuous batching: admissions after completions
+
+
+def test_serve_admission_by_memory(cuda):
+    """A pool that holds two 200^2 requests at a time: the rest wait, are
+    admitted as memory frees, and all finish with the plain results."""
+    src2d, src3d = _sources()
+    srv = Server(Op(True, 2), None, pool_bytes=2 * 48 * 200 * 200 + 4096, slots=16)
+    xs, outs = [], []
+    for j in range(7):
+        x = src2d[j % 3][j * 10:j * 10 + 200, 5:205]
+        out = torch.empty((200, 200), device="cuda")
+        srv.submit(x, 0.05 + 0.02 * j, tol=1e-5, out=out)
+        xs.append(x)
+        outs.append(out)
+    res = sorted(srv.drain(), key=lambda d: d["id"])
+    assert len(res) == 7
+    # never more than two live at once (a request is live from its admission to its completion)
+    for d in res:
+        live = sum(1 for e in res if e["t_admit"] <= d["t_admit"] < e["t_done"])
+        assert live <= 2
+    assert any(d["t_admit"] > min(e["t_done"] for e in res) for d in res)
+    for j, d in enumerate(res):
+        z, its, gap, st, _ = _alone(xs[j], 0.05 + 0.02 * j, 1e-5)
+        assert torch.equal(outs[j], z) and d["iters"] == its
+
+
+def test_serve_warm_start_from_a_kept_dual(cuda):
+    src2d, _ = _sources()
+    x = src2d[1][:128, :150]
+    srv = Server(Op(True, 2), None, pool_bytes=32 << 20, slots=4)
+    za = torch.empty((128, 150), device="cuda")
+    a = srv.submit(x, 0.1, tol=1e-5, out=za, keep_dual=True)
+    srv.drain()
+    zb = torch.empty_like(za)
+    srv.submit(x, 0.12, tol=1e-6, out=zb, warm_from=a)
+    (rb,) = srv.drain()
+    _, _, _, _, dual_a = _alone(x, 0.1, 1e-5)
+    z, its, gap, st, _ = _alone(x, 0.12, 1e-6, warm=dual_a)
+    assert torch.equal(zb, z) and rb["iters"] == its
+    srv.release(a)
+    with pytest.raises(Exception):
+        srv.release(a)
+
+
+def test_serve_edge_cases(cuda):
+    src2d, src3d = _sources()
+    srv = Server(Op(True, 2), Op(True, 3), pool_bytes=32 << 20, slots=8)
+    x0 = src2d[0][:40, :50].contiguous()
+    xn = x0.clone()
+    xn[3, 4] = float("nan")
+    xi = x0.clone()
+    xi[1, 1] = float("inf")
+    outs = [torch.empty_like(x0) for _ in range(3)]
+    ids = [srv.submit(x0, 0.0, out=outs[0]), srv.submit(xn, 0.1, out=outs[1]), srv.submit(xi, 0.1, out=outs[2])]
+    res = {d["id"]: d for d in srv.drain()}
+    assert torch.equal(outs[0], x0) and res[ids[0]]["iters"] == 0
+    assert torch.isnan(outs[1]).all() and np.isnan(res[ids[1]]["gap"])
+    assert res[ids[2]]["status"] != 0
+
+
+def test_replay_reports_latency(cuda):
+    src2d, src3d = _sources()
+    trace = serving_trace(seconds=0.5, rate=80, seed=5, side2d=(24, 320), side3d=(8, 40))
+    srv = Server(Op(True, 2), Op(True, 3), pool_bytes=128 << 20, slots=32)
+    res = replay(srv, trace, src2d, src3d)
+    assert len(res) == len(trace)
+    st = latency_stats(res)
+    assert st["p99_ms"] >= st["p50_ms"] > 0 and st["converged"] == len(trace)
