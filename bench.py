@@ -75,6 +75,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB five times (the HBM counters' scale for the traffic figure)")
+    ap.add_argument("--legs", default="uneven,aniso,backward,serve",
+                    help="the other configurations to measure after the batched one: any of uneven (256 uneven "
+                         "images with pixel weights), aniso (64 x 512^2 f32 and f64), backward ((16,3,256,256) "
+                         "autograd), serve (the request trace); '' for none")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-pool-gib", type=float, default=24.0)
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -167,6 +174,17 @@ def main():
         plain = P / (time.perf_counter() - tp0)
         log(f"plain per-call: {plain:.1f} images/s over {P} images")
 
+    legs = [v for v in args.legs.split(",") if v]
+    extra = {}
+    if "uneven" in legs:
+        extra["uneven_batch"] = leg_uneven(args)
+    if "aniso" in legs:
+        extra["aniso"] = leg_aniso(args)
+    if "backward" in legs:
+        extra["backward"] = leg_backward(args)
+    if "serve" in legs:
+        extra["served"] = leg_serve(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, args.gamma, args.tol, args.max_iter)
@@ -202,7 +220,178 @@ def main():
                      "alg_bytes_per_batch": alg_bytes / args.steps},
         "cpu_baseline": cpu,
     }
+    res.update(extra)
     print(json.dumps(res), flush=True)
+
+
+def _sync_time(fn, steps):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, r
+
+
+def leg_uneven(args, n=256, seed=0):
+    """256 float32 images, H and W uniform in [64, 1024], 16 rectangles +
+    N(0, 0.1^2) noise, pixel weights lognormal(0, 0.3), gamma log-uniform in
+    [0.02, 0.3], gap tol 1e-6: one persistent solve over the uneven batch."""
+    import torch
+
+    from fastprox_amd import Op
+    from fastprox_amd.synth import rectangles
+    from fastprox_amd.torch import Batch
+    rng = np.random.default_rng(seed)
+    shapes = [(int(rng.integers(64, 1025)), int(rng.integers(64, 1025))) for _ in range(n)]
+    gam = [float(np.exp(rng.uniform(np.log(0.02), np.log(0.3)))) for _ in range(n)]
+    dev = torch.device("cuda:0")
+    xs, ws, zs, ds = [], [], [], []
+    for i, sh in enumerate(shapes):
+        xs.append(torch.tensor(rectangles(1, sh, n=16, sigma=0.1, seed=1000 + i)[0], device=dev))
+        ws.append(torch.tensor(rng.lognormal(0.0, 0.3, size=sh).astype(np.float32), device=dev))
+        zs.append(torch.empty(sh, dtype=torch.float32, device=dev))
+        ds.append(torch.empty((2,) + sh, dtype=torch.float32, device=dev))
+    px = sum(a * b for a, b in shapes)
+    bt = Batch(Op(True, 2), torch.float32, shapes, weighted=True)
+    bt.set_timing(True)
+
+    def run():
+        return bt.solve(xs, zs, ds, gam, weights=ws, tol=1e-6, max_iter=args.max_iter)
+    run()
+    sec, st = _sync_time(run, max(1, min(args.steps, 3)))
+    gaps, its, zerrs, conv = bt.image_info()
+    a, b_, c = bt.timing()
+    out = {"workload": f"{n} float32 images, H,W ~ U[64,1024] ({px / 1e6:.1f} Mpx), pixel weights lognormal(0,0.3), "
+                       f"gamma log-uniform [0.02,0.3], tol 1e-6, one persistent solve",
+           "images_per_s": n / sec, "mpx_per_s": px / sec / 1e6, "ms_per_batch": sec * 1e3,
+           "iters_mean": float(its.mean()), "iters_max": int(its.max()), "converged": bool(conv.all()),
+           "kernel": kernel_name(bt), "step_kernel_ms": a, "epilogue_ms": b_, "launches": c}
+    log(f"uneven batch: {out}")
+    return out
+
+
+def leg_aniso(args, B=64, S=512):
+    """64 x 512^2 disks (20 disks, radius U[8,96], + N(0, 0.15^2)), weights
+    (1, 0.5), gamma 0.05, gap tol 1e-6, in float32 and float64."""
+    import torch
+
+    from fastprox_amd import Op
+    from fastprox_amd.synth import digest, disks
+    from fastprox_amd.torch import Batch
+    x = disks(B, (S, S), seed=0, dtype=np.float64)
+    out = {"workload": f"anisotropic TV, {B} x {S}x{S} disks, w (1, 0.5), gamma 0.05, tol 1e-6 (device method)",
+           "input_digest": digest(x)}
+    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+        xs = torch.tensor(x, dtype=dt, device="cuda")
+        zs = torch.empty_like(xs)
+        ds = torch.empty((B, 2, S, S), dtype=dt, device="cuda")
+        bt = Batch(Op(False, 2, [1.0, 0.5]), dt, [(S, S)] * B)
+        xl, zl, dl = list(xs), list(zs), list(ds)
+
+        def run():
+            return bt.solve(xl, zl, dl, [0.05] * B, tol=1e-6, max_iter=args.max_iter)
+        run()
+        sec, st = _sync_time(run, 1)
+        gaps, its, zerrs, conv = bt.image_info()
+        out[name] = {"images_per_s": B / sec, "ms_per_batch": sec * 1e3, "iters_mean": float(its.mean()),
+                     "iters_max": int(its.max()), "converged": bool(conv.all())}
+        log(f"aniso {name}: {out[name]}")
+        del xs, zs, ds, bt
+    return out
+
+
+def leg_backward(args):
+    """x (16, 3, 256, 256) float32 (the 256^2 rectangles generator per
+    channel, noise 0.1), gamma 0.1 with requires_grad, loss mean((z -
+    clean)^2): forward solve (tol 1e-6) and backward (the VJP) times."""
+    import torch
+
+    import fastprox_amd.torch as fpt
+    from fastprox_amd.synth import rectangles
+    from fastprox_amd.torch import tv_iso
+    clean = rectangles(48, (256, 256), n=12, sigma=0.0, seed=0).reshape(16, 3, 256, 256)
+    noisy = clean + np.random.default_rng(1).normal(0.0, 0.1, size=clean.shape).astype(np.float32)
+    xc = torch.tensor(noisy, device="cuda")
+    ct = torch.tensor(clean, device="cuda")
+    res = {}
+    for it in range(3):
+        x = xc.clone().requires_grad_(True)
+        g = torch.tensor(0.1, device="cuda", requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        z, info = tv_iso(x, g, tol=1e-6, max_iter=args.max_iter, return_info=True)
+        loss = ((z - ct) ** 2).mean()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res = {"workload": "x (16,3,256,256) float32, gamma 0.1 requires_grad, loss mean((z - clean)^2), "
+                           "solve tol 1e-6, VJP at the default tolerance",
+               "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
+               "solve_iters_max": int(info.iters), "cg_residual_max": float(fpt.LAST_DERIV[0]),
+               "cg_iters_max": int(fpt.LAST_DERIV[1]), "regions": int(fpt.LAST_DERIV[2]),
+               "grad_gamma": float(g.grad), "grad_x_norm": float(x.grad.norm())}
+    log(f"backward: {res}")
+    return res
+
+
+def leg_serve(args):
+    """The request trace: Poisson arrivals at --serve-rate per second for
+    --serve-seconds; 80% 2D float32 squares of side U[128, 1024], 20% 3D
+    cubes of side U[48, 128]; gamma log-uniform [0.01, 0.3]; tol from
+    {1e-4, 1e-5, 1e-6}; each request a crop of a pre-generated source image."""
+    import torch
+
+    from fastprox_amd import Op
+    from fastprox_amd.serve import Server, latency_stats, replay
+    from fastprox_amd.synth import crop_of, serving_sources, serving_trace
+    trace = serving_trace(seconds=args.serve_seconds, rate=args.serve_rate, seed=0)
+    a, b = serving_sources()
+    src2d, src3d = torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda")
+    srv = Server(Op(True, 2), Op(True, 3), pool_bytes=int(args.serve_pool_gib * (1 << 30)), slots=512)
+    # warm-up: a few requests, not recorded
+    replay(srv, serving_trace(seconds=0.2, rate=100, seed=99), src2d, src3d)
+    log(f"serving {len(trace)} requests over {args.serve_seconds:.0f} s ...")
+    res = replay(srv, trace, src2d, src3d, progress=log)
+    st = latency_stats(res)
+    st["workload"] = (f"Poisson {args.serve_rate:.0f} req/s for {args.serve_seconds:.0f} s: 80% 2D f32 side "
+                      f"U[128,1024], 20% 3D f32 side U[48,128], gamma log-uniform [0.01,0.3], tol in "
+                      f"{{1e-4,1e-5,1e-6}}; continuous batching, 512 slots per class, "
+                      f"{args.serve_pool_gib:.0f} GiB pool per class")
+    st["launches"] = srv.launches()
+    log(f"served: {st}")
+    if not args.no_cpu:
+        st["cpu"] = serve_cpu_baseline(trace, a, b)
+        log(f"served cpu: {st['cpu']}")
+    return st
+
+
+def serve_cpu_baseline(trace, src2d, src3d, budget_s=25.0):
+    """The reference CPU solver's latency per request (no queueing) on the
+    first requests of the same trace, at 16 threads, while the budget lasts."""
+    ref = ROOT / "oracle" / "_ref" / "libfastprox_ref.so"
+    if not ref.exists():
+        return None
+    from fastprox_amd.synth import crop_of
+    sys.path.insert(0, str(ROOT / "tests"))
+    import refbind
+    threads = min(16, os.cpu_count() or 1)
+    ops = {2: refbind.Op(True, 2), 3: refbind.Op(True, 3)}
+    lat, t_all = [], time.perf_counter()
+    for r in trace:
+        if time.perf_counter() - t_all > budget_s:
+            break
+        x = np.ascontiguousarray(crop_of(r, src2d, src3d))
+        t0 = time.perf_counter()
+        ops[r.k].solve(x, r.gamma, tol=r.tol, max_iter=100000, threads=threads)
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat) * 1e3
+    return {"requests": int(lat.size), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "cores": threads,
+            "kind": "reference", "sample": f"the first {lat.size} requests of the trace, one fp_prox call each"}
 
 
 def bt_kernel(bt):

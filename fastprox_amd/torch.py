@@ -401,6 +401,11 @@ def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None, keep=False):
     return z, dual.transpose(0, 1), gaps, its, zerrs, conv, exps, uedge
 
 
+# The last backward's (CG residual, CG iterations, regions): the largest over
+# the images (0, 0 for anisotropic TV, whose region means are exact).
+LAST_DERIV = None
+
+
 class _DeviceTV(torch.autograd.Function):
     """torch.py:466-524: forward solves on the device; backward and forward
     mode are the derivative systems at the solution (deriv.py)."""
@@ -444,6 +449,8 @@ class _DeviceTV(torch.autograd.Function):
         with torch.no_grad():
             r, ggamma, gw, gsw, cfg.deriv = S.vjp(c.reshape(ctx.z.shape).contiguous(), cfg.deriv_tol,
                                                  cfg.deriv_max_iter)
+        global LAST_DERIV
+        LAST_DERIV = (cfg.deriv[0], cfg.deriv[1], S.regions())
         out_g = ggamma if ctx.needs_input_grad[2] else None
         out_w = None
         if ctx.wt is not None and ctx.needs_input_grad[3]:
