@@ -130,6 +130,14 @@ class Workspace:
     def set_kernel(self, generic, tt=1, ry=4):
         _check(lib().fpr_workspace_set_kernel(self.h, int(generic), int(tt), int(ry)), "set_kernel")
 
+    def set_tail(self, tt, ry, frac):
+        """fpr_workspace_set_tail: the streaming kernel (tt, ry) once the live
+        tiles fall below frac (0: off)."""
+        f = lib().fpr_workspace_set_tail
+        f.restype = C.c_int
+        f.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double]
+        _check(f(self.h, int(tt), int(ry), float(frac)), "set_tail")
+
     def kernel(self):
         g, t, r = C.c_int(), C.c_int(), C.c_int()
         _check(lib().fpr_workspace_get_kernel(self.h, C.byref(g), C.byref(t), C.byref(r)), "get_kernel")

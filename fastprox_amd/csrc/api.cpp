@@ -405,7 +405,12 @@ fp_status build_engine(fp_workspace &w) {
     }
     w.ptrs.assign((size_t)w.B, fpr::ImagePtrs{});
     w.eng.reset(new fpr::Engine());
-    return hip_status(w.eng->setup((int)w.tsize, P, w.kc, specs));
+    if (int e = w.eng->setup((int)w.tsize, P, w.kc, specs)) return hip_status(e);
+    double frac = 0.0;
+    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)w.tsize, &frac);
+    if (frac > 0.0 && w.kc.stream && !w.kc.generic && tk.iters() >= w.kc.iters())
+        return hip_status(w.eng->set_tail(tk, frac));
+    return FP_OK;
 }
 
 // kind 0: the streaming kernel (tt iterations, segments of ry rows); 1: the
@@ -1044,7 +1049,12 @@ fp_status batch_build(fpr_batch &b) {
     P.wmax_norm = op.iso ? 1 : 0;
     P.alpha_max = 1.0;  // per image, from its weights (the prologue)
     b.eng.reset(new fpr::Engine());
-    return hip_status(b.eng->setup((int)b.tsize, P, b.kc, b.specs));
+    if (int e = b.eng->setup((int)b.tsize, P, b.kc, b.specs)) return hip_status(e);
+    double frac = 0.0;
+    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)b.tsize, &frac);
+    if (frac > 0.0 && b.kc.stream && !b.kc.generic && tk.iters() >= b.kc.iters())
+        return hip_status(b.eng->set_tail(tk, frac));
+    return FP_OK;
 }
 
 }  // namespace
@@ -1418,6 +1428,34 @@ FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt,
     if (tt) *tt = b->kc.TT;
     if (ry) *ry = b->kc.RY;
     return FP_OK;
+}
+
+FP_API fp_status fpr_batch_set_tail(fpr_batch *b, int tt, int ry, double frac) {
+    return guard([&] {
+        if (!b || !b->eng) return FP_EINVAL;
+        fpr::KernelChoice kc;
+        kc.stream = true;
+        kc.TT = tt;
+        kc.RY = ry;
+        if (frac > 0.0 && (!b->kc.stream || b->kc.generic || !fpr::kernel_available((int)b->tsize, true, tt, ry)))
+            return FP_EINVAL;
+        DevGuard dg(b->device);
+        return hip_status(b->eng->set_tail(kc, frac));
+    });
+}
+
+FP_API fp_status fpr_workspace_set_tail(fp_workspace *ws, int tt, int ry, double frac) {
+    return guard([&] {
+        if (!ws || !ws->eng) return FP_EINVAL;
+        fpr::KernelChoice kc;
+        kc.stream = true;
+        kc.TT = tt;
+        kc.RY = ry;
+        if (frac > 0.0 && (!ws->kc.stream || ws->kc.generic || !fpr::kernel_available((int)ws->tsize, true, tt, ry)))
+            return FP_EINVAL;
+        DevGuard dg(ws->device);
+        return hip_status(ws->eng->set_tail(kc, frac));
+    });
 }
 
 FP_API fp_status fpr_batch_scale_exps(const fpr_batch *b, int *exps, int64_t count) {

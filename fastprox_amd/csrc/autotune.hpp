@@ -14,4 +14,8 @@ namespace fpr {
 // image size, the built-in default without one.
 KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B);
 
+// The tail kernel and the live-tile fraction that switches to it (Engine::
+// set_tail); frac 0 when there is none.
+KernelChoice default_tail(int k, int tsize, double *frac);
+
 }  // namespace fpr

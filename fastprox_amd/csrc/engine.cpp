@@ -26,6 +26,8 @@ void Engine::release() {
     f(counts_);
     f(gam_);
     f(propart_);
+    f(desc_tail_);
+    if (hdesc_tail_pinned_) (void)hipHostFree(hdesc_tail_pinned_);
     if (hstate_) (void)hipHostFree(hstate_);
     if (hdesc_pinned_) (void)hipHostFree(hdesc_pinned_);
     if (hcount_) (void)hipHostFree(hcount_);
@@ -38,6 +40,11 @@ void Engine::release() {
     counts_ = nullptr;
     gam_ = nullptr;
     propart_ = nullptr;
+    desc_tail_ = nullptr;
+    hdesc_tail_pinned_ = nullptr;
+    tail_frac_ = 0.0;
+    total_tiles_tail_ = 0;
+    tiles_cap_ = 0;
     hstate_ = nullptr;
     hdesc_pinned_ = nullptr;
     hcount_ = nullptr;
@@ -86,6 +93,7 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
         }
     }
     total_tiles_ = tiles;
+    tiles_cap_ = tiles;
     hipError_t e;
     if ((e = hipMalloc(&arena_, (size_t)std::max<int64_t>(arena, 1) * (size_t)tsize))) return e;
     if ((e = hipMalloc((void **)&desc_, sizeof(ImgDesc) * (size_t)std::max(B, 1)))) return e;
@@ -105,11 +113,58 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
     return hipSuccess;
 }
 
+// The tiles of every image under kc: descs[b]'s tile fields and the total.
+int Engine::tile_layout(const KernelChoice &kc, std::vector<ImgDesc> &descs) const {
+    int tiles = 0;
+    for (size_t b = 0; b < descs.size(); ++b) {
+        ImgDesc &D = descs[b];
+        D.ntx = (int32_t)((D.L[2] + kc.tile_x() - 1) / kc.tile_x());
+        D.nty = (int32_t)((D.L[1] + kc.tile_y() - 1) / kc.tile_y());
+        D.ntz = (int32_t)(kc.generic ? D.L[0] : 1);
+        D.ntiles = D.ntx * D.nty * D.ntz;
+        D.tile_base = tiles;
+        tiles += D.ntiles;
+    }
+    return tiles;
+}
+
+int Engine::set_tail(const KernelChoice &kc, double frac) {
+    tail_frac_ = 0.0;
+    if (!(frac > 0.0) || kc.generic != kc_.generic || P_.k != 2) return hipSuccess;
+    if (!kernel_available(tsize_, kc.stream, kc.TT, kc.RY)) return hipErrorInvalidValue;
+    // The last launch before the switch was planned for up to kc_.TT
+    // iterations; the tail kernel must be able to run them.
+    if (kc.iters() < kc_.iters()) return hipErrorInvalidValue;
+    hdesc_tail_ = hdesc_;
+    const int tiles = tile_layout(kc, hdesc_tail_);
+    hipError_t e;
+    if (!desc_tail_) {
+        const size_t B = std::max<size_t>(hdesc_.size(), 1);
+        if ((e = hipMalloc((void **)&desc_tail_, sizeof(ImgDesc) * B))) return e;
+        if ((e = hipHostMalloc((void **)&hdesc_tail_pinned_, sizeof(ImgDesc) * B, 0))) return e;
+    }
+    if (tiles > tiles_cap_) {  // the partials and the lists hold either tiling
+        for (void *p : {(void *)part_, (void *)list_[0], (void *)list_[1]}) (void)hipFree(p);
+        part_ = nullptr;
+        list_[0] = list_[1] = nullptr;
+        if ((e = hipMalloc((void **)&part_, sizeof(double) * kSlots * (size_t)tiles))) return e;
+        for (int i = 0; i < 2; ++i)
+            if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)tiles))) return e;
+        tiles_cap_ = tiles;
+    }
+    tail_kc_ = kc;
+    tail_kc_.stream = kc.stream;
+    total_tiles_tail_ = tiles;
+    tail_frac_ = frac;
+    return hipSuccess;
+}
+
 template <class T>
 static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, void *arena,
                      ImgDesc *desc, ImgState *state, double *part, Tile **list,
                      int *counts, double *gam, double *propart, int B, int total_tiles, int64_t maxN, bool tv1d,
-                     int *hcount, int steps_per_sync, int max_iter) {
+                     int *hcount, int steps_per_sync, int max_iter, const KernelChoice &tail_kc,
+                     ImgDesc *desc_tail, int total_tiles_tail, double tail_frac) {
     const double u_round = 0.5 * (sizeof(T) == 4 ? (double)FLT_EPSILON : DBL_EPSILON);
     const double eta = 0.5 * (sizeof(T) == 4 ? (double)FLT_TRUE_MIN : DBL_TRUE_MIN);
     launch_prologue<T>(P, desc, state, gam, B, propart, maxN, s);
@@ -123,14 +178,19 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     A.state = state;
     A.part = part;
     A.k = P.k;
-    int grid;
-    if (kc.generic) grid = std::max(1, std::min(total_tiles, 8192));
-    else grid = std::max(8, std::min(((total_tiles + 3) / 4 + 7) / 8 * 8, 2048));
-    const int TT = kc.iters();
+    auto grid_of = [](const KernelChoice &c, int tiles) {
+        if (c.generic) return std::max(1, std::min(tiles, 8192));
+        return std::max(8, std::min(((tiles + 3) / 4 + 7) / 8 * 8, 2048));
+    };
+    int grid = grid_of(kc, total_tiles);
+    KernelChoice cur_kc = kc;
+    const ImgDesc *cur_desc = desc;
+    int TT = kc.iters();
+    bool tail = false, want_tail = false;
     // A bound on the launches any image can need: every accepted launch runs
     // at least one iteration or ends the image, and each restart or stop
     // costs at most two more.
-    const int64_t cap = 8LL * ((int64_t)max_iter / std::max(1, TT) + 8) + 64;
+    const int64_t cap = 8LL * ((int64_t)max_iter / std::max(1, std::min(TT, tail_kc.iters())) + 8) + 64;
     int p = 0;
     int64_t launched = 0;
     (void)0;
@@ -157,15 +217,29 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     // host reads chunk c's count of live tiles, so the device never waits on
     // the host; the chunk enqueued after the last one only meets empty lists.
     auto enqueue = [&](int slot) -> hipError_t {
+        if (want_tail && !tail) {
+            // Relist the live images on the tail kernel's tiling: the list
+            // the next step reads is rebuilt from the image modes.
+            tail = true;
+            cur_kc = tail_kc;
+            cur_desc = desc_tail;
+            A.desc = desc_tail;
+            TT = tail_kc.iters();
+            const int live = hcount[slot] > 0 ? hcount[slot] : total_tiles;  // the count just read
+            grid = grid_of(cur_kc, (int)std::min<int64_t>(
+                                       total_tiles_tail, (int64_t)live * total_tiles_tail / std::max(1, total_tiles) + 64));
+            launch_reset_counts(counts, s);
+            launch_initial_list(desc_tail, state, list[p], counts + p, B, s);
+        }
         for (int i = 0; i < steps_per_sync; ++i) {
             A.list_in = list[p];
             A.count_in = counts + p;
             A.count_out = counts + (p ^ 1);
             const size_t eb = (size_t)(3 * (slot * steps_per_sync + i));
             if (timing) (void)hipEventRecord(ev[eb], s);
-            launch_step<T>(kc, P.iso != 0, P.sw != 0, A, grid, s);
+            launch_step<T>(cur_kc, P.iso != 0, P.sw != 0, A, grid, s);
             if (timing) (void)hipEventRecord(ev[eb + 1], s);
-            launch_epilogue(desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s);
+            launch_epilogue(cur_desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s);
             if (timing) (void)hipEventRecord(ev[eb + 2], s);
             p ^= 1;
             ++launched;
@@ -207,6 +281,7 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
             err = (hipError_t)-1;  // the schedule did not finish: an internal error
             break;
         }
+        if (tail_frac > 0.0 && !tail && (double)hcount[slot] < tail_frac * (double)total_tiles) want_tail = true;
         slot ^= 1;
     }
     destroy();
@@ -233,13 +308,28 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
     hipError_t e = hipMemcpyAsync(gam_, hgam_, sizeof(double) * (size_t)B, hipMemcpyHostToDevice, s);
     if (e) return e;
     if ((e = hipMemcpyAsync(desc_, hdesc_pinned_, sizeof(ImgDesc) * (size_t)B, hipMemcpyHostToDevice, s))) return e;
+    if (tail_frac_ > 0.0) {
+        for (int b = 0; b < B; ++b) {
+            ImgDesc D = hdesc_tail_[(size_t)b];
+            D.d = ptrs[b].d;
+            D.z = ptrs[b].z;
+            D.q = ptrs[b].q;
+            D.sw = ptrs[b].sw;
+            hdesc_tail_pinned_[b] = D;
+        }
+        if ((e = hipMemcpyAsync(desc_tail_, hdesc_tail_pinned_, sizeof(ImgDesc) * (size_t)B, hipMemcpyHostToDevice,
+                                s)))
+            return e;
+    }
     int r;
     if (tsize_ == 4)
         r = run_solve<float>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, propart_, B,
-                             total_tiles_, maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
+                             total_tiles_, maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter, tail_kc_,
+                             desc_tail_, total_tiles_tail_, tail_frac_);
     else
         r = run_solve<double>(*this, P_, kc_, s, arena_, desc_, state_, part_, list_, counts_, gam_, propart_, B,
-                              total_tiles_, maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter);
+                              total_tiles_, maxN_, tv1d_possible_, hcount_, steps_per_sync, max_iter, tail_kc_,
+                              desc_tail_, total_tiles_tail_, tail_frac_);
     if (r) return r;
     if ((e = hipMemcpyAsync(hstate_, state_, sizeof(ImgState) * (size_t)B, hipMemcpyDeviceToHost, s))) return e;
     return hipStreamSynchronize(s);

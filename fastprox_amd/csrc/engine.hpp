@@ -53,6 +53,16 @@ public:
     // arrays of nimg()). Blocks until done; results in state().
     int solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
 
+    // The tail kernel: once the live tiles of a solve drop below frac of the
+    // batch's tiles, the solve continues with kernel kc on its own tiling
+    // (every kernel computes the same iterates, so results do not change;
+    // a latency-oriented tiling shortens the launches of the last images).
+    // frac 0 turns it off; kc must run at least as many iterations per launch
+    // as the main kernel. Call after setup. Returns a hipError_t.
+    int set_tail(const KernelChoice &kc, double frac);
+    const KernelChoice &tail_kernel() const { return tail_kc_; }
+    double tail_fraction() const { return tail_frac_; }
+
     // Copies every image's dual iterate u (k planes of N entries, the solver's
     // own scaling) to dst + i * img_stride elements, after a solve.
     int copy_dual_iterate(hipStream_t s, void *dst, int64_t img_stride) const;
@@ -69,7 +79,15 @@ public:
 
 private:
     void release();
+    int tile_layout(const KernelChoice &kc, std::vector<ImgDesc> &descs) const;
     int tsize_ = 4;
+    KernelChoice tail_kc_{};
+    double tail_frac_ = 0.0;
+    int total_tiles_tail_ = 0;
+    int tiles_cap_ = 0;           // capacity of the partials and the lists
+    std::vector<ImgDesc> hdesc_tail_;
+    ImgDesc *desc_tail_ = nullptr;
+    ImgDesc *hdesc_tail_pinned_ = nullptr;
     Problem P_{};
     KernelChoice kc_{};
     std::vector<ImageSpec> specs_;

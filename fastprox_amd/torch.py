@@ -253,6 +253,8 @@ def _declare_batch(L):
     L.fpr_batch_set_timing.argtypes = [P, C.c_int]
     L.fpr_batch_timing.restype = C.c_int
     L.fpr_batch_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.fpr_batch_set_tail.restype = C.c_int
+    L.fpr_batch_set_tail.argtypes = [P, C.c_int, C.c_int, C.c_double]
     L.fpr_batch_scale_exps.restype = C.c_int
     L.fpr_batch_scale_exps.argtypes = [P, C.POINTER(C.c_int), C.c_int64]
     L.fpr_batch_copy_dual_iterate.restype = C.c_int
@@ -314,6 +316,11 @@ class Batch:
         _check(lib().fpr_batch_stats(self.h, C.byref(a), C.byref(b)), "fpr_batch_stats")
         return a.value, b.value
 
+    def set_tail(self, tt, ry, frac):
+        """Once the live tiles drop below frac, continue on the streaming
+        kernel (tt, ry) (fpr_batch_set_tail); frac 0 turns it off."""
+        _check(lib().fpr_batch_set_tail(self.h, int(tt), int(ry), float(frac)), "fpr_batch_set_tail")
+
     def scale_exps(self):
         e = np.zeros(len(self.shapes), np.int32)
         _check(lib().fpr_batch_scale_exps(self.h, e.ctypes.data_as(C.POINTER(C.c_int)), len(self.shapes)),
@@ -338,12 +345,12 @@ class Batch:
         return a.value, b.value, c.value
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value:
-            try:
+        try:
+            if getattr(self, "h", None) and self.h.value:
                 lib().fpr_batch_free(self.h)
-            except Exception:
-                pass
-            self.h = C.c_void_p()
+                self.h = None
+        except Exception:  # interpreter shutdown
+            pass
 
 
 _CACHE = {}

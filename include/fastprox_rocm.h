@@ -223,6 +223,15 @@ FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_sy
  * step launches timed. */
 FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on);
 FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches);
+/* The tail kernel of a batch or workspace (2D, streaming kernels): once the
+ * live tiles of a solve fall below frac of all its tiles, the remaining
+ * launches use the streaming kernel with tt iterations and segments of ry
+ * rows on its own tiling (tt at least the main kernel's). Results do not
+ * change (every kernel computes the same iterates). frac = 0 turns it off;
+ * the default comes from
+ * FASTPROX_ROCM_TAIL ("stream:TT,SEG,FRAC") or the tuning. */
+FP_API fp_status fpr_batch_set_tail(fpr_batch *b, int tt, int ry, double frac);
+FP_API fp_status fpr_workspace_set_tail(fp_workspace *ws, int tt, int ry, double frac);
 /* The last solve's power-of-two scale exponent per image (isotropic solves
  * far from unit scale run on 2^-e x; src/ops/tv_iso.cpp:367-377). */
 FP_API fp_status fpr_batch_scale_exps(const fpr_batch *b, int *exps, int64_t count);

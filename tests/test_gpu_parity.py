@@ -313,3 +313,29 @@ def test_3d_float64_bitwise(cuda, iso):
         r = (oracle.tv_iso if iso else oracle.tv_aniso_dev)(x[b], 0.2, w, tol=1e-8, max_iter=100000)
         assert its[b] == r.iters
         np.testing.assert_array_equal(out[b].cpu().numpy(), r.z)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_tail_kernel_switch_keeps_results(cuda, dtype):
+    """The switch to the tail kernel's tiling (fpr_workspace_set_tail) in the
+    middle of a solve changes no bit: images with different steps finish at
+    different times, so the switch happens while others still iterate."""
+    x = torch.tensor(image((6, 150, 170), 50), dtype=dtype, device=cuda)
+    steps = torch.tensor([0.02, 0.05, 0.1, 0.2, 0.4, 0.8], dtype=dtype, device=cuda)
+    outs = []
+    for tail in [(4, 32, 0.0), (4, 16, 0.9), (8, 8, 0.5), (6, 40, 0.99)]:
+        op = Op(True, 2)
+        ws = Workspace(op, x)
+        ws.set_kernel(0, 4, 32)
+        ws.set_tail(*tail)
+        z = torch.empty_like(x)
+        ws.prox(x, steps, z, tol=1e-7, max_iter=100000)
+        dual = torch.empty((2,) + tuple(x.shape), dtype=dtype, device=cuda)
+        ws.get_dual(dual)
+        gaps, its, _ = ws.image_info(6)
+        outs.append((z.cpu(), dual.cpu(), its))
+    for z, dual, its in outs[1:]:
+        assert torch.equal(z, outs[0][0]) and torch.equal(dual, outs[0][1])
+        np.testing.assert_array_equal(its, outs[0][2])
+    with pytest.raises(FastproxError):  # fewer iterations per launch than the main kernel
+        Workspace(Op(True, 2), x).set_tail(2, 16, 0.5)
