@@ -213,7 +213,8 @@ def main():
         "speedup_vs_plain": (value / plain) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": {0: "stream2d_kernel", 1: "step1_kernel", 2: "step2d_kernel"}[bt_kernel(bt)[0]],
+                     "kernel": {0: "stream2d_kernel", 1: "step1_kernel", 2: "step2d_kernel",
+                                3: "stream3d_kernel"}[bt_kernel(bt)[0]],
                      "step_kernel_ms_per_batch": step_ms_sum / args.steps,
                      "epilogue_ms_per_batch": epi_ms_sum / args.steps,
                      "launches_per_batch": launches_sum / args.steps,
@@ -407,7 +408,7 @@ def bt_kernel(bt):
 
 def kernel_name(bt):
     kind, tt, ry = bt_kernel(bt)
-    return {0: f"stream:TT={tt},SEG={ry}", 1: "generic", 2: f"block:TT={tt},RY={ry}"}[kind]
+    return {0: f"stream:TT={tt},SEG={ry}", 1: "generic", 2: f"block:TT={tt},RY={ry}", 3: f"s3:SLICES={ry}"}[kind]
 
 
 if __name__ == "__main__":

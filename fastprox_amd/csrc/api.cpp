@@ -424,7 +424,18 @@ fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, fpr::KernelC
         kc.RY = 4;
         return FP_OK;
     }
-    if (kind != 0 && kind != 2) return fail(FP_EINVAL, "kernel kind must be 0 (stream), 1 (generic) or 2 (block)");
+    if (kind == 3) {
+        kc.generic = false;
+        kc.stream = false;
+        kc.s3 = true;
+        kc.TT = 1;
+        kc.RY = ry;
+        if (k != 3) return fail(FP_EINVAL, "the 3D streaming kernel is 3D only");
+        if (!fpr::kernel_available_s3(ry)) return fail(FP_EINVAL, "slices per tile out of range");
+        return FP_OK;
+    }
+    if (kind != 0 && kind != 2)
+        return fail(FP_EINVAL, "kernel kind must be 0 (stream), 1 (generic), 2 (block) or 3 (3D stream)");
     kc.stream = kind == 0;
     kc.TT = tt;
     kc.RY = ry;
@@ -1283,7 +1294,7 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
 
 FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, int *tt, int *ry) {
     if (!ws) return FP_EINVAL;
-    if (generic) *generic = ws->kc.generic ? 1 : (ws->kc.stream ? 0 : 2);
+    if (generic) *generic = ws->kc.s3 ? 3 : (ws->kc.generic ? 1 : (ws->kc.stream ? 0 : 2));
     if (tt) *tt = ws->kc.TT;
     if (ry) *ry = ws->kc.RY;
     return FP_OK;
@@ -1424,7 +1435,7 @@ FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry)
 
 FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry) {
     if (!b) return FP_EINVAL;
-    if (generic) *generic = b->kc.generic ? 1 : (b->kc.stream ? 0 : 2);
+    if (generic) *generic = b->kc.s3 ? 3 : (b->kc.generic ? 1 : (b->kc.stream ? 0 : 2));
     if (tt) *tt = b->kc.TT;
     if (ry) *ry = b->kc.RY;
     return FP_OK;

@@ -16,10 +16,23 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
     (void)B;
     KernelChoice kc;
     if (k != 2) {
-        kc.generic = true;
+        // 3D: the streaming kernel over slabs of 32 slices ("generic" or
+        // "s3:SLICES" in FASTPROX_ROCM_KERNEL3D).
         kc.stream = false;
         kc.TT = 1;
-        kc.RY = 4;
+        const char *e = std::getenv("FASTPROX_ROCM_KERNEL3D");
+        if (e && e[0] == 'g') {
+            kc.generic = true;
+            kc.RY = 4;
+            return kc;
+        }
+        kc.s3 = true;
+        kc.RY = 32;
+        if (e && e[0] == 's') {
+            const char *c = e;
+            while (*c && *c != ':') ++c;
+            if (*c == ':' && kernel_available_s3(std::atoi(c + 1))) kc.RY = std::atoi(c + 1);
+        }
         return kc;
     }
     // FASTPROX_ROCM_KERNEL: "generic", "stream:TT,SEG" or "block:TT,RY".

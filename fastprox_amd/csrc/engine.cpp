@@ -81,7 +81,7 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
         arena += 4 * (int64_t)k * N;
         D.ntx = (int32_t)((s.L[2] + kc.tile_x() - 1) / kc.tile_x());
         D.nty = (int32_t)((s.L[1] + kc.tile_y() - 1) / kc.tile_y());
-        D.ntz = (int32_t)(kc.generic ? s.L[0] : 1);
+        D.ntz = (int32_t)kc.tiles_z(s.L[0]);
         D.ntiles = D.ntx * D.nty * D.ntz;
         D.tile_base = tiles;
         tiles += D.ntiles;
@@ -120,7 +120,7 @@ int Engine::tile_layout(const KernelChoice &kc, std::vector<ImgDesc> &descs) con
         ImgDesc &D = descs[b];
         D.ntx = (int32_t)((D.L[2] + kc.tile_x() - 1) / kc.tile_x());
         D.nty = (int32_t)((D.L[1] + kc.tile_y() - 1) / kc.tile_y());
-        D.ntz = (int32_t)(kc.generic ? D.L[0] : 1);
+        D.ntz = (int32_t)kc.tiles_z(D.L[0]);
         D.ntiles = D.ntx * D.nty * D.ntz;
         D.tile_base = tiles;
         tiles += D.ntiles;
@@ -180,6 +180,7 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     A.k = P.k;
     auto grid_of = [](const KernelChoice &c, int tiles) {
         if (c.generic) return std::max(1, std::min(tiles, 8192));
+        if (c.s3) return std::max(8, std::min((tiles + 7) / 8 * 8, 4096));
         return std::max(8, std::min(((tiles + 3) / 4 + 7) / 8 * 8, 2048));
     };
     int grid = grid_of(kc, total_tiles);

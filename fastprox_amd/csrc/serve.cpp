@@ -182,7 +182,7 @@ bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
     for (int a = 0; a < 3; ++a) D.L[a] = (int32_t)r.L[a];
     D.ntx = (int32_t)((r.L[2] + kc_.tile_x() - 1) / kc_.tile_x());
     D.nty = (int32_t)((r.L[1] + kc_.tile_y() - 1) / kc_.tile_y());
-    D.ntz = (int32_t)(kc_.generic ? r.L[0] : 1);
+    D.ntz = (int32_t)kc_.tiles_z(r.L[0]);
     D.ntiles = D.ntx * D.nty * D.ntz;
     const int64_t n_main = (2 + 4 * (int64_t)k) * N, n_dual = (int64_t)k * N;
     const int64_t off_main = mem_.alloc(n_main);
@@ -281,6 +281,7 @@ int SlotPool::enqueue_chunk(bool *enqueued) {
     const int64_t tl = std::max<int64_t>(live_tiles_, 1);
     int grid;
     if (kc_.generic) grid = (int)std::max<int64_t>(1, std::min<int64_t>(tl, 8192));
+    else if (kc_.s3) grid = (int)std::max<int64_t>(8, std::min<int64_t>((tl + 7) / 8 * 8, 4096));
     else grid = (int)std::max<int64_t>(8, std::min<int64_t>(((tl + 3) / 4 + 7) / 8 * 8, 2048));
     for (int j = 0; j < steps_per_sync; ++j) {
         if (tsize_ == 4) {

@@ -1530,6 +1530,232 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
 }
 
 // ---------------------------------------------------------------------------
+// The 3D streaming step kernel: one iteration per launch, one HBM pass.
+// A workgroup of NW waves owns a tile of 62 x (NW - 1) pixels of every slice
+// of a slab of SEGZ slices: wave w is row y0 + w (the last wave a halo row
+// that only supplies z at y + 1), lane l column x0 - 1 + l (lanes 0 and 63
+// halo columns). The slab is walked slice by slice with a lag of one: at
+// slice s every lane forms z(v) at its voxel (v_2 at x - 1 by a DPP shift,
+// v_1 at y - 1 from the row above, v_0 at s - 1 from the previous step),
+// publishes it in a triple-buffered LDS row, and updates slice s - 1, whose
+// forward neighbours are then all known: s from the registers, y + 1 from
+// the LDS row of the wave below, x + 1 by a DPP shift. The per-voxel
+// arithmetic is step1_kernel's, expression for expression.
+// ---------------------------------------------------------------------------
+
+template <class T, bool ISO, bool SW, int NW>
+__global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SEGZ) {
+    constexpr int RYU = NW - 1;
+    __shared__ T zrow[3][NW][64];
+    __shared__ double zurow[3][NW][64];
+    __shared__ double red[NW][2];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (blockIdx.x == 0 && tid == 0) *A.count_out = 0;
+    const int ntiles = *A.count_in;
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, j8 = blockIdx.x >> 3;
+    const int cpx = (ntiles + 7) >> 3;
+    const int cend = min((xcd + 1) * cpx, ntiles);
+    for (int ti = xcd * cpx + j8; ti < cend; ti += per) {
+        const Tile tl = A.list_in[ti];
+        const int img = tl.img;
+        const ImgDesc &D = A.desc[img];
+        ImgState &S = A.state[img];
+        const int mode = S.mode;
+        const int L0 = D.L[0], H = D.L[1], W = D.L[2];
+        const int64_t SY = W, SZ = (int64_t)H * W, N = SZ * L0;
+        const int x = tl.tx * 62 - 1 + lane;
+        const int y = tl.ty * RYU + wv;
+        const int z0 = tl.tz * SEGZ, z1 = min(z0 + SEGZ, L0);
+        const bool xin = x >= 0 && x < W, yin = y < H, in = xin && yin;
+        const bool upd = in && lane >= 1 && lane <= 62 && wv < RYU;
+        double *part = A.part + (int64_t)(D.tile_base + tl.pad_) * kSlots;
+        if (mode == kOutputNaN) {
+            if (upd)
+                for (int t = z0; t < z1; ++t) {
+                    const int64_t o = t * SZ + (int64_t)y * SY + x;
+                    ((T *)D.z)[o] = (T)NAN;
+                    for (int a = 0; a < 3; ++a) ((T *)D.q)[a * D.q_cstride + o] = (T)NAN;
+                }
+            continue;
+        }
+        const int e = S.e, cnt = S.cnt;
+        const uint32_t tm = S.test_mask, gm = S.gap_mask;
+        const bool g0on = gm & 1u;
+        const bool step = cnt >= 1 && mode == kIterate;
+        const bool dton = step && ((tm >> 1) & 1u);
+        const bool out_mode = mode == kOutput;
+        const bool fresh = S.fresh;
+        T w[3];
+        double wd[3];
+        for (int a = 0; a < 3; ++a) {
+            wd[a] = S.wa[a];
+            w[a] = (T)wd[a];
+        }
+        const T tau = (T)S.tau, beta = (T)S.beta[1];
+        const double gamd = S.gamma;
+        const T *dp = (const T *)D.d;
+        const T *U = A.arena + D.st_off + (int64_t)S.cur * 6 * N;
+        const T *V = S.vsrc_u ? U : U + 3 * N;
+        T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 6 * N;
+        const T *sw = SW ? (const T *)D.sw : nullptr;
+        const int64_t oxy = in ? (int64_t)y * SY + x : 0;
+        const bool yup = in && y > 0;
+        // The previous slice's state at this voxel, and z at it.
+        T pv[3] = {0, 0, 0}, pu[3] = {0, 0, 0}, zvp = 0;
+        double zup = 0.0;
+        double gacc = 0.0, dacc = 0.0;
+        auto load = [&](int s, T *v, T *u, T &v1m, T &u1m, T &dd) {
+            const bool ok = in && s >= 0 && s < L0;
+            const int64_t o = ok ? s * SZ + oxy : 0;
+            dd = ok ? dp[o] : (T)0;
+            for (int a = 0; a < 3; ++a) {
+                v[a] = (ok && !fresh) ? V[a * N + o] : (T)0;
+                u[a] = (ok && !fresh) ? U[a * N + o] : (T)0;
+            }
+            v1m = (ok && !fresh && yup) ? V[N + o - SY] : (T)0;
+            u1m = (ok && !fresh && yup) ? U[N + o - SY] : (T)0;
+        };
+        {  // the slice before the slab: only its axis-0 planes are needed
+            T v[3], u[3], v1m, u1m, dd;
+            load(z0 - 1, v, u, v1m, u1m, dd);
+            pv[0] = v[0];
+            pu[0] = u[0];
+        }
+        for (int s = z0; s <= z1; ++s) {
+            const bool sin = s < L0;
+            T cv[3], cu[3], v1m, u1m, dd;
+            load(s, cv, cu, v1m, u1m, dd);
+            // z(v) and, on gap launches, z(u) at slice s (step1_kernel's zval).
+            T zv = 0;
+            double zu = 0.0;
+            const T v2l = from_left(cv[2]);
+            const double u2l = (g0on || out_mode) ? from_left((double)cu[2]) : 0.0;
+            if (in && sin) {
+                zv = scale_in(dd, e);
+                zv = zv + w[0] * (cv[0] - (s > 0 ? pv[0] : (T)0));
+                zv = zv + w[1] * (cv[1] - v1m);
+                zv = zv + w[2] * (cv[2] - (x > 0 ? v2l : (T)0));
+                if (g0on || out_mode) {
+                    zu = ldexp((double)dd, -e);
+                    zu = zu + wd[0] * ((double)cu[0] - (s > 0 ? (double)pu[0] : 0.0));
+                    zu = zu + wd[1] * ((double)cu[1] - (double)u1m);
+                    zu = zu + wd[2] * ((double)cu[2] - (x > 0 ? u2l : 0.0));
+                }
+            }
+            const int b = s % 3, bp = (s + 2) % 3;
+            zrow[b][wv][lane] = zv;
+            if (g0on) zurow[b][wv][lane] = zu;
+            __syncthreads();
+            // Slice t = s - 1 of the slab: its update, gap and output.
+            const int t = s - 1;
+            const T zr = from_right(zvp);
+            const double zur = (g0on) ? from_right(zup) : 0.0;
+            if (t >= z0 && upd) {
+                const int64_t o = t * SZ + oxy;
+                const bool has0 = wd[0] != 0.0 && t + 1 < L0;
+                const bool has1 = wd[1] != 0.0 && y + 1 < H;
+                const bool has2 = wd[2] != 0.0 && x + 1 < W;
+                double rad[3] = {gamd, gamd, gamd};
+                if (SW) {
+                    if (ISO) rad[0] = iso_radius(gamd, (double)sw[o], S.bound);
+                    else
+                        for (int a = 0; a < 3; ++a) rad[a] = gamd * (double)sw[a * D.sw_cstride + o];
+                }
+                if (g0on) {
+                    double g[3] = {0.0, 0.0, 0.0};
+                    if (has0) g[0] = wd[0] * (zu - zup);
+                    if (has1) g[1] = wd[1] * (zurow[bp][wv + 1][lane] - zup);
+                    if (has2) g[2] = wd[2] * (zur - zup);
+                    double acc = 0.0;
+                    if (ISO) {
+                        double nn = 0.0, dot = 0.0;
+                        for (int a = 0; a < 3; ++a) {
+                            nn += g[a] * g[a];
+                            dot += g[a] * (fresh ? 0.0 : (double)pu[a]);
+                        }
+                        acc = rad[0] * sqrt(nn) - dot;
+                    } else {
+                        for (int a = 0; a < 3; ++a)
+                            acc += rad[a] * fabs(g[a]) - g[a] * (fresh ? 0.0 : (double)pu[a]);
+                    }
+                    gacc += acc;
+                }
+                if (out_mode) {
+                    // cnt is 0 here: z(u) and the dual at the voxel.
+                    const double ul1 = y > 0 ? (double)U[N + o - SY] : 0.0;
+                    const double ul2 = x > 0 ? (double)U[2 * N + o - 1] : 0.0;
+                    const double ul0 = t > 0 ? (double)U[o - SZ] : 0.0;
+                    const double fpv[3] = {fresh ? 0.0 : ul0, fresh ? 0.0 : ul1, fresh ? 0.0 : ul2};
+                    for (int a = 0; a < 3; ++a) {
+                        const double fc = fresh ? 0.0 : (double)pu[a];
+                        double qv;
+                        if (ISO) qv = ldexp(fc / S.wmax, e);
+                        else qv = wd[a] * (fpv[a] - fc);
+                        ((T *)D.q)[a * D.q_cstride + o] = (T)qv;
+                    }
+                    ((T *)D.z)[o] = (T)ldexp(zup, e);
+                } else if (step) {
+                    T g[3] = {0, 0, 0};
+                    if (has0) g[0] = w[0] * (zv - zvp);
+                    if (has1) g[1] = w[1] * (zrow[bp][wv + 1][lane] - zvp);
+                    if (has2) g[2] = w[2] * (zr - zvp);
+                    T c[3];
+                    T nn = 0;
+                    for (int a = 0; a < 3; ++a) {
+                        c[a] = wd[a] != 0.0 ? pv[a] + tau * g[a] : (T)0;
+                        nn += c[a] * c[a];
+                    }
+                    T un[3], vn[3];
+                    if (ISO) {
+                        const T nr = sqrt(nn), rp = (T)rad[0];
+                        const T sc = nr > rp ? rp / nr : (T)1;
+                        for (int a = 0; a < 3; ++a) un[a] = c[a] * sc;
+                    } else {
+                        for (int a = 0; a < 3; ++a) {
+                            const T r = (T)rad[a];
+                            un[a] = c[a] > r ? r : (c[a] < -r ? -r : c[a]);
+                        }
+                    }
+                    for (int a = 0; a < 3; ++a) {
+                        if (dton) dacc += ((double)pv[a] - (double)un[a]) * ((double)un[a] - (double)pu[a]);
+                        vn[a] = un[a] + beta * (un[a] - pu[a]);
+                    }
+                    for (int a = 0; a < 3; ++a) {
+                        Uo[a * N + o] = un[a];
+                        Uo[(3 + a) * N + o] = vn[a];
+                    }
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                pv[a] = cv[a];
+                pu[a] = cu[a];
+            }
+            zvp = zv;
+            zup = zu;
+        }
+        // The tile's partial sums: a fixed tree over the workgroup.
+        if (g0on || dton) {
+            const double a0 = wave_sum(gacc), a1 = wave_sum(dacc);
+            if (lane == 0) {
+                red[wv][0] = a0;
+                red[wv][1] = a1;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double s0 = 0.0, s1 = 0.0;
+                for (int i = 0; i < NW; ++i) {
+                    s0 += red[i][0];
+                    s1 += red[i][1];
+                }
+                if (g0on) part[0] = s0;
+                if (dton) part[3] = s1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Bookkeeping kernels
 // ---------------------------------------------------------------------------
 
@@ -1645,6 +1871,11 @@ void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, T
 
 template <class T, bool ISO, bool SW>
 static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid, hipStream_t s) {
+    if (kc.s3) {
+        hipLaunchKernelGGL((stream3d_kernel<T, ISO, SW, kS3Waves>), dim3(grid), dim3(64 * kS3Waves), 0, s, A,
+                           kc.RY);
+        return;
+    }
     if (kc.generic) {
         hipLaunchKernelGGL((step1_kernel<T, ISO, SW>), dim3(grid), dim3(256), 0, s, A);
         return;
@@ -1686,6 +1917,8 @@ void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A
         else step_dispatch<T, false, false>(kc, A, grid, s);
     }
 }
+
+bool kernel_available_s3(int RY) { return RY >= 1 && RY <= 4096; }
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY) {
     if (stream) return (TT == 2 || TT == 4 || TT == 6 || TT == 8) && RY >= 1 && RY <= 4096;

@@ -21,17 +21,23 @@ struct StridedShape {
 // one-iteration kernel (2D and 3D).
 // stream: the streaming kernel (TT iterations, segments of RY rows); else
 // the block kernel (tiles of RY rows in registers, TT iterations).
+// s3: the 3D streaming kernel (one iteration per launch; tiles of 62 x
+// (kS3Waves - 1) pixels by RY slices).
+constexpr int kS3Waves = 8;
 struct KernelChoice {
     bool generic = false;
     bool stream = true;
+    bool s3 = false;
     int TT = 4;
     int RY = 64;
-    int tile_x() const { return generic ? 64 : 64 - 2 * TT; }
-    int tile_y() const { return generic ? 4 : (stream ? RY : RY - 2 * TT); }
-    int iters() const { return generic ? 1 : TT; }
+    int tile_x() const { return s3 ? 62 : (generic ? 64 : 64 - 2 * TT); }
+    int tile_y() const { return s3 ? kS3Waves - 1 : (generic ? 4 : (stream ? RY : RY - 2 * TT)); }
+    int64_t tiles_z(int64_t L0) const { return s3 ? (L0 + RY - 1) / RY : (generic ? L0 : 1); }
+    int iters() const { return (generic || s3) ? 1 : TT; }
 };
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY);
+bool kernel_available_s3(int RY);
 
 // The prologue's per-block partials: 4 doubles per block, prologue_blocks(maxN) blocks per image.
 int prologue_blocks(int64_t maxN);

@@ -339,3 +339,34 @@ def test_tail_kernel_switch_keeps_results(cuda, dtype):
         np.testing.assert_array_equal(its, outs[0][2])
     with pytest.raises(FastproxError):  # fewer iterations per launch than the main kernel
         Workspace(Op(True, 2), x).set_tail(2, 16, 0.5)
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("iso", [True, False])
+def test_3d_streaming_kernel_matches_generic(cuda, dtype, iso):
+    """The 3D streaming kernel (slabs of slices, LDS rows, DPP columns) runs
+    the generic kernel's per-voxel arithmetic: the same iterations, z and
+    dual, on shapes that cut its tiles every way (thin axes, an inactive
+    axis, pixel weights)."""
+    rng = np.random.default_rng(60)
+    cases = [((3, 37, 29, 70), [1.0, 0.7, 1.3], None), ((2, 1, 9, 130), [1.0, 1.0, 1.0], None),
+             ((2, 20, 8, 63), [0.0, 1.0, 0.5], None), ((1, 33, 15, 62), [1.0, 1.0, 1.0], "sw")]
+    for shape, w, sw in cases:
+        x = torch.tensor(image(shape, 61), dtype=dtype, device=cuda)
+        spatial = None
+        if sw:
+            spatial = rng.uniform(0.5, 1.5, shape[1:] if iso else (3,) + shape[1:])
+        outs = []
+        for kernel in [(1, 1, 4), (3, 1, 32), (3, 1, 5)]:
+            op = Op(iso, 3, w, spatial=spatial)
+            ws = Workspace(op, x)
+            ws.set_kernel(*kernel)
+            z = torch.empty_like(x)
+            ws.prox(x, 0.15, z, tol=1e-8, max_iter=100000)
+            dual = torch.empty((3,) + tuple(x.shape), dtype=dtype, device=cuda)
+            ws.get_dual(dual)
+            gaps, its, _ = ws.image_info(shape[0])
+            outs.append((z.cpu(), dual.cpu(), its))
+        for z, dual, its in outs[1:]:
+            np.testing.assert_array_equal(its, outs[0][2])
+            assert torch.equal(z, outs[0][0]) and torch.equal(dual, outs[0][1])

@@ -16,7 +16,7 @@ from fastprox_amd.synth import crop_of, serving_sources, serving_trace
 
 pytestmark = pytest.mark.gpu
 
-KERNEL = {2: (0, 4, 32), 3: (1, 1, 4)}  # the server's choices: streaming TT=4 SEG=32; generic
+KERNEL = {2: (0, 4, 32), 3: (3, 1, 32)}  # the server's choices: streaming TT=4 SEG=32; 3D streaming, 32 slices
 
 
 def _alone(x, gamma, tol, warm=None):
