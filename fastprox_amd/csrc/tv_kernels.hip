@@ -1621,10 +1621,19 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
             pv[0] = v[0];
             pu[0] = u[0];
         }
+        // Slice s + 1's input is loaded one step ahead, so that its latency
+        // overlaps slice s's work (the barriers wait on LDS only).
+        T nv[3], nu[3], nv1m, nu1m, nd;
+        load(z0, nv, nu, nv1m, nu1m, nd);
         for (int s = z0; s <= z1; ++s) {
             const bool sin = s < L0;
-            T cv[3], cu[3], v1m, u1m, dd;
-            load(s, cv, cu, v1m, u1m, dd);
+            T cv[3], cu[3];
+            for (int a = 0; a < 3; ++a) {
+                cv[a] = nv[a];
+                cu[a] = nu[a];
+            }
+            const T v1m = nv1m, u1m = nu1m, dd = nd;
+            if (s < z1) load(s + 1, nv, nu, nv1m, nu1m, nd);
             // z(v) and, on gap launches, z(u) at slice s (step1_kernel's zval).
             T zv = 0;
             double zu = 0.0;
