@@ -407,7 +407,7 @@ fp_status build_engine(fp_workspace &w) {
     w.eng.reset(new fpr::Engine());
     if (int e = w.eng->setup((int)w.tsize, P, w.kc, specs)) return hip_status(e);
     double frac = 0.0;
-    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)w.tsize, &frac);
+    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)w.tsize, w.img, op.iso, !op.spatial.empty(), &frac);
     if (frac > 0.0 && w.kc.stream && !w.kc.generic && tk.iters() >= w.kc.iters())
         return hip_status(w.eng->set_tail(tk, frac));
     return FP_OK;
@@ -1015,6 +1015,7 @@ fp_status vjp_impl(const fp_op *op, fp_workspace *ws, const fp_array *ga, fp_arr
 
 struct fpr_batch {
     const fp_op *op = nullptr;
+    int64_t rep[3] = {1, 1, 1};  // a representative image for the tuning table
     int dtype = FP_F32;
     size_t tsize = 4;
     int weighted = 0;
@@ -1062,7 +1063,7 @@ fp_status batch_build(fpr_batch &b) {
     b.eng.reset(new fpr::Engine());
     if (int e = b.eng->setup((int)b.tsize, P, b.kc, b.specs)) return hip_status(e);
     double frac = 0.0;
-    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)b.tsize, &frac);
+    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)b.tsize, b.rep, op.iso, b.weighted != 0, &frac);
     if (frac > 0.0 && b.kc.stream && !b.kc.generic && tk.iters() >= b.kc.iters())
         return hip_status(b.eng->set_tail(tk, frac));
     return FP_OK;
@@ -1338,8 +1339,14 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
                     if (sp.L[3 - k + a] != op->sshape[a]) return FP_ESHAPE;
         }
         (void)total;
-        b->kc = fpr::default_kernel(k, (int)b->tsize, count ? b->specs[0].L : nullptr, op->iso,
-                                    b->weighted != 0, count);
+        // The tuning entry of a representative image: the batch's mean
+        // pixel count, as a square (or cube).
+        {
+            const double mean = count ? (double)total / (double)count : 1.0;
+            const double side = k == 3 ? std::cbrt(mean) : std::sqrt(mean);
+            for (int a = 0; a < 3; ++a) b->rep[a] = (a < 3 - k) ? 1 : std::max<int64_t>(1, (int64_t)(side + 0.5));
+        }
+        b->kc = fpr::default_kernel(k, (int)b->tsize, b->rep, op->iso, b->weighted != 0, count);
         b->ptrs.assign((size_t)count, fpr::ImagePtrs{});
         b->gammas.assign((size_t)count, 0.0);
         b->gaps.assign((size_t)count, 0.0);

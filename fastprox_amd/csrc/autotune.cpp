@@ -1,38 +1,134 @@
-// Default step-kernel choice (autotune.hpp). The table below is written by
-// fastprox_amd/autotune.py from measurements on the device; without an entry
-// the defaults are the measured best general choices.
+// The step-kernel choice (autotune.hpp): a tuning table measured on the
+// device by fastprox_amd/autotune.py, read once from tuning.txt beside this
+// library (or the file FASTPROX_ROCM_TUNING names). Each line is
+//     k dtype_bytes iso spatial max_pixels kind TT RY tail_TT tail_RY tail_frac
+// (kind 0: the streaming 2D kernel, 1: generic, 2: block, 3: the 3D streaming
+// kernel); the entry for an image is the first line of its class whose
+// max_pixels is at least the image's pixel count (the last line of the class
+// beyond that). Without a table the built-in defaults below apply.
+// FASTPROX_ROCM_KERNEL / FASTPROX_ROCM_KERNEL3D / FASTPROX_ROCM_TAIL override
+// everything (for sweeps).
 //
 // SPDX-License-Identifier: MIT
 #include "autotune.hpp"
 
+#include <dlfcn.h>
+
+#include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
 
 namespace fpr {
 
+namespace {
+
+struct Entry {
+    int k, tsize, iso, sw;
+    long long max_px;
+    int kind, TT, RY, tTT, tRY;
+    double tfrac;
+};
+
+std::string table_path() {
+    if (const char *e = std::getenv("FASTPROX_ROCM_TUNING")) return e;
+    Dl_info info;
+    if (dladdr((void *)&table_path, &info) && info.dli_fname) {
+        std::string f = info.dli_fname;
+        const size_t cut = f.find_last_of('/');
+        return (cut == std::string::npos ? std::string() : f.substr(0, cut + 1)) + "tuning.txt";
+    }
+    return "tuning.txt";
+}
+
+std::vector<Entry> load_table() {
+    std::vector<Entry> t;
+    FILE *f = std::fopen(table_path().c_str(), "r");
+    if (!f) return t;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        Entry e{};
+        if (std::sscanf(line, "%d %d %d %d %lld %d %d %d %d %d %lf", &e.k, &e.tsize, &e.iso, &e.sw, &e.max_px,
+                        &e.kind, &e.TT, &e.RY, &e.tTT, &e.tRY, &e.tfrac) == 11)
+            t.push_back(e);
+    }
+    std::fclose(f);
+    return t;
+}
+
+const std::vector<Entry> &table() {
+    static const std::vector<Entry> t = load_table();
+    return t;
+}
+
+const Entry *lookup(int k, int tsize, const int64_t *img, bool iso, bool sw) {
+    long long n = 1;
+    for (int a = 0; a < 3; ++a) n *= img ? (long long)img[a] : 1LL;
+    const Entry *last = nullptr;
+    for (const Entry &e : table()) {
+        if (e.k != k || e.tsize != tsize || e.iso != (iso ? 1 : 0) || e.sw != (sw ? 1 : 0)) continue;
+        last = &e;
+        if (n <= e.max_px) return &e;
+    }
+    return last;
+}
+
+bool entry_kernel(const Entry &e, int tsize, KernelChoice &kc) {
+    kc = KernelChoice{};
+    switch (e.kind) {
+        case 0:
+        case 2:
+            kc.stream = e.kind == 0;
+            kc.TT = e.TT;
+            kc.RY = e.RY;
+            return e.k == 2 && kernel_available(tsize, kc.stream, e.TT, e.RY);
+        case 1:
+            kc.generic = true;
+            kc.stream = false;
+            kc.TT = 1;
+            kc.RY = 4;
+            return true;
+        case 3:
+            kc.stream = false;
+            kc.s3 = true;
+            kc.TT = 1;
+            kc.RY = e.RY;
+            return e.k == 3 && kernel_available_s3(e.RY);
+    }
+    return false;
+}
+
+}  // namespace
+
 KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B) {
-    (void)img;
-    (void)iso;
-    (void)sw;
     (void)B;
     KernelChoice kc;
     if (k != 2) {
-        // 3D: the streaming kernel over slabs of 32 slices ("generic" or
-        // "s3:SLICES" in FASTPROX_ROCM_KERNEL3D).
         kc.stream = false;
         kc.TT = 1;
-        const char *e = std::getenv("FASTPROX_ROCM_KERNEL3D");
-        if (e && e[0] == 'g') {
-            kc.generic = true;
-            kc.RY = 4;
-            return kc;
-        }
-        kc.s3 = true;
-        kc.RY = 32;
-        if (e && e[0] == 's') {
+        if (const char *e = std::getenv("FASTPROX_ROCM_KERNEL3D")) {  // "generic" or "s3:SLICES"
+            if (e[0] == 'g') {
+                kc.generic = true;
+                kc.RY = 4;
+                return kc;
+            }
             const char *c = e;
             while (*c && *c != ':') ++c;
-            if (*c == ':' && kernel_available_s3(std::atoi(c + 1))) kc.RY = std::atoi(c + 1);
+            if (*c == ':' && kernel_available_s3(std::atoi(c + 1))) {
+                kc.s3 = true;
+                kc.RY = std::atoi(c + 1);
+                return kc;
+            }
         }
+        if (const Entry *t = lookup(k, tsize, img, iso, sw))
+            if (entry_kernel(*t, tsize, kc)) return kc;
+        kc = KernelChoice{};
+        kc.stream = false;
+        kc.s3 = true;
+        kc.TT = 1;
+        kc.RY = 32;
         return kc;
     }
     // FASTPROX_ROCM_KERNEL: "generic", "stream:TT,SEG" or "block:TT,RY".
@@ -59,13 +155,16 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
             }
         }
     }
+    if (const Entry *t = lookup(k, tsize, img, iso, sw))
+        if (entry_kernel(*t, tsize, kc)) return kc;
+    kc = KernelChoice{};
     kc.stream = true;
     kc.TT = 4;
     kc.RY = 32;
     return kc;
 }
 
-KernelChoice default_tail(int k, int tsize, double *frac) {
+KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, double *frac) {
     KernelChoice kc;
     *frac = 0.0;
     if (k != 2) return kc;
@@ -89,6 +188,15 @@ KernelChoice default_tail(int k, int tsize, double *frac) {
                 kc.RY = v[1];
                 *frac = f;
             }
+        }
+        return kc;
+    }
+    if (const Entry *t = lookup(k, tsize, img, iso, sw)) {
+        if (t->tfrac > 0.0 && kernel_available(tsize, true, t->tTT, t->tRY)) {
+            kc.stream = true;
+            kc.TT = t->tTT;
+            kc.RY = t->tRY;
+            *frac = t->tfrac;
         }
         return kc;
     }

@@ -16,6 +16,6 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
 
 // The tail kernel and the live-tile fraction that switches to it (Engine::
 // set_tail); frac 0 when there is none.
-KernelChoice default_tail(int k, int tsize, double *frac);
+KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, double *frac);
 
 }  // namespace fpr
