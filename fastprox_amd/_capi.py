@@ -87,6 +87,7 @@ def declare(lib):
         "fp_workspace_deriv_info": (I, [P, C.POINTER(D), C.POINTER(I)]),
         "fp_workspace_info": (I, [P, C.POINTER(fp_info)]),
         "fp_workspace_image_info": (I, [P, C.POINTER(D), C.POINTER(I), C.POINTER(D), I64]),
+        "fp_workspace_refine": (I, [P, P, D, C.c_int, A, C.POINTER(fp_info)]),
         "fp_workspace_deriv_exact": (I, [P, C.POINTER(I), C.POINTER(D)]),
         "fp_jvp": (I, [P, P, A, D, A, A]),
         "fp_vjp": (I, [P, P, A, A, C.POINTER(D), A]),

@@ -179,6 +179,15 @@ class Workspace:
                "fp_workspace_image_info")
         return gaps, its, zerrs
 
+    def refine(self, tol, max_iter=100000, out=None):
+        """fp_workspace_refine: the kept call solved again from its input and
+        dual to tol; returns (status, gap, iters)."""
+        inf = A.fp_info()
+        s = lib().fp_workspace_refine(self.op.h, self.h, float(tol), int(max_iter),
+                                      as_array(out).ref if out is not None else None, C.byref(inf))
+        _check(s, "fp_workspace_refine", ok=(A.FP_OK, A.FP_ENOTCONV))
+        return s, inf.gap, inf.iters
+
     def get_dual(self, out):
         _check(lib().fp_workspace_get_dual(self.h, as_array(out).ref), "fp_workspace_get_dual")
         return out

@@ -506,6 +506,86 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     return FP_OK;
 }
 
+struct DevGuard {  // the workspace's device current for the call
+    int dev, was;
+    explicit DevGuard(int d) : dev(d), was(d) {
+        (void)hipGetDevice(&was);
+        if (was != dev) (void)hipSetDevice(dev);
+    }
+    ~DevGuard() {
+        if (was != dev) (void)hipSetDevice(was);
+    }
+};
+
+// The per-image results of the engine's last solve into ws (driver.hpp:
+// 1216-1236, 1331-1341): gaps at the input's scale, iterations, certified
+// errors, fp_info; FP_EINVAL for an infinite entry.
+fp_status record_results(const fp_op *op, fp_workspace *ws, fp_info *info, fp_status *status) {
+    const fpr::ImgState *S = ws->eng->state();
+    bool bad = false, notconv = false;
+    double gsum = 0.0;
+    int itmax = 0;
+    for (int64_t b = 0; b < ws->B; ++b) {
+        const fpr::ImgState &t = S[b];
+        if (t.status) bad = true;
+        double g = t.gp;
+        if (op->iso && g == g) g = g < 0.0 ? 0.0 : g;
+        if (op->iso && t.e) g = std::ldexp(g, 2 * t.e);
+        ws->gaps[(size_t)b] = g;
+        ws->iters[(size_t)b] = t.it;
+        ws->zerrs[(size_t)b] = t.zerr;
+        if (!t.converged) notconv = true;
+        gsum += g;
+        itmax = std::max(itmax, t.it);
+    }
+    if (bad) {
+        ws->dual_valid = false;
+        return FP_EINVAL;
+    }
+    ws->dual_valid = true;
+    *status = notconv ? FP_ENOTCONV : FP_OK;
+    ws->info = fp_info{gsum, itmax, 0};
+    ws->info_status = *status;
+    ws->info_valid = true;
+    if (info) *info = ws->info;
+    return FP_OK;
+}
+
+// The kept point's extras after xsave and zsave hold it: the per-edge dual
+// (aniso), the scale exponents and the smoothing of the derivative system
+// (tv_iso.cpp:566-568: mu = 0.1 w_max sqrt(2 gap / N) at the solved scale when
+// the structure is read off the dual, two or more active axes; back at the
+// input's scale). Waits for the stream.
+fp_status record_kept(const fp_op *op, fp_workspace *ws, hipStream_t s) {
+    const int k = op->k;
+    if (!op->iso && ws->eng->copy_dual_iterate(s, ws->usave, (int64_t)k * ws->N)) return FP_EDEVICE;
+    if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    const fpr::ImgState *S = ws->eng->state();
+    for (int64_t b = 0; b < ws->B; ++b) {
+        const fpr::ImgState &t = S[b];
+        ws->exps[(size_t)b] = t.e;
+        double mu = 0.0;
+        if (op->iso) {
+            double wmax = 0.0, wL = 0.0;
+            int na = 0;
+            for (int a = 0; a < k; ++a)
+                if (ws->img[3 - k + a] > 1) wL = std::max(wL, op->w[a]);
+            for (int a = 0; a < k; ++a)
+                if (ws->gammas[(size_t)b] > 0.0 && op->w[a] > 0.0 && ws->img[3 - k + a] > 1 &&
+                    op->w[a] >= 1e-100 * wL) {
+                    wmax = std::max(wmax, op->w[a]);
+                    ++na;
+                }
+            const double gs = std::ldexp(ws->gaps[(size_t)b], -2 * t.e);
+            if (na >= 2 && gs == gs) mu = 0.1 * wmax * std::sqrt(2.0 * std::max(gs, 0.0) / (double)ws->N);
+        }
+        ws->smooth[(size_t)b] = std::ldexp(mu, t.e);
+        ws->exact[(size_t)b] = !(mu > 0.0);
+    }
+    ws->valid = true;
+    return FP_OK;
+}
+
 fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *oa, const fp_options *opt,
                     fp_workspace *ws, fp_info *info, const fp_array *sa) {
     if (!op || !xa || !oa) return FP_EINVAL;
@@ -621,71 +701,78 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     }
     if (!z_direct)
         if (fp_status st2 = stage_out(*ws, z, out, s)) return st2;
-    // Per-image results (driver.hpp:1216-1236, 1331-1341).
-    const fpr::ImgState *S = ws->eng->state();
-    bool bad = false, notconv = false;
-    double gsum = 0.0;
-    int itmax = 0;
-    for (int64_t b = 0; b < B; ++b) {
-        const fpr::ImgState &t = S[b];
-        if (t.status) bad = true;
-        double g = t.gp;
-        if (op->iso && g == g) g = g < 0.0 ? 0.0 : g;
-        if (op->iso && t.e) g = std::ldexp(g, 2 * t.e);
-        ws->gaps[(size_t)b] = g;
-        ws->iters[(size_t)b] = t.it;
-        ws->zerrs[(size_t)b] = t.zerr;
-        if (!t.converged) notconv = true;
-        gsum += g;
-        itmax = std::max(itmax, t.it);
-    }
-    if (bad) {
-        ws->dual_valid = false;
-        return FP_EINVAL;
-    }
-    ws->dual_valid = true;
-    const fp_status status = notconv ? FP_ENOTCONV : FP_OK;
-    ws->info = fp_info{gsum, itmax, 0};
-    ws->info_status = status;
-    ws->info_valid = true;
-    if (info) *info = ws->info;
+    fp_status status = FP_OK;
+    if (fp_status st2 = record_results(op, ws, info, &status)) return st2;
     if (keep) {
         const size_t bytes = (size_t)ws->numel * ws->tsize;
         if (hipMemcpyAsync(ws->xsave, d, bytes, hipMemcpyDeviceToDevice, s)) return FP_EDEVICE;
         if (hipMemcpyAsync(ws->zsave, z, bytes, hipMemcpyDeviceToDevice, s)) return FP_EDEVICE;
-        if (!op->iso && ws->eng->copy_dual_iterate(s, ws->usave, (int64_t)op->k * ws->N)) return FP_EDEVICE;
-        if (hipStreamSynchronize(s)) return FP_EDEVICE;
-        // The smoothing of the derivative system (tv_iso.cpp:566-568): mu =
-        // 0.1 w_max sqrt(2 gap / N) at the solved scale when the structure is
-        // read off the dual (two or more active axes), back at the input's.
-        for (int64_t b = 0; b < B; ++b) {
-            const fpr::ImgState &t = S[b];
-            ws->exps[(size_t)b] = t.e;
-            double mu = 0.0;
-            if (op->iso) {
-                double wmax = 0.0, wL = 0.0;
-                int na = 0;
-                for (int a = 0; a < k; ++a)
-                    if (ws->img[3 - k + a] > 1) wL = std::max(wL, op->w[a]);
-                for (int a = 0; a < k; ++a)
-                    if (ws->gammas[(size_t)b] > 0.0 && op->w[a] > 0.0 && ws->img[3 - k + a] > 1 &&
-                        op->w[a] >= 1e-100 * wL) {
-                        wmax = std::max(wmax, op->w[a]);
-                        ++na;
-                    }
-                const double gs = std::ldexp(ws->gaps[(size_t)b], -2 * t.e);
-                if (na >= 2 && gs == gs)
-                    mu = 0.1 * wmax * std::sqrt(2.0 * std::max(gs, 0.0) / (double)ws->N);
-            }
-            ws->smooth[(size_t)b] = std::ldexp(mu, t.e);
-            ws->exact[(size_t)b] = !(mu > 0.0);
-        }
-        ws->valid = true;
+        if (fp_status st2 = record_kept(op, ws, s)) return st2;
         ws->steps_on = sa != nullptr;
         ws->gamma = step;
     }
     if (!z_direct || !x_direct)
         if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    return status;
+}
+
+// fp_workspace_refine (the reference's refine_impl, src/api/api.cpp:1185-1217):
+// the kept call solved again from its saved input, warm-started from its
+// dual, to tol within max_iter; the derivatives then read the refined point.
+fp_status refine_impl(const fp_op *op, fp_workspace *ws, double tol, int max_iter, fp_array *oa, fp_info *info) {
+    if (!op || !ws || ws->op != op || !ws->valid) return FP_EINVAL;
+    if (!(tol >= 0.0) || !std::isfinite(tol) || max_iter < 1) return FP_EPARAM;
+    Arr out;
+    if (oa) {
+        if (fp_status s = to_arr(oa, out)) return s;
+        if (out.dtype != ws->dtype) return FP_EDTYPE;
+        if (out.ndim != ws->ndim) return FP_ESHAPE;
+        for (int d = 0; d < out.ndim; ++d)
+            if (out.shape[d] != ws->shape[d]) return FP_ESHAPE;
+        if (fp_status s = check_output(nullptr, out)) return s;
+    }
+    if (info) *info = fp_info{0.0, 0, 0};
+    ws->info_valid = false;
+    ws->deriv.reset();
+    if (ws->numel == 0) {
+        ws->info = fp_info{0.0, 0, 0};
+        ws->info_status = FP_OK;
+        ws->info_valid = true;
+        return FP_OK;
+    }
+    DevGuard guard(ws->device);
+    hipStream_t s = (hipStream_t)ws->stream;
+    hipError_t err;
+    const void *sw = op->device_weights(ws->device, (int)ws->tsize, err);
+    if (err) return hip_status(err);
+    for (int64_t b = 0; b < ws->B; ++b) {
+        fpr::ImagePtrs &p = ws->ptrs[(size_t)b];
+        p.d = (const char *)ws->xsave + (size_t)(b * ws->N) * ws->tsize;
+        p.z = (char *)ws->zsave + (size_t)(b * ws->N) * ws->tsize;
+        p.q = (char *)ws->q + (size_t)(b * ws->N) * ws->tsize;
+        p.sw = sw;
+    }
+    int r = ws->eng->solve(s, ws->ptrs.data(), ws->gammas.data(), true, tol, max_iter);
+    if (r) {
+        ws->dual_valid = false;
+        ws->valid = false;
+        return hip_status(r);
+    }
+    fp_status status = FP_OK;
+    if (fp_status st = record_results(op, ws, info, &status)) {
+        ws->valid = false;
+        return st;
+    }
+    if (fp_status st = record_kept(op, ws, s)) return st;
+    if (oa) {
+        if (out.device == FP_DEVICE_ROCM && out.contiguous()) {
+            if (hipMemcpyAsync(out.data, ws->zsave, (size_t)ws->numel * ws->tsize, hipMemcpyDeviceToDevice, s))
+                return FP_EDEVICE;
+        } else if (fp_status st = stage_out(*ws, ws->zsave, out, s)) {
+            return st;
+        }
+        if (hipStreamSynchronize(s)) return FP_EDEVICE;
+    }
     return status;
 }
 
@@ -795,17 +882,6 @@ fp_status op_check(const fp_op *op, int *row_parallel_only) {
 // Derivatives at a kept solve (the reference's jvp_impl / vjp_impl,
 // src/api/api.cpp:584-760, over the device derivatives of deriv.hip)
 // ---------------------------------------------------------------------------
-
-struct DevGuard {  // the workspace's device current for the call
-    int dev, was;
-    explicit DevGuard(int d) : dev(d), was(d) {
-        (void)hipGetDevice(&was);
-        if (was != dev) (void)hipSetDevice(dev);
-    }
-    ~DevGuard() {
-        if (was != dev) (void)hipSetDevice(was);
-    }
-};
 
 fp_status check_like(const fp_workspace &ws, const Arr &a) {
     if (a.dtype != ws.dtype) return FP_EDTYPE;
@@ -1220,6 +1296,11 @@ FP_API fp_status fp_workspace_info(const fp_workspace *ws, fp_info *info) {
     if (!ws || !info || !ws->info_valid) return FP_EINVAL;
     *info = ws->info;
     return ws->info_status;
+}
+
+FP_API fp_status fp_workspace_refine(const fp_op *op, fp_workspace *ws, double tol, int max_iter, fp_array *out,
+                                     fp_info *info) {
+    return guard([&] { return refine_impl(op, ws, tol, max_iter, out, info); });
 }
 
 FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, double *smoothing) {

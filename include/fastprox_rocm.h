@@ -144,6 +144,12 @@ FP_API fp_status fp_workspace_get_stream(const fp_workspace *ws, void **stream);
 /* include/fastprox.h:732, 769, 773-774, 779-780 */
 FP_API fp_status fp_workspace_info(const fp_workspace *ws, fp_info *info);
 FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, double *smoothing);
+/* include/fastprox.h:759-771: the kept call solved again from its saved input
+ * and its dual (a warm start) to tol within max_iter; the derivatives then
+ * read the refined point and the results and dual report it; out (x's shape
+ * and dtype) receives the refined z when not NULL. */
+FP_API fp_status fp_workspace_refine(const fp_op *op, fp_workspace *ws, double tol, int max_iter,
+                                     fp_array *out, fp_info *info);
 FP_API fp_status fp_workspace_image_info(const fp_workspace *ws, double *gaps, int *iters, double *zerrs,
                                          int64_t count);
 FP_API fp_status fp_vjp_device(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx,

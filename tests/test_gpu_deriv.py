@@ -306,3 +306,35 @@ def test_torch_backward_aniso_weights(cuda):
         assert _rel(be.grad[:, b].cpu().numpy(), v.gspatial) <= 1e-12
         gw += v.gw
     np.testing.assert_allclose(wt.grad.cpu().numpy(), gw, rtol=1e-10, atol=1e-12)
+
+
+def test_refine_then_derivatives(cuda):
+    """fp_workspace_refine (fastprox.h:759-771): the kept call solved again from
+    its input and dual to a tighter tolerance is the warm-started solve of the
+    oracle to the bit (float64); the derivatives then read the refined point,
+    and the first call's z is untouched."""
+    x = image((2, 40, 46), 71)
+    op, ws, xt, z, dual, gaps = _kept(x, 0.15, [1.0, 1.0], None, 1e-4)
+    z_first = z.clone()
+    zr = torch.empty_like(z)
+    st, gap, its = ws.refine(1e-11, 10 ** 6, out=zr)
+    assert st == 0
+    assert torch.equal(z, z_first)
+    dual_loose = dual.cpu().numpy()
+    gaps2, its2, _ = ws.image_info(2)
+    d2 = torch.empty_like(dual)
+    ws.get_dual(d2)
+    for b in range(2):
+        r = oracle.tv_iso(x[b], 0.15, tol=1e-11, max_iter=10 ** 6, warm=dual_loose[:, b])
+        assert r.iters == its2[b]
+        np.testing.assert_array_equal(zr[b].cpu().numpy(), r.z)
+        np.testing.assert_array_equal(d2[:, b].cpu().numpy(), r.dual)
+    ws.set_deriv_tol(1e-12, 100000)
+    g = torch.tensor(np.random.default_rng(3).normal(size=x.shape), device="cuda")
+    gx = torch.empty_like(g)
+    ws.vjp(g, gx)
+    zc, dc = zr.cpu().numpy(), d2.cpu().numpy()
+    for b in range(2):
+        v = oracle.iso_deriv(x[b], zc[b], dc[:, b], gaps2[b], 0, 0.15, g=g[b].cpu().numpy(), deriv_tol=1e-12,
+                             deriv_max_iter=100000)
+        assert _rel(gx[b].cpu().numpy(), v.out) <= 1e-8
