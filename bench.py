@@ -75,8 +75,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB five times (the HBM counters' scale for the traffic figure)")
-    ap.add_argument("--legs", default="uneven,aniso,3d,backward,serve",
-                    help="the other configurations to measure after the batched one: any of uneven (256 uneven "
+    ap.add_argument("--legs", default="b256,uneven,aniso,3d,backward,serve",
+                    help="the other configurations to measure after the batched one: any of b256 (the same "
+                         "images, 256 per batch), uneven (256 uneven "
                          "images with pixel weights), aniso (64 x 512^2 f32 and f64), 3d (16 x 128^3), "
                          "backward ((16,3,256,256) "
                          "autograd), serve (the request trace); '' for none")
@@ -183,6 +184,8 @@ def main():
 
     legs = [v for v in args.legs.split(",") if v]
     extra = {}
+    if "b256" in legs and S == 512:
+        extra["batched_b256"] = leg_b256(args, plain)
     if "uneven" in legs:
         extra["uneven_batch"] = leg_uneven(args)
     if "aniso" in legs:
@@ -246,6 +249,34 @@ def _sync_time(fn, steps):
         r = fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps, r
+
+
+def leg_b256(args, plain, B=256, S=512):
+    """The headline workload at 256 images per batch (the same generator)."""
+    import torch
+
+    from fastprox_amd import Op
+    from fastprox_amd.synth import rectangles
+    from fastprox_amd.torch import Batch
+    x = rectangles(B, (S, S), n=12, sigma=0.1, seed=0, dtype=np.float32)
+    xs = torch.tensor(x, device="cuda")
+    zs = torch.empty_like(xs)
+    ds = torch.empty((B, 2, S, S), dtype=torch.float32, device="cuda")
+    bt = Batch(Op(True, 2), torch.float32, [(S, S)] * B)
+    xl, zl, dl = list(xs), list(zs), list(ds)
+
+    def run():
+        return bt.solve(xl, zl, dl, [args.gamma] * B, tol=args.tol, max_iter=args.max_iter)
+    run()
+    sec, st = _sync_time(run, 2)
+    gaps, its, zerrs, conv = bt.image_info()
+    out = {"workload": f"isotropic TV, batch {B} x {S}x{S} float32 (the headline generator, seed 0), gamma "
+                       f"{args.gamma}, tol {args.tol}", "images_per_s": B / sec, "ms_per_batch": sec * 1e3,
+           "iters_mean": float(its.mean()), "iters_max": int(its.max()), "converged": bool(conv.all()),
+           "speedup_vs_plain": (B / sec / plain) if plain else None}
+    log(f"batched b256: {out}")
+    del xs, zs, ds, bt
+    return out
 
 
 def leg_uneven(args, n=256, seed=0):
