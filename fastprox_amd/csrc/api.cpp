@@ -1778,6 +1778,16 @@ FP_API fp_status fpr_server_stats(const fpr_server *s, int64_t *launches) {
     return FP_OK;
 }
 
+FP_API fp_status fpr_server_get_kernel(const fpr_server *s, int k, int *kind, int *tt, int *ry) {
+    if (!s || (k != 2 && k != 3)) return FP_EINVAL;
+    const fpr::KernelChoice *kc = s->s->kernel(k - 2);
+    if (!kc) return FP_EINVAL;
+    if (kind) *kind = kc->s3 ? 3 : (kc->generic ? 1 : (kc->stream ? 0 : 2));
+    if (tt) *tt = kc->TT;
+    if (ry) *ry = kc->RY;
+    return FP_OK;
+}
+
 FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs) {
     if (!b) return FP_EINVAL;
     if (launches) *launches = b->eng ? b->eng->stats.launches : 0;

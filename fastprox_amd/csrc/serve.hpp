@@ -77,6 +77,7 @@ public:
     int64_t footprint(const ServeRequest &r) const;
     bool release_dual(int64_t id);
     bool evict_one_dual();
+    const KernelChoice &kernel() const { return kc_; }
     int steps_per_sync = 8;
     hipStream_t stream = nullptr;
     int64_t launches = 0;
@@ -144,6 +145,8 @@ public:
     double now() const;
     bool release_dual(int64_t id);
     int64_t launches() const;
+    // The step kernel of class c (0: 2D, 1: 3D); null when c is not served.
+    const KernelChoice *kernel(int c) const { return pools_[c] ? &pools_[c]->kernel() : nullptr; }
 
 private:
     std::unique_ptr<SlotPool> pools_[2];

@@ -135,6 +135,10 @@ __device__ void walk(ImgState &S, const double *gaps, const double *dots, int TT
                      double eta) {
     if (S.mode == kOutput || S.mode == kOutputNaN) {
         S.it += S.cnt;
+        if (S.mode == kOutput && S.cnt > 0) {  // the output launch stored its iterate
+            S.cur ^= 1;
+            S.fresh = 0;
+        }
         S.converged = S.mode == kOutputNaN ? 1 : (S.gp <= S.target);
         if (S.mode == kOutputNaN) S.it = 0;
         finish(S, N, u_round, eta);
@@ -973,9 +977,8 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
                 }
             }
         }
-        return;
     }
-    if (cnt == 0) return;
+    if (cnt == 0) return;  // an output launch of cnt > 0 levels also stores the iterate (the kept dual)
     T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N;
 #pragma unroll
     for (int r = TT; r < RY - TT; ++r)
@@ -1288,7 +1291,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                             ((T *)D.q)[D.q_cstride + so] = (T)(w1d * (ul - (double)u1n));
                         }
                     }
-                } else if (st && cnt > 0) {
+                }
+                if (st && cnt > 0) {  // the iterate (also of an output launch: the kept dual)
                     Uo[so] = u0n;
                     Uo[N + so] = u1n;
                     Uo[2 * N + so] = v0n;

@@ -56,6 +56,8 @@ def _declare(L):
     L.fpr_server_release.argtypes = [P, C.c_int64]
     L.fpr_server_stats.restype = C.c_int
     L.fpr_server_stats.argtypes = [P, C.POINTER(C.c_int64)]
+    L.fpr_server_get_kernel.restype = C.c_int
+    L.fpr_server_get_kernel.argtypes = [P, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L._serve_declared = True
 
 
@@ -133,6 +135,14 @@ class Server:
         n = C.c_int64()
         _check(lib().fpr_server_stats(self.h, C.byref(n)), "fpr_server_stats")
         return n.value
+
+    def kernel(self, k):
+        """(kind, TT, RY) of the step kernel serving k-dimensional requests
+        (fpr_server_get_kernel)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().fpr_server_get_kernel(self.h, int(k), C.byref(a), C.byref(b), C.byref(c)),
+               "fpr_server_get_kernel")
+        return a.value, b.value, c.value
 
     def drain(self):
         """Steps until every submitted request is done; their results."""

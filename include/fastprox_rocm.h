@@ -332,6 +332,10 @@ FP_API int64_t   fpr_server_poll(fpr_server *s, fpr_result *out, int64_t max);
 FP_API double    fpr_server_clock(const fpr_server *s);
 FP_API fp_status fpr_server_release(fpr_server *s, int64_t id);
 FP_API fp_status fpr_server_stats(const fpr_server *s, int64_t *launches);
+/* The step kernel serving k-dimensional requests (kind as fpr_batch_get_kernel:
+ * 0 streaming 2D, 1 generic, 2 block, 3 streaming 3D; tt iterations per launch,
+ * ry rows per segment or slices per slab). No counterpart in the reference. */
+FP_API fp_status fpr_server_get_kernel(const fpr_server *s, int k, int *kind, int *tt, int *ry);
 
 #ifdef __cplusplus
 }

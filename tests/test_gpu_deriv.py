@@ -213,6 +213,29 @@ def test_aniso_deriv_matches_oracle(cuda, shape, beta):
     assert np.abs(_q_to_u(dual[:, 0].cpu().numpy(), w) - uc[0]).max() <= 1e-12
 
 
+@pytest.mark.parametrize("kernel", [(0, 2, 32), (0, 6, 32), (0, 8, 16), (2, 2, 16)])
+def test_kept_iterate_is_the_output_dual(cuda, kernel):
+    """The stop can land inside a launch of several iterations: the kept dual
+    iterate (copy_dual_iterate) is then the one the output dual and z come
+    from, not the launch's start."""
+    from fastprox_amd.torch import Batch
+    rng = np.random.default_rng(8)
+    shape = (44, 70)
+    x = image((6,) + shape, 8, scale=0.2)
+    w = [0.9, 1.2]
+    op = Op(False, 2, w)
+    bt = Batch(op, torch.float64, [shape] * 6, kernel=kernel)
+    xs = [torch.tensor(v, device="cuda") for v in x]
+    zs = [torch.empty_like(v) for v in xs]
+    ds = [torch.empty((2,) + shape, dtype=torch.float64, device="cuda") for _ in xs]
+    gam = list(rng.uniform(0.05, 0.3, 6))
+    bt.solve(xs, zs, ds, gam, tol=1e-11, max_iter=100000)
+    ue = torch.empty((6, 2) + shape, dtype=torch.float64, device="cuda")
+    bt.copy_dual_iterate(ue)
+    for b in range(6):
+        assert np.abs(_q_to_u(ds[b].cpu().numpy(), w) - ue[b].cpu().numpy()).max() <= 1e-12
+
+
 def test_aniso_workspace_vjp_matches_structure(cuda):
     """fp_vjp on an anisotropic keep_structure workspace reads the same
     per-edge dual as the autograd path."""

@@ -16,15 +16,15 @@ from fastprox_amd.synth import crop_of, serving_sources, serving_trace
 
 pytestmark = pytest.mark.gpu
 
-KERNEL = {2: (0, 4, 32), 3: (3, 1, 32)}  # the server's choices: streaming TT=4 SEG=32; 3D streaming, 32 slices
-
-
-def _alone(x, gamma, tol, warm=None):
+def _alone(x, gamma, tol, warm=None, kernel=None):
+    """The plain solve of x alone, on the step kernel the server uses for
+    x's class (the iterates do not depend on the kernel; the gaps' partial
+    sums follow its tiling)."""
     k = x.dim()
     xc = x.contiguous()
     op = Op(True, k)
     ws = Workspace(op, xc)
-    ws.set_kernel(*KERNEL[k])
+    ws.set_kernel(*kernel[k])
     if warm is not None:
         ws.set_dual(warm)
         ws.set_warm(True)
@@ -58,12 +58,13 @@ def test_serve_same_alone_and_in_the_mix(cuda):
                 srv.step()
     res = srv.drain()
     torch.cuda.synchronize()
+    kern = {2: srv.kernel(2), 3: srv.kernel(3)}
     assert len(res) == len(trace)
     admitted_late = 0
     first_done = min(d["t_done"] for d in res)
     for d in res:
         x, r = reqs[d["id"]]
-        z, its, gap, st, _ = _alone(x, r.gamma, r.tol)
+        z, its, gap, st, _ = _alone(x, r.gamma, r.tol, kernel=kern)
         print(f"request {d['id']} k={r.k} side={r.side} iters {d['iters']} vs {its}", flush=True)
         assert d["status"] == 0
         assert d["iters"] == its
@@ -86,6 +87,7 @@ def test_serve_admission_by_memory(cuda):
         xs.append(x)
         outs.append(out)
     res = sorted(srv.drain(), key=lambda d: d["id"])
+    kern = {2: srv.kernel(2)}
     assert len(res) == 7
     # never more than two live at once (a request is live from its admission to its completion)
     for d in res:
@@ -93,7 +95,7 @@ def test_serve_admission_by_memory(cuda):
         assert live <= 2
     assert any(d["t_admit"] > min(e["t_done"] for e in res) for d in res)
     for j, d in enumerate(res):
-        z, its, gap, st, _ = _alone(xs[j], 0.05 + 0.02 * j, 1e-5)
+        z, its, gap, st, _ = _alone(xs[j], 0.05 + 0.02 * j, 1e-5, kernel=kern)
         assert torch.equal(outs[j], z) and d["iters"] == its
 
 
@@ -107,8 +109,9 @@ def test_serve_warm_start_from_a_kept_dual(cuda):
     zb = torch.empty_like(za)
     srv.submit(x, 0.12, tol=1e-6, out=zb, warm_from=a)
     (rb,) = srv.drain()
-    _, _, _, _, dual_a = _alone(x, 0.1, 1e-5)
-    z, its, gap, st, _ = _alone(x, 0.12, 1e-6, warm=dual_a)
+    kern = {2: srv.kernel(2)}
+    _, _, _, _, dual_a = _alone(x, 0.1, 1e-5, kernel=kern)
+    z, its, gap, st, _ = _alone(x, 0.12, 1e-6, warm=dual_a, kernel=kern)
     assert torch.equal(zb, z) and rb["iters"] == its
     srv.release(a)
     with pytest.raises(Exception):
