@@ -1046,13 +1046,14 @@ __device__ __forceinline__ T from_right(T v) {  // lane i receives lane i + 1's 
 
 // The projection factor min(1, r / |c|) of the isotropic ball. float64
 // follows the reference's correctly rounded sqrt and division
-// (tv_iso.cpp:509-510); float32 uses the hardware's square root and
-// reciprocal (about 1 ulp each), its iterate being float32 anyway.
+// (tv_iso.cpp:509-510); float32 uses the hardware's reciprocal square root
+// (about 1 ulp, one quarter-rate instruction where a square root and a
+// reciprocal take two), its iterate being float32 anyway; |c| > r is tested
+// as |c|^2 > r^2.
 template <class T>
 __device__ __forceinline__ T ball_scale(T nn, T rp) {
     if constexpr (sizeof(T) == 4) {
-        const float nr = __builtin_amdgcn_sqrtf(nn);
-        return nr > rp ? rp * __builtin_amdgcn_rcpf(nr) : 1.0f;
+        return nn > rp * rp ? rp * __builtin_amdgcn_rsqf(nn) : 1.0f;
     } else {
         const T nr = sqrt(nn);
         return nr > rp ? rp / nr : (T)1;
