@@ -1479,8 +1479,7 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
                 nn += cv[a] * cv[a];
             }
             if (ISO) {
-                const T nr = sqrt(nn), rp = (T)rad[0];
-                const T sc = nr > rp ? rp / nr : (T)1;
+                const T sc = ball_scale(nn, (T)rad[0]);
                 for (int a = 0; a < k; ++a) un[a] = cv[a] * sc;
             } else {
                 for (int a = 0; a < k; ++a) {
@@ -1721,8 +1720,7 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
                     }
                     T un[3], vn[3];
                     if (ISO) {
-                        const T nr = sqrt(nn), rp = (T)rad[0];
-                        const T sc = nr > rp ? rp / nr : (T)1;
+                        const T sc = ball_scale(nn, (T)rad[0]);
                         for (int a = 0; a < 3; ++a) un[a] = c[a] * sc;
                     } else {
                         for (int a = 0; a < 3; ++a) {
