@@ -1801,6 +1801,12 @@ FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on) {
     return FP_OK;
 }
 
+FP_API fp_status fpr_batch_set_graph(fpr_batch *b, int on) {
+    if (!b || !b->eng) return FP_EINVAL;
+    b->eng->graphs = on != 0;
+    return FP_OK;
+}
+
 FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches) {
     if (!b || !b->eng) return FP_EINVAL;
     if (step_ms) *step_ms = b->eng->stats.step_ms;

@@ -6,14 +6,53 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 namespace fpr {
 
-Engine::~Engine() { release(); }
+Engine::~Engine() {
+    release();
+    for (auto &x : tev) (void)hipEventDestroy(x);
+    for (auto &x : done_ev)
+        if (x) (void)hipEventDestroy(x);
+    if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+}
+
+int Engine::ensure_events(int steps, bool timed) {
+    hipError_t e;
+    for (auto &x : done_ev)
+        if (!x && (e = hipEventCreateWithFlags(&x, hipEventDisableTiming))) return e;
+    if (timed && tev.size() != (size_t)(6 * steps)) {
+        drop_graphs();  // graphs record into the events they were captured with
+        for (auto &x : tev) (void)hipEventDestroy(x);
+        tev.assign((size_t)(6 * steps), nullptr);
+        for (auto &x : tev)  // timestamps only: no system-scope fence at each record
+            if ((e = hipEventCreateWithFlags(&x, hipEventDisableSystemFence))) return e;
+    }
+    return hipSuccess;
+}
+
+void Engine::drop_graphs() {
+    for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+    graphs_.clear();
+}
+
+hipGraphExec_t Engine::find_graph(int tail, int slot, int p, int grid, int steps, int timed) const {
+    for (const auto &g : graphs_)
+        if (g.tail == tail && g.slot == slot && g.p == p && g.grid == grid && g.steps == steps && g.timed == timed)
+            return g.exec;
+    return nullptr;
+}
+
+hipStream_t Engine::capture_stream() {
+    if (!cap_stream_ && hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking)) cap_stream_ = nullptr;
+    return cap_stream_;
+}
 
 void Engine::release() {
+    drop_graphs();
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
@@ -129,6 +168,7 @@ int Engine::tile_layout(const KernelChoice &kc, std::vector<ImgDesc> &descs) con
 }
 
 int Engine::set_tail(const KernelChoice &kc, double frac) {
+    drop_graphs();
     tail_frac_ = 0.0;
     if (!(frac > 0.0) || kc.generic != kc_.generic || P_.k != 2) return hipSuccess;
     if (!kernel_available(tsize_, kc.stream, kc.TT, kc.RY)) return hipErrorInvalidValue;
@@ -200,23 +240,34 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     // Optional timing: an event before and after every step and epilogue,
     // read back after each chunk of launches.
     const bool timing = E.timing;
-    std::vector<hipEvent_t> ev;
-    hipEvent_t done_ev[2] = {nullptr, nullptr};
-    if (timing) {
-        ev.resize((size_t)(2 * 3 * steps_per_sync));
-        for (auto &x : ev)
-            if ((err = hipEventCreate(&x))) return err;
-    }
-    for (auto &x : done_ev)
-        if ((err = hipEventCreateWithFlags(&x, hipEventDisableTiming))) return err;
-    auto destroy = [&]() {
-        for (auto &x : ev) (void)hipEventDestroy(x);
-        for (auto &x : done_ev)
-            if (x) (void)hipEventDestroy(x);
-    };
+    if ((err = (hipError_t)E.ensure_events(steps_per_sync, timing))) return err;
+    const std::vector<hipEvent_t> &ev = E.tev;
+    hipEvent_t *done_ev = E.done_ev;
     // Chunks of launches are pipelined: chunk c + 1 is enqueued before the
     // host reads chunk c's count of live tiles, so the device never waits on
     // the host; the chunk enqueued after the last one only meets empty lists.
+    // Graphs: each chunk's launches are captured once per key and replayed.
+    static const bool graph_env = [] {
+        const char *e = std::getenv("FASTPROX_ROCM_GRAPH");
+        return !(e && e[0] == '0');
+    }();
+    const bool use_graphs = E.graphs && graph_env && !timing;  // timing events stay outside graphs
+    // The chunk's launches and its count readback, on stream st.
+    auto record = [&](hipStream_t st, int slot) -> hipError_t {
+        for (int i = 0; i < steps_per_sync; ++i) {
+            A.list_in = list[p];
+            A.count_in = counts + p;
+            A.count_out = counts + (p ^ 1);
+            const size_t eb = (size_t)(3 * (slot * steps_per_sync + i));
+            if (timing) (void)hipEventRecord(ev[eb], st);
+            launch_step<T>(cur_kc, P.iso != 0, P.sw != 0, A, grid, st);
+            if (timing) (void)hipEventRecord(ev[eb + 1], st);
+            launch_epilogue(cur_desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, st);
+            if (timing) (void)hipEventRecord(ev[eb + 2], st);
+            p ^= 1;
+        }
+        return hipMemcpyAsync(hcount + slot, counts + p, sizeof(int), hipMemcpyDeviceToHost, st);
+    };
     auto enqueue = [&](int slot) -> hipError_t {
         if (want_tail && !tail) {
             // Relist the live images on the tail kernel's tiling: the list
@@ -227,26 +278,47 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
             A.desc = desc_tail;
             TT = tail_kc.iters();
             const int live = hcount[slot] > 0 ? hcount[slot] : total_tiles;  // the count just read
-            grid = grid_of(cur_kc, (int)std::min<int64_t>(
-                                       total_tiles_tail, (int64_t)live * total_tiles_tail / std::max(1, total_tiles) + 64));
+            // The tail's tiles, estimated and rounded up to a power of two
+            // (few distinct grids, so few captured graphs).
+            int64_t est = (int64_t)live * total_tiles_tail / std::max(1, total_tiles) + 64, r = 64;
+            while (r < est) r <<= 1;
+            grid = grid_of(cur_kc, (int)std::min<int64_t>(total_tiles_tail, r));
             launch_reset_counts(counts, s);
             launch_initial_list(desc_tail, state, list[p], counts + p, B, s);
         }
-        for (int i = 0; i < steps_per_sync; ++i) {
-            A.list_in = list[p];
-            A.count_in = counts + p;
-            A.count_out = counts + (p ^ 1);
-            const size_t eb = (size_t)(3 * (slot * steps_per_sync + i));
-            if (timing) (void)hipEventRecord(ev[eb], s);
-            launch_step<T>(cur_kc, P.iso != 0, P.sw != 0, A, grid, s);
-            if (timing) (void)hipEventRecord(ev[eb + 1], s);
-            launch_epilogue(cur_desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s);
-            if (timing) (void)hipEventRecord(ev[eb + 2], s);
-            p ^= 1;
-            ++launched;
-        }
+        launched += steps_per_sync;
         E.stats.launches += steps_per_sync;
-        hipError_t e2 = hipMemcpyAsync(hcount + slot, counts + p, sizeof(int), hipMemcpyDeviceToHost, s);
+        hipError_t e2 = hipSuccess;
+        hipStream_t cs = use_graphs ? E.capture_stream() : nullptr;
+        if (cs) {
+            const int t = tail ? 1 : 0, p0 = p;
+            hipGraphExec_t ex = E.find_graph(t, slot, p0, grid, steps_per_sync, timing ? 1 : 0);
+            if (ex) {
+                p ^= steps_per_sync & 1;
+            } else {
+                hipGraph_t g = nullptr;
+                hipError_t e1 = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+                if (!e1) {
+                    e1 = record(cs, slot);
+                    const hipError_t e3 = hipStreamEndCapture(cs, &g);
+                    if (!e1) e1 = e3;
+                }
+                if (!e1) e1 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e1) {
+                    // No graph: this engine launches every kernel from now on.
+                    (void)hipGetLastError();
+                    E.graphs = false;
+                    ex = nullptr;
+                    p = p0;
+                } else {
+                    E.add_graph(Engine::ChunkGraph{t, slot, p0, grid, steps_per_sync, timing ? 1 : 0, ex});
+                }
+            }
+            e2 = ex ? hipGraphLaunch(ex, s) : record(s, slot);
+        } else {
+            e2 = record(s, slot);
+        }
         if (!e2) e2 = hipEventRecord(done_ev[slot], s);
         return e2;
     };
@@ -263,10 +335,7 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
         }
     };
     int slot = 0;
-    if ((err = enqueue(slot))) {
-        destroy();
-        return err;
-    }
+    if ((err = enqueue(slot))) return err;
     for (;;) {
         if ((err = enqueue(slot ^ 1))) break;
         if ((err = hipEventSynchronize(done_ev[slot]))) break;
@@ -285,7 +354,6 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
         if (tail_frac > 0.0 && !tail && (double)hcount[slot] < tail_frac * (double)total_tiles) want_tail = true;
         slot ^= 1;
     }
-    destroy();
     return err == (hipError_t)-1 ? -1 : err;
 }
 

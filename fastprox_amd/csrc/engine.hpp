@@ -76,9 +76,36 @@ public:
     SolveStats stats;
     int steps_per_sync = 8;
     bool timing = false;   // time every step and epilogue launch with HIP events
+    // Replay each chunk of step/epilogue launches from a captured hipGraph
+    // (captured on first use, kept until the engine's configuration changes);
+    // off, or with timing on, every kernel is launched on its own. Results
+    // are the same either way.
+    bool graphs = true;
+
+    // The graph of one chunk: the launches depend only on the engine's
+    // buffers and on (tail, slot, p, grid, steps), so the key names them.
+    struct ChunkGraph {
+        int tail, slot, p, grid, steps, timed;
+        hipGraphExec_t exec;
+    };
+    hipGraphExec_t find_graph(int tail, int slot, int p, int grid, int steps, int timed) const;
+    void add_graph(const ChunkGraph &g) {
+        if (graphs_.size() >= 64) drop_graphs();
+        graphs_.push_back(g);
+    }
+    hipStream_t capture_stream();  // a stream of its own that chunks are captured on (null if none)
+    // The solve loop's events, kept across solves: with timing, 3 per launch
+    // of two chunks of steps launches; and each chunk's completion. Returns a
+    // hipError_t.
+    int ensure_events(int steps, bool timed);
+    std::vector<hipEvent_t> tev;
+    hipEvent_t done_ev[2] = {nullptr, nullptr};
 
 private:
     void release();
+    void drop_graphs();
+    std::vector<ChunkGraph> graphs_;
+    hipStream_t cap_stream_ = nullptr;
     int tile_layout(const KernelChoice &kc, std::vector<ImgDesc> &descs) const;
     int tsize_ = 4;
     KernelChoice tail_kc_{};

@@ -1317,8 +1317,18 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     }
 }
 
+// Waves per SIMD the register allocation is held to (float32: four where
+// the variant fits in 128 VGPRs without spilling, three otherwise; float64:
+// the compiler's choice). More resident waves hide the per-row load latency.
 template <class T, bool ISO, bool SW, int TT>
-__global__ __launch_bounds__(256) void stream2d_kernel(StepArgs<T> A, int SEG) {
+constexpr int stream2d_waves() {
+    if (sizeof(T) != 4) return 1;
+    return (TT <= 4 || (TT == 6 && !(ISO && SW))) ? 4 : 3;
+}
+
+template <class T, bool ISO, bool SW, int TT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stream2d_waves<T, ISO, SW, TT>())))
+void stream2d_kernel(StepArgs<T> A, int SEG) {
     __shared__ StreamAcc<TT> accs[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;

@@ -251,6 +251,8 @@ def _declare_batch(L):
     L.fpr_batch_stats.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fpr_batch_set_timing.restype = C.c_int
     L.fpr_batch_set_timing.argtypes = [P, C.c_int]
+    L.fpr_batch_set_graph.restype = C.c_int
+    L.fpr_batch_set_graph.argtypes = [P, C.c_int]
     L.fpr_batch_timing.restype = C.c_int
     L.fpr_batch_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.fpr_batch_set_tail.restype = C.c_int
@@ -337,6 +339,10 @@ class Batch:
 
     def set_timing(self, on):
         _check(lib().fpr_batch_set_timing(self.h, int(on)), "fpr_batch_set_timing")
+
+    def set_graph(self, on):
+        """Replay captured hipGraphs of the launch chunks (fpr_batch_set_graph)."""
+        _check(lib().fpr_batch_set_graph(self.h, int(on)), "fpr_batch_set_graph")
 
     def timing(self):
         """(step-kernel ms, epilogue ms, step launches) of the last solve."""

@@ -229,6 +229,11 @@ FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_sy
  * step launches timed. */
 FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on);
 FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches);
+/* Each chunk of a solve's step and epilogue launches is captured once as a
+ * hipGraph and replayed (on by default; timing turns it off for the timed
+ * solves); on = 0 launches every kernel on its own. Results are the same
+ * either way. No counterpart in the reference. */
+FP_API fp_status fpr_batch_set_graph(fpr_batch *b, int on);
 /* The tail kernel of a batch or workspace (2D, streaming kernels): once the
  * live tiles of a solve fall below frac of all its tiles, the remaining
  * launches use the streaming kernel with tt iterations and segments of ry

@@ -342,6 +342,40 @@ def test_tail_kernel_switch_keeps_results(cuda, dtype):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_graph_replays_match_eager_launches(cuda, dtype):
+    """Launch chunks replayed from captured hipGraphs (fpr_batch_set_graph)
+    give the eager launches' bits on every replay, with eager work of the
+    program and synchronisations between the replays, through a tail switch
+    and across solves with other inputs."""
+    shapes = [(150, 170), (64, 300), (200, 90), (33, 40)]
+    rng = np.random.default_rng(71)
+    xs = [torch.tensor(image((1,) + s, 70 + i)[0], dtype=dtype, device=cuda) for i, s in enumerate(shapes)]
+    gam = [0.05, 0.2, 0.1, 0.4]
+
+    def solve(bt, xl):
+        zs = [torch.empty_like(v) for v in xl]
+        ds = [torch.empty((2,) + tuple(v.shape), dtype=dtype, device=cuda) for v in xl]
+        bt.solve(xl, zs, ds, gam, tol=1e-7, max_iter=100000)
+        return [z.cpu() for z in zs], [d.cpu() for d in ds], bt.image_info()[1]
+
+    eager = Batch(Op(True, 2), dtype, shapes, kernel=(0, 4, 32))
+    eager.set_tail(8, 8, 0.6)
+    eager.set_graph(False)
+    graph = Batch(Op(True, 2), dtype, shapes, kernel=(0, 4, 32))
+    graph.set_tail(8, 8, 0.6)
+    for rep in range(4):
+        xl = [v * (1.0 + 0.25 * rep) for v in xs]
+        want = solve(eager, xl)
+        junk = torch.tensor(rng.normal(size=(256, 256)), dtype=dtype, device=cuda)
+        (junk @ junk).sum().item()  # the program's own eager work between replays
+        got = solve(graph, xl)
+        torch.cuda.synchronize()
+        for a, b in zip(want[0] + want[1], got[0] + got[1]):
+            assert torch.equal(a, b), f"replay {rep}"
+        np.testing.assert_array_equal(want[2], got[2])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("iso", [True, False])
 def test_3d_streaming_kernel_matches_generic(cuda, dtype, iso):
     """The 3D streaming kernel (slabs of slices, LDS rows, DPP columns) runs
