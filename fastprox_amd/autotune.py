@@ -10,7 +10,10 @@ with or without a tail kernel; the 3D streaming kernel with slabs of SLICES
 slices. The fastest candidate (wall time of the whole solve) is kept. Results
 do not depend on the choice (every kernel computes the same iterates).
 
-    python -m fastprox_amd.autotune [--out fastprox_amd/tuning.txt] [--quick]
+    python -m fastprox_amd.autotune [--out fastprox_amd/tuning.txt] [--quick] [--classes 0,1,...]
+
+--classes measures only those classes (indices into CLASSES) and replaces
+their lines in the table at --out, keeping the others.
 """
 
 from __future__ import annotations
@@ -27,7 +30,7 @@ HERE = pathlib.Path(__file__).resolve().parent
 SIZES_2D = (128, 256, 512, 1024)
 SIZES_3D = (32, 64, 128)
 MAIN_2D = [(tt, seg) for tt in (2, 4, 6, 8) for seg in (16, 32, 64)]
-TAILS = [(0, 0, 0.0), (4, 16, 0.5), (4, 8, 0.3), (8, 16, 0.5)]
+TAILS = [(0, 0, 0.0), (4, 16, 0.5), (4, 8, 0.3), (6, 16, 0.5), (8, 16, 0.5), (8, 8, 0.3)]
 SLICES_3D = (8, 16, 32, 64)
 
 
@@ -98,28 +101,52 @@ def tune_class(k, dtype, iso, sw, sizes, log, quick=False):
     return rows
 
 
+# (k, dtype bytes, iso, spatial weights)
+CLASSES = [(2, 4, True, False), (2, 4, True, True), (2, 8, True, False), (2, 4, False, False), (2, 8, False, False),
+           (3, 4, True, False), (3, 4, False, False)]
+
+
+def _read_table(path):
+    """{class key: [line, ...]} of an existing table."""
+    out = {}
+    try:
+        lines = pathlib.Path(path).read_text().splitlines()
+    except OSError:
+        return out
+    for ln in lines:
+        v = ln.split()
+        if not v or ln.startswith("#"):
+            continue
+        out.setdefault((int(v[0]), int(v[1]), bool(int(v[2])), bool(int(v[3]))), []).append(ln)
+    return out
+
+
 def main(argv=None):
     import torch
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=str(HERE / "tuning.txt"))
     ap.add_argument("--quick", action="store_true", help="a few candidates per class (a smoke run)")
+    ap.add_argument("--classes", default=None, help="indices into CLASSES to measure (the rest are kept)")
     args = ap.parse_args(argv)
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
-    classes = [(2, torch.float32, True, False), (2, torch.float32, True, True), (2, torch.float64, True, False),
-               (2, torch.float32, False, False), (2, torch.float64, False, False), (3, torch.float32, True, False),
-               (3, torch.float32, False, False)]
-    rows = []
-    for k, dt, iso, sw in classes:
-        rows += tune_class(k, dt, iso, sw, SIZES_2D if k == 2 else SIZES_3D, log, args.quick)
+    pick = range(len(CLASSES)) if args.classes is None else [int(v) for v in args.classes.split(",")]
+    kept = _read_table(args.out) if args.classes is not None else {}
+    lines = {}
+    for i in pick:
+        k, tb, iso, sw = CLASSES[i]
+        dt = torch.float32 if tb == 4 else torch.float64
+        rows = tune_class(k, dt, iso, sw, SIZES_2D if k == 2 else SIZES_3D, log, args.quick)
+        lines[CLASSES[i]] = [f"{r[0]} {r[1]} {r[2]} {r[3]} {r[4]} {r[5]} {r[6]} {r[7]} {r[8]} {r[9]} {r[10]:g}   "
+                             f"# {r[11]:.1f}" for r in rows]
     name = torch.cuda.get_device_name(0)
     with open(args.out, "w") as f:
         f.write(f"# libfastprox_rocm tuning table, measured by fastprox_amd/autotune.py on {name}\n")
         f.write("# k dtype_bytes iso spatial max_pixels kind TT RY tail_TT tail_RY tail_frac   (images/s)\n")
-        for r in rows:
-            f.write(f"{r[0]} {r[1]} {r[2]} {r[3]} {r[4]} {r[5]} {r[6]} {r[7]} {r[8]} {r[9]} {r[10]:g}   "
-                    f"# {r[11]:.1f}\n")
+        for c in CLASSES:
+            for ln in lines.get(c, kept.get(c, [])):
+                f.write(ln + "\n")
     log(f"wrote {args.out}")
 
 

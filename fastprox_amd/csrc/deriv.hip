@@ -3,7 +3,8 @@
 // Isotropic TV (src/ops/tv_iso.cpp:551-631 save, :672-1067 Sys, cg, jvp_img,
 // vjp_img): z - d + G^T u = 0 with G = W D. Pixels with the dual strictly
 // inside the ball (|u_p| < r_p (1 - 1e-9)), or a gradient at the rounding
-// level of the data (|g_p| <= w_max 64 2.2e-16 max|d|), are flat: z is
+// level of the data (|g_p| <= w_max 64 2.2e-16 max|d|; a float32 dual: 64
+// eps and max(1e-9, 64 eps) instead), are flat: z is
 // constant across their forward edges, so dz lives on the connected regions
 // of those edges (one unknown per region). On the others the dual moves on
 // the sphere, du_p = H_p (G dz)_p with H_p = r_p / max(|g_p|, mu) (I - n n^T)
@@ -78,7 +79,7 @@ struct DImg {
     double thr, rel, mu;
     double wsave[3];  // iso: the structure's weights (the solve's active axes)
     double wa[3];     // iso: the system's weights; aniso: w
-    double floor_, eta;  // aniso flat test
+    double floor_, eta;  // the flat test: the jumps' rounding floor (aniso), the dual margin
     int e, nan, from_dual, use_dual;
     int rbase, rcount, bbase, bcount;
 };
@@ -144,8 +145,14 @@ __device__ double block_max(double v, double *sh) {
 // threshold, the certified relative error and the curvature cap
 // (tv_iso.cpp:557-569); aniso: the rounding floor of the jumps
 // (_tv_torch.py:348-350).
+// iso_floor_eps, iso_eta: the flat test's rounding floor (w_max 64
+// iso_floor_eps max|d|) and dual margin: for a float64 dual the C kernels'
+// 2.2e-16 and 1e-9 (tv_iso.cpp:109, 565); for a float32 dual the torch
+// device path's 2^-23 and max(1e-9, 64 2^-23) (_tv_torch.py:192-195, 404-410),
+// since a float32 dual sits on its sphere only to float32 rounding.
 template <class T>
-__global__ __launch_bounds__(kThr) void prep_kernel(Geo G, int iso, double eps, const T *x, DImg *di) {
+__global__ __launch_bounds__(kThr) void prep_kernel(Geo G, int iso, double eps, double iso_floor_eps, double iso_eta,
+                                                    const T *x, DImg *di) {
     __shared__ double sh[kThr];
     const int b = blockIdx.x;
     DImg &D = di[b];
@@ -171,7 +178,8 @@ __global__ __launch_bounds__(kThr) void prep_kernel(Geo G, int iso, double eps, 
             D.from_dual = D.from_dual && !D.nan;
             D.use_dual = D.use_dual && !D.nan;
             const double gap = D.nan ? 0.0 : D.gap_s;
-            D.thr = D.from_dual ? wm * 64.0 * 2.2e-16 * m : 0.0;
+            D.thr = D.from_dual ? wm * 64.0 * iso_floor_eps * m : 0.0;
+            D.eta = iso_eta;
             D.rel = ss > 0.0 ? sqrt(2.0 * dmaxd(gap, 0.0) / ss) : 0.0;
             D.mu = D.from_dual ? kCurvRms * wm * sqrt(2.0 * dmaxd(gap, 0.0) / (double)G.N) : 0.0;
         } else {
@@ -228,7 +236,7 @@ __global__ __launch_bounds__(kThr) void flat_iso_kernel(Geo G, const T *x, const
             const double ua = ldexp(ld(u, b * u_img + a * u_plane + p), -D.e);
             uu += ua * ua;
         }
-        fl = sqrt(uu) < radius(D, sw, sw_img, b, p) * (1.0 - kDualEta);
+        fl = sqrt(uu) < radius(D, sw, sw_img, b, p) * (1.0 - D.eta);
     }
     flat[gp] = (uint8_t)((fl ? 1 : 0) | (any ? 2 : 0));
     parent[gp] = (int)gp;
@@ -1150,7 +1158,9 @@ int build(Deriv &d, hipStream_t s) {
     }
     if (hipMemcpyAsync(d.di, d.hdi.data(), sizeof(DImg) * (size_t)B, hipMemcpyHostToDevice, s)) return FP_EDEVICE;
     const double eps = sizeof(T) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
-    hipLaunchKernelGGL(prep_kernel<T>, dim3(B), dim3(kThr), 0, s, G, (int)sp.iso, eps, x, d.di);
+    const double floor_eps = sizeof(T) == 4 ? eps : 2.2e-16;
+    const double iso_eta = sizeof(T) == 4 ? std::max(kDualEta, 64.0 * eps) : kDualEta;
+    hipLaunchKernelGGL(prep_kernel<T>, dim3(B), dim3(kThr), 0, s, G, (int)sp.iso, eps, floor_eps, iso_eta, x, d.di);
     const int gpx = grid_of(BN);
     if (sp.iso)
         hipLaunchKernelGGL(flat_iso_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, x, z, u, sp.u_img, sp.u_plane, sw,

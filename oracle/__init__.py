@@ -42,6 +42,8 @@ def lib():
         L.or_iso_deriv.restype = C.c_int
         L.or_iso_deriv.argtypes = [C.c_int, i64p, dp, dp, C.c_double, dp, dp, dp, C.c_double, C.c_int, C.c_int,
                                    dp, C.c_double, dp, C.c_double, C.c_int, dp, dp, dp, ip, i64p]
+        L.or_deriv_set_eps.restype = None
+        L.or_deriv_set_eps.argtypes = [C.c_double, C.c_double]
         L.or_aniso_deriv.restype = C.c_int
         L.or_aniso_deriv.argtypes = [C.c_int, i64p, dp, dp, C.c_double, dp, dp, dp, C.c_double, C.c_double,
                                      C.c_int, dp, C.c_double, dp, dp, dp, dp, i64p]
@@ -147,11 +149,20 @@ class Deriv:
 
 
 def iso_deriv(x, z, u, gap, scale_exp, gamma, w=None, alpha=None, *, g=None, dx=None, dgamma=0.0, dparams=None,
-              deriv_tol=0.0, deriv_max_iter=10000):
+              deriv_tol=0.0, deriv_max_iter=10000, dual_dtype=np.float64):
     """The isotropic derivative at a solve's output (z, dual u, gap, scale
     exponent) of one image x: with g, the VJP (out = J^T g; gw, gspatial,
     ggamma the parameter gradients); else the JVP along (dx, dgamma,
-    dparams)."""
+    dparams). dual_dtype float32: the flat test of a float32 dual (the torch
+    device path's 64 eps rules) instead of the C kernels'."""
+    if np.dtype(dual_dtype) == np.float32:
+        e32 = float(np.finfo(np.float32).eps)
+        lib().or_deriv_set_eps(e32, max(1e-9, 64 * e32))
+        try:
+            return iso_deriv(x, z, u, gap, scale_exp, gamma, w, alpha, g=g, dx=dx, dgamma=dgamma, dparams=dparams,
+                             deriv_tol=deriv_tol, deriv_max_iter=deriv_max_iter)
+        finally:
+            lib().or_deriv_set_eps(2.2e-16, 1e-9)
     x, z, u = _c(x), _c(z), _c(u)
     k = x.ndim
     w = np.ones(k) if w is None else _c(np.broadcast_to(np.asarray(w, np.float64), (k,)))

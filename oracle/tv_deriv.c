@@ -92,6 +92,18 @@ static int64_t uf_find(int32_t *parent, int64_t i) {
     return i;
 }
 
+/* The flat test's constants: the C kernels' by default (a float64 dual:
+ * the rounding floor w_max 64 2.2e-16 max|d|, tv_iso.cpp:565, and
+ * DUAL_ETA); for a float32 dual the torch device path's, w_max 64 eps
+ * max|x| and max(1e-9, 64 eps) with eps = 2^-23 (_tv_torch.py:192-195,
+ * 404-410). or_deriv_set_eps(2.2e-16, DUAL_ETA) restores the default. */
+static double g_floor_eps = 2.2e-16, g_dual_eta = DUAL_ETA;
+
+void or_deriv_set_eps(double floor_eps, double eta) {
+    g_floor_eps = floor_eps;
+    g_dual_eta = eta;
+}
+
 /* TVIso::save (tv_iso.cpp:551-631). wsave: the active axis weights of
  * prox_img; d, z, u at the solved scale; u NULL for a zero dual. */
 static int build_structure(sys_t *S, const double *wsave, const double *d, const double *u, double gap,
@@ -105,7 +117,7 @@ static int build_structure(sys_t *S, const double *wsave, const double *d, const
         dd += d[i] * d[i];
     }
     for (int a = 0; a < k; ++a) wmax = dmax(wmax, wsave[a]);
-    const double thr = from_dual ? wmax * 64.0 * 2.2e-16 * scale : 0.0;
+    const double thr = from_dual ? wmax * 64.0 * g_floor_eps * scale : 0.0;
     S->rel = dd > 0.0 ? sqrt(2.0 * dmax(gap, 0.0) / dd) : 0.0;
     S->mu = from_dual ? CURV_RMS * wmax * sqrt(2.0 * dmax(gap, 0.0) / (double)N) : 0.0;
     int32_t *parent = S->label;
@@ -125,7 +137,7 @@ static int build_structure(sys_t *S, const double *wsave, const double *d, const
         if (from_dual && !fl) {
             double uu = 0.0;
             for (int a = 0; a < k; ++a) uu += S->u[a * N + p] * S->u[a * N + p];
-            fl = sqrt(uu) < rad(S, p) * (1.0 - DUAL_ETA);
+            fl = sqrt(uu) < rad(S, p) * (1.0 - g_dual_eta);
         }
         S->flat[p] = (unsigned char)fl;
         if (!fl || !any) continue;

@@ -205,3 +205,32 @@ def test_aniso_deriv_region_means_closed_form():
     left, right = g[:, :7].mean(), g[:, 7:].mean()
     np.testing.assert_allclose(v.out[:, :7], left, rtol=0, atol=1e-14)
     np.testing.assert_allclose(v.out[:, 7:], right, rtol=0, atol=1e-14)
+
+
+def test_iso_float32_dual_flat_rule():
+    """A float32 dual sits on its sphere only to float32 rounding: the C
+    kernels' margin (1e-9) then takes about half the sphere's pixels for
+    strictly inside and merges regions across jumps, while the float32 rule
+    (64 eps margins, _tv_torch.py:192-195, 404-410) keeps the float64
+    solution's structure up to jumps below the float32 floor. Pinned against
+    the float64 structure of the same problem: the float32-rounded point under
+    the float32 rule gives nearly its regions and gradient, under the C rule
+    not."""
+    from fastprox_amd.synth import rectangles
+    x = rectangles(1, (64, 72), n=10, sigma=0.1, seed=4, dtype=np.float64)[0]
+    r = oracle.tv_iso(x, 0.1, tol=1e-10, max_iter=10 ** 6)
+    g = np.random.default_rng(1).normal(size=x.shape)
+    ref = oracle.iso_deriv(x, r.z, r.dual, r.gap, 0, 0.1, g=g, deriv_tol=1e-12, deriv_max_iter=100000)
+    x32, z32, u32 = (v.astype(np.float32).astype(np.float64) for v in (x, r.z, r.dual))
+    c_rule = oracle.iso_deriv(x32, z32, u32, r.gap, 0, 0.1, g=g, deriv_tol=1e-12, deriv_max_iter=100000)
+    f32_rule = oracle.iso_deriv(x32, z32, u32, r.gap, 0, 0.1, g=g, deriv_tol=1e-12, deriv_max_iter=100000,
+                                dual_dtype=np.float32)
+
+    def rel(d):
+        return np.abs(d.out - ref.out).max() / np.abs(ref.out).max()
+    assert abs(f32_rule.regions - ref.regions) <= 0.05 * ref.regions
+    assert rel(f32_rule) <= 0.05
+    assert c_rule.regions < 0.5 * ref.regions and rel(c_rule) > 0.5
+    # the default is restored: the float64 rule again on the float64 point
+    again = oracle.iso_deriv(x, r.z, r.dual, r.gap, 0, 0.1, g=g, deriv_tol=1e-12, deriv_max_iter=100000)
+    np.testing.assert_array_equal(again.out, ref.out)
