@@ -61,6 +61,24 @@ __device__ __forceinline__ T lane_next(T v) {  // the value of lane + 1 (lane 63
     return __shfl_down(v, 1, 64);
 }
 
+// The square root in a gap term. float64 iterates: the correctly rounded
+// sqrt, as the reference (its stops at the same iterations). float32
+// iterates: the hardware's reciprocal square root (relative error about
+// 5e-8) and one Newton step on the root (about 4e-15 for normal v, far below
+// the float32 iterate's own rounding), without the library sqrt's special
+// cases: v is a finite sum of squares here, and v = 0 gives 0.
+template <class T>
+__device__ __forceinline__ double gap_sqrt(double v) {
+    if constexpr (sizeof(T) == 8) {
+        return sqrt(v);
+    } else {
+        const double y = __builtin_amdgcn_rsq(v);
+        const double s = v * y;
+        const double r = __builtin_fma(-s, s, v);
+        return v > 0.0 ? __builtin_fma(0.5 * y, r, s) : 0.0;
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {  // fixed butterfly: the same tree on every call
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -891,7 +909,7 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
                     const double nn = g0 * g0 + g1 * g1;
                     const double dot = g0 * (double)u0[r] + g1 * (double)u1[r];
                     const double rp = SW ? (double)ra[r] : gamd;
-                    acc += rp * sqrt(nn) - dot;
+                    acc += rp * gap_sqrt<T>(nn) - dot;
                 } else {
                     const double r0 = SW ? (double)ra[r] : gamd, r1 = SW ? (double)rb[r] : gamd;
                     acc += r0 * fabs(g0) - g0 * (double)u0[r];
@@ -1151,7 +1169,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
 
     // The gap term of one pixel (tv_iso.cpp:290-306; _tv_torch.py:179-186).
     auto gap_term = [&](double g0, double g1, double u0, double u1, double r0, double r1) -> double {
-        if constexpr (ISO) return r0 * sqrt(g0 * g0 + g1 * g1) - (g0 * u0 + g1 * u1);
+        if constexpr (ISO) return r0 * gap_sqrt<T>(g0 * g0 + g1 * g1) - (g0 * u0 + g1 * u1);
         else return (r0 * fabs(g0) - g0 * u0) + (r1 * fabs(g1) - g1 * u1);
     };
 
@@ -1450,7 +1468,7 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
                     nn += g[a] * g[a];
                     dot += g[a] * (fresh ? 0.0 : (double)U[a * N + p]);
                 }
-                acc = rad[0] * sqrt(nn) - dot;
+                acc = rad[0] * gap_sqrt<T>(nn) - dot;
             } else {
                 for (int a = 0; a < k; ++a) acc += rad[a] * fabs(g[a]) - g[a] * (fresh ? 0.0 : (double)U[a * N + p]);
             }
@@ -1696,7 +1714,7 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
                             nn += g[a] * g[a];
                             dot += g[a] * (fresh ? 0.0 : (double)pu[a]);
                         }
-                        acc = rad[0] * sqrt(nn) - dot;
+                        acc = rad[0] * gap_sqrt<T>(nn) - dot;
                     } else {
                         for (int a = 0; a < 3; ++a)
                             acc += rad[a] * fabs(g[a]) - g[a] * (fresh ? 0.0 : (double)pu[a]);
