@@ -14,6 +14,8 @@ import pathlib
 
 FP_F32, FP_F64, FP_I8, FP_I32, FP_I64 = 1, 2, 3, 4, 5
 FP_DEVICE_CPU, FP_DEVICE_CUDA, FP_DEVICE_ROCM = 0, 2, 10
+FP_ENTRY_FREE, FP_ENTRY_ZERO, FP_ENTRY_LOWER, FP_ENTRY_UPPER = 0, 1, 2, 3
+FP_STRUCT_NONE, FP_STRUCT_PROJECTOR, FP_STRUCT_WEIGHTED, FP_STRUCT_CURVED = 0, 1, 2, 3
 
 (FP_OK, FP_EINVAL, FP_EDTYPE, FP_ESHAPE, FP_ENOMEM, FP_ENOTCONV, FP_EPARAM, FP_EINTERNAL,
  FP_EDEVICE) = range(9)
@@ -92,6 +94,12 @@ def declare(lib):
         "fp_jvp": (I, [P, P, A, D, A, A]),
         "fp_vjp": (I, [P, P, A, A, C.POINTER(D), A]),
         "fp_vjp_device": (I, [P, P, A, A, A, A]),
+        "fp_op_struct_kind": (I, [P, C.POINTER(I)]),
+        "fp_workspace_units": (I, [P, C.POINTER(I64), C.POINTER(I64)]),
+        "fp_workspace_states": (I, [P, A, C.POINTER(I64)]),
+        "fp_workspace_regions": (I, [P, A, A, C.POINTER(I64), I64]),
+        "fp_workspace_constraint": (I, [P, A, C.POINTER(C.c_int8), I64]),
+        "fp_workspace_kinks": (I, [P, D, A, C.POINTER(I64)]),
         "fp_device_available": (I, [I]),
         "fp_op_device_check": (I, [P, I, C.POINTER(I)]),
         "fp_workspace_set_stream": (I, [P, P]),

@@ -89,6 +89,12 @@ class Op:
             f = L.fp_op_tv_iso_weighted if iso else L.fp_op_tv_aniso_weighted
             _check(f(self.k, wa.ref if wa else None, sa.ref, C.byref(self.h)), "fp_op")
 
+    def struct_kind(self):
+        """fp_op_struct_kind: A.FP_STRUCT_CURVED (iso) or FP_STRUCT_PROJECTOR (aniso)."""
+        k = C.c_int()
+        _check(lib().fp_op_struct_kind(self.h, C.byref(k)), "fp_op_struct_kind")
+        return k.value
+
     @property
     def nparams(self):
         return int(lib().fp_op_nparams(self.h))
@@ -187,6 +193,50 @@ class Workspace:
                                       as_array(out).ref if out is not None else None, C.byref(inf))
         _check(s, "fp_workspace_refine", ok=(A.FP_OK, A.FP_ENOTCONV))
         return s, inf.gap, inf.iters
+
+    # ---- structure of the derivative and kinks (fp_workspace_units ...) ----
+
+    def units(self):
+        """(count, size): the images of the kept point and their entries."""
+        c, n = C.c_int64(), C.c_int64()
+        _check(lib().fp_workspace_units(self.h, C.byref(c), C.byref(n)), "fp_workspace_units")
+        return c.value, n.value
+
+    def states(self, out=None):
+        """fp_workspace_states into out (int8, x's shape) if given; the number
+        of entries not FREE."""
+        nf = C.c_int64()
+        _check(lib().fp_workspace_states(self.h, as_array(out).ref if out is not None else None, C.byref(nf)),
+               "fp_workspace_states")
+        return nf.value
+
+    def regions(self, labels=None, sign=None):
+        """fp_workspace_regions into labels (int32 or int64) and sign (int8),
+        each x's shape, if given; the regions per image (numpy int64)."""
+        count = self.units()[0]
+        nr = np.zeros(max(count, 1), np.int64)
+        _check(lib().fp_workspace_regions(self.h, as_array(labels).ref if labels is not None else None,
+                                          as_array(sign).ref if sign is not None else None,
+                                          nr.ctypes.data_as(C.POINTER(C.c_int64)), count), "fp_workspace_regions")
+        return nr[:count]
+
+    def constraint(self, coef=None):
+        """fp_workspace_constraint: coef (x's dtype and shape) if given; the
+        per-image active flags (numpy int8)."""
+        count = self.units()[0]
+        act = np.ones(max(count, 1), np.int8)
+        _check(lib().fp_workspace_constraint(self.h, as_array(coef).ref if coef is not None else None,
+                                             act.ctypes.data_as(C.POINTER(C.c_int8)), count),
+               "fp_workspace_constraint")
+        return act[:count]
+
+    def kinks(self, tol, mask=None):
+        """fp_workspace_kinks at tol into mask (int8, x's shape) if given; the
+        entries at a kink."""
+        c = C.c_int64()
+        _check(lib().fp_workspace_kinks(self.h, float(tol), as_array(mask).ref if mask is not None else None,
+                                        C.byref(c)), "fp_workspace_kinks")
+        return c.value
 
     def get_dual(self, out):
         _check(lib().fp_workspace_get_dual(self.h, as_array(out).ref), "fp_workspace_get_dual")

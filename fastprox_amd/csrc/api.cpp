@@ -1303,6 +1303,193 @@ FP_API fp_status fp_workspace_refine(const fp_op *op, fp_workspace *ws, double t
     return guard([&] { return refine_impl(op, ws, tol, max_iter, out, info); });
 }
 
+// ---- structure of the derivative and kinks (include/fastprox.h:440-579) ----
+
+namespace {
+
+// The queries read the point kept by the last prox call (src/api/api.cpp:
+// 1031-1036 in the reference, which has them on the CPU only).
+fp_status struct_ws(const fp_workspace *ws) {
+    if (!ws || !ws->op || !ws->keep || !ws->valid) return FP_EINVAL;
+    return FP_OK;
+}
+
+int64_t struct_units(const fp_workspace &ws) { return ws.numel == 0 ? 0 : ws.B; }
+
+enum class OutKind { Int8, Label, Float };
+
+// An output of the queries: x's shape, of the kind's dtype (Float: x's),
+// writable (src/api/api.cpp:1058-1065).
+fp_status struct_out(const fp_workspace &ws, const fp_array *a, Arr &out, OutKind kind) {
+    if (fp_status s = to_arr(a, out, kind == OutKind::Float)) return s;
+    if (kind == OutKind::Int8 && out.dtype != FP_I8) return FP_EDTYPE;
+    if (kind == OutKind::Label && out.dtype != FP_I32 && out.dtype != FP_I64) return FP_EDTYPE;
+    if (kind == OutKind::Float && out.dtype != ws.dtype) return FP_EDTYPE;
+    if (out.ndim != ws.ndim) return FP_ESHAPE;
+    for (int d = 0; d < out.ndim; ++d)
+        if (out.shape[d] != ws.shape[d]) return FP_ESHAPE;
+    if (out.device == FP_DEVICE_ROCM && !out.contiguous() && out.ndim > fpr::kMaxStrided) return FP_ESHAPE;
+    return check_output(nullptr, out);
+}
+
+// A device buffer for one query, freed after the stream has finished with it.
+struct QueryBuf {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    explicit QueryBuf(hipStream_t st) : s(st) {}
+    ~QueryBuf() {
+        if (p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    }
+    fp_status alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 16)) ? FP_ENOMEM : FP_OK; }
+};
+
+// The contiguous device entries src (C order, out's item size) into out
+// (host or device, any strides).
+fp_status put_struct(const void *src, const Arr &a, hipStream_t s) {
+    const int64_t n = a.numel();
+    if (n == 0) return FP_OK;
+    const size_t bytes = (size_t)n * a.itemsize();
+    if (a.device == FP_DEVICE_CPU) {
+        std::vector<char> host(bytes);
+        if (hipMemcpyAsync(host.data(), src, bytes, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s))
+            return FP_EDEVICE;
+        unpack_host(host.data(), a);
+        return FP_OK;
+    }
+    if (a.contiguous()) return hipMemcpyAsync(a.data, src, bytes, hipMemcpyDeviceToDevice, s) ? FP_EDEVICE : FP_OK;
+    const fpr::StridedShape sh = strided_of(a);
+    switch (a.itemsize()) {
+        case 1: fpr::launch_scatter<int8_t>((const int8_t *)src, (int8_t *)a.data, n, a.ndim, sh, s); break;
+        case 4: fpr::launch_scatter<int32_t>((const int32_t *)src, (int32_t *)a.data, n, a.ndim, sh, s); break;
+        default: fpr::launch_scatter<int64_t>((const int64_t *)src, (int64_t *)a.data, n, a.ndim, sh, s); break;
+    }
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+// Zeros into out (a state array of FREE entries, or a coefficient array).
+fp_status put_zeros(const Arr &a, hipStream_t s) {
+    QueryBuf z(s);
+    const size_t bytes = (size_t)a.numel() * a.itemsize();
+    if (fp_status st = z.alloc(bytes)) return st;
+    if (hipMemsetAsync(z.p, 0, bytes, s)) return FP_EDEVICE;
+    return put_struct(z.p, a, s);
+}
+
+fp_status states_impl(const fp_workspace *ws, fp_array *sa, int64_t *nfixed) {
+    if (fp_status s = struct_ws(ws)) return s;
+    Arr st;
+    if (sa)
+        if (fp_status s = struct_out(*ws, sa, st, OutKind::Int8)) return s;
+    if (sa && ws->numel) {
+        DevGuard guard(ws->device);
+        if (fp_status s = put_zeros(st, (hipStream_t)ws->stream)) return s;
+    }
+    if (nfixed) *nfixed = 0;  // every entry of an image operator is FREE
+    return FP_OK;
+}
+
+fp_status regions_impl(const fp_op *op, fp_workspace *ws, fp_array *la, fp_array *ga, int64_t *nregions,
+                       int64_t count) {
+    if (fp_status s = struct_ws(ws)) return s;
+    if (nregions && count != struct_units(*ws)) return FP_ESHAPE;
+    Arr lab, sg;
+    if (la)
+        if (fp_status s = struct_out(*ws, la, lab, OutKind::Label)) return s;
+    if (ga)
+        if (fp_status s = struct_out(*ws, ga, sg, OutKind::Int8)) return s;
+    if (ws->numel == 0) return FP_OK;
+    DevGuard guard(ws->device);
+    hipStream_t s = (hipStream_t)ws->stream;
+    if (fp_status st = ensure_deriv(op, ws, s)) return st;
+    QueryBuf lb(s), sb(s);
+    const size_t n = (size_t)ws->numel;
+    if (la)
+        if (fp_status st = lb.alloc(n * lab.itemsize())) return st;
+    if (ga)
+        if (fp_status st = sb.alloc(n)) return st;
+    if (int r = fpr::deriv_regions(ws->deriv.get(), s, lb.p, la && lab.dtype == FP_I64, (int8_t *)sb.p, nregions))
+        return (fp_status)r;
+    if (la)
+        if (fp_status st = put_struct(lb.p, lab, s)) return st;
+    if (ga)
+        if (fp_status st = put_struct(sb.p, sg, s)) return st;
+    return FP_OK;
+}
+
+fp_status constraint_impl(const fp_workspace *ws, fp_array *ca, int8_t *active, int64_t count) {
+    if (fp_status s = struct_ws(ws)) return s;
+    if (active && count != struct_units(*ws)) return FP_ESHAPE;
+    Arr c;
+    if (ca)
+        if (fp_status s = struct_out(*ws, ca, c, OutKind::Float)) return s;
+    if (ca && ws->numel) {
+        DevGuard guard(ws->device);
+        if (fp_status s = put_zeros(c, (hipStream_t)ws->stream)) return s;
+    }
+    if (active) std::fill(active, active + count, int8_t{0});  // TV has no linear constraint
+    return FP_OK;
+}
+
+fp_status kinks_impl(const fp_op *op, fp_workspace *ws, double tol, fp_array *ma, int64_t *count) {
+    if (!(tol >= 0.0) || !std::isfinite(tol)) return FP_EPARAM;
+    if (fp_status s = struct_ws(ws)) return s;
+    Arr m;
+    if (ma)
+        if (fp_status s = struct_out(*ws, ma, m, OutKind::Int8)) return s;
+    int64_t c = 0;
+    if (ws->numel) {
+        DevGuard guard(ws->device);
+        hipStream_t s = (hipStream_t)ws->stream;
+        if (fp_status st = ensure_deriv(op, ws, s)) return st;
+        QueryBuf mb(s);
+        if (fp_status st = mb.alloc((size_t)ws->numel)) return st;
+        if (int r = fpr::deriv_kinks(ws->deriv.get(), s, ws->q, ws->N, ws->B * ws->N, tol, (int8_t *)mb.p, &c))
+            return (fp_status)r;
+        if (ma)
+            if (fp_status st = put_struct(mb.p, m, s)) return st;
+    }
+    if (count) *count = c;
+    return FP_OK;
+}
+
+}  // namespace
+
+FP_API fp_status fp_op_struct_kind(const fp_op *op, int *kind) {
+    if (!op || !kind) return FP_EINVAL;
+    *kind = op->iso ? FP_STRUCT_CURVED : FP_STRUCT_PROJECTOR;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_units(const fp_workspace *ws, int64_t *count, int64_t *size) {
+    if (fp_status s = struct_ws(ws)) return s;
+    if (count) *count = struct_units(*ws);
+    if (size) *size = ws->N;
+    return FP_OK;
+}
+
+FP_API fp_status fp_workspace_states(const fp_workspace *ws, fp_array *state, int64_t *nfixed) {
+    return guard([&] { return states_impl(ws, state, nfixed); });
+}
+
+FP_API fp_status fp_workspace_regions(const fp_workspace *ws, fp_array *labels, fp_array *sign, int64_t *nregions,
+                                      int64_t count) {
+    // The structure is built on first use: the workspace is logically const.
+    return guard([&] {
+        return regions_impl(ws ? ws->op : nullptr, const_cast<fp_workspace *>(ws), labels, sign, nregions, count);
+    });
+}
+
+FP_API fp_status fp_workspace_constraint(const fp_workspace *ws, fp_array *coef, int8_t *active, int64_t count) {
+    return guard([&] { return constraint_impl(ws, coef, active, count); });
+}
+
+FP_API fp_status fp_workspace_kinks(const fp_workspace *ws, double tol, fp_array *mask, int64_t *count) {
+    return guard([&] { return kinks_impl(ws ? ws->op : nullptr, const_cast<fp_workspace *>(ws), tol, mask, count); });
+}
+
 FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, double *smoothing) {
     if (!ws || !ws->valid) return FP_EINVAL;
     int all = 1;

@@ -1423,4 +1423,162 @@ void deriv_result(const Deriv *d, double *residual, int *iters, int64_t *regions
     if (regions) *regions = d->nc;
 }
 
+// ---------------------------------------------------------------------------
+// Structure queries (fp_workspace_regions, fp_workspace_kinks)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Labels within each image: the union-find roots are the smallest index of
+// their region and are ranked in index order, so label - rbase[b] numbers an
+// image's regions in the order of their first entries
+// (driver.hpp:1133-1148 renumbers the saved labels the same way).
+__global__ __launch_bounds__(kThr) void region_labels_kernel(Geo G, const DImg *di, const int *label, const int *rbase,
+                                                             void *lab, int lab64, int8_t *sign) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    const bool nan = di[b].nan != 0;
+    const int64_t v = nan ? -1 : (int64_t)(label[gp] - rbase[b]);
+    if (lab) {
+        if (lab64) ((int64_t *)lab)[gp] = v;
+        else ((int *)lab)[gp] = (int)v;
+    }
+    if (sign) sign[gp] = nan ? 0 : 1;
+}
+
+__global__ void region_counts_kernel(int B, const DImg *di, const int *rbase, int64_t *out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) out[b] = di[b].nan ? -1 : (int64_t)(rbase[b + 1] - rbase[b]);
+}
+
+// Iso kinks (tv_iso.cpp:633-664) at the input's scale: the step as given is
+// gamma_s 2^e; q is the dual as returned (|q_p| <= gamma alpha_p).
+template <class T>
+__global__ __launch_bounds__(kThr) void kinks_iso_kernel(Geo G, const DImg *di, const T *z, const T *q, int64_t q_img,
+                                                         int64_t q_plane, const T *sw, int64_t sw_img, double tol,
+                                                         int8_t *mask) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = (int)(gp / G.N);
+    const int64_t p = gp - (int64_t)b * G.N;
+    const DImg &D = di[b];
+    const double gamma = ldexp(D.gamma_s, D.e);
+    double r = gamma;
+    if (sw) {
+        const double a = ld(sw, b * sw_img + p);
+        r = a == 0.0 ? 0.0 : gamma * a;
+    }
+    int64_t c[3];
+    coords(G, p, c);
+    double gg = 0.0, uu = 0.0;
+    const double z0 = ld(z, gp);
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        if (D.wa[a] > 0.0 && c[ax] + 1 < G.L[ax]) {
+            const double t = D.wa[a] * (ld(z, gp + G.S[ax]) - z0);
+            gg += t * t;
+        }
+        const double ua = ld(q, b * q_img + a * q_plane + p);
+        uu += ua * ua;
+    }
+    mask[gp] = (int8_t)(r > 0.0 && sqrt(gg) <= tol && sqrt(uu) >= r - tol ? 1 : 0);
+}
+
+// Aniso kinks of axis a (tv_aniso.cpp:1303-1335): one thread per fiber of
+// the axis walks it, summing q_a into the edge's 1D dual. One launch per
+// axis, so each entry's bit is set by one thread.
+template <class T>
+__global__ __launch_bounds__(kThr) void kinks_aniso_kernel(Geo G, int a, const DImg *di, const T *z, const T *q,
+                                                           int64_t q_img, int64_t q_plane, const T *sw, int64_t sw_img,
+                                                           int64_t sw_plane, double tol, int8_t *mask) {
+    const int ax = 3 - G.k + a;
+    const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;  // the other two axes, in order
+    const int64_t La = G.L[ax], nf = G.N / La, B = G.BN / G.N;
+    const int64_t t = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (t >= nf * B) return;
+    const int b = (int)(t / nf);
+    const int64_t f = t - (int64_t)b * nf;
+    const int64_t st = (f / G.L[o2]) * G.S[o1] + (f % G.L[o2]) * G.S[o2];
+    const DImg &D = di[b];
+    const double lam = ldexp(D.gamma_s, D.e) * D.wa[a];
+    if (!(lam > 0.0) || La == 1) return;
+    const int64_t s = G.S[ax], zb = (int64_t)b * G.N;
+    double acc = 0.0;
+    for (int64_t l = 0; l + 1 < La; ++l) {
+        const int64_t i = st + l * s;
+        acc += ld(q, b * q_img + a * q_plane + i);
+        const double r = sw ? lam * ld(sw, b * sw_img + a * sw_plane + i) : lam;
+        if (!(r > 0.0) || !(fabs(ld(z, zb + i + s) - ld(z, zb + i)) <= tol) || !(fabs(acc) >= r - tol)) continue;
+        mask[zb + i] = (int8_t)(mask[zb + i] | (1 << a));
+    }
+}
+
+__global__ __launch_bounds__(kThr) void zero_bytes_kernel(int64_t n, int8_t *m) {
+    const int64_t i = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (i < n) m[i] = 0;
+}
+
+__global__ __launch_bounds__(kThr) void count_nonzero_kernel(int64_t n, const int8_t *m, unsigned long long *count) {
+    const int64_t i = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    const bool nz = i < n && m[i] != 0;
+    const unsigned long long bal = __ballot(nz);
+    if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (unsigned long long)__popcll(bal));
+}
+
+template <class T>
+int kinks_t(Deriv &d, hipStream_t s, const void *qv, int64_t q_img, int64_t q_plane, double tol, int8_t *mask,
+            int64_t *count) {
+    const Geo G = d.G;
+    const T *z = (const T *)d.sp.z, *q = (const T *)qv, *sw = (const T *)d.sp.sw;
+    const int gpx = grid_of(G.BN);
+    if (d.sp.iso) {
+        hipLaunchKernelGGL(kinks_iso_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, d.di, z, q, q_img, q_plane, sw,
+                           d.sp.sw_img, tol, mask);
+    } else {
+        hipLaunchKernelGGL(zero_bytes_kernel, dim3(gpx), dim3(kThr), 0, s, G.BN, mask);
+        for (int a = 0; a < G.k; ++a) {
+            const int64_t nf = G.BN / G.L[3 - G.k + a];
+            hipLaunchKernelGGL(kinks_aniso_kernel<T>, dim3(grid_of(nf)), dim3(kThr), 0, s, G, a, d.di, z, q, q_img,
+                               q_plane, sw, d.sp.sw_img, d.sp.sw_plane, tol, mask);
+        }
+    }
+    unsigned long long *cnt = (unsigned long long *)d.cnt;
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3(1), dim3(kThr), 0, s, (int64_t)sizeof(unsigned long long),
+                       (int8_t *)cnt);
+    hipLaunchKernelGGL(count_nonzero_kernel, dim3(gpx), dim3(kThr), 0, s, G.BN, mask, cnt);
+    unsigned long long hc = 0;
+    if (hipMemcpyAsync(&hc, cnt, sizeof hc, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s)) return FP_EDEVICE;
+    if (count) *count = (int64_t)hc;
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+}  // namespace
+
+int deriv_regions(Deriv *d, hipStream_t s, void *labels, bool lab64, int8_t *sign, int64_t *nregions) {
+    if (!d) return FP_EINVAL;
+    const Geo G = d->G;
+    if (labels || sign)
+        hipLaunchKernelGGL(region_labels_kernel, dim3(grid_of(G.BN)), dim3(kThr), 0, s, G, d->di, d->label, d->rbase,
+                           labels, lab64 ? 1 : 0, sign);
+    if (nregions) {
+        const int B = (int)d->sp.B;
+        int64_t *dev = nullptr;
+        if (hipMalloc((void **)&dev, sizeof(int64_t) * (size_t)B)) return FP_ENOMEM;
+        hipLaunchKernelGGL(region_counts_kernel, dim3((B + 255) / 256), dim3(256), 0, s, B, d->di, d->rbase, dev);
+        const bool bad = hipMemcpyAsync(nregions, dev, sizeof(int64_t) * (size_t)B, hipMemcpyDeviceToHost, s) ||
+                         hipStreamSynchronize(s);
+        (void)hipFree(dev);
+        if (bad) return FP_EDEVICE;
+    }
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+int deriv_kinks(Deriv *d, hipStream_t s, const void *q, int64_t q_img, int64_t q_plane, double tol, int8_t *mask,
+                int64_t *count) {
+    if (!d || !q || !mask) return FP_EINVAL;
+    return d->sp.tsize == 4 ? kinks_t<float>(*d, s, q, q_img, q_plane, tol, mask, count)
+                            : kinks_t<double>(*d, s, q, q_img, q_plane, tol, mask, count);
+}
+
 }  // namespace fpr

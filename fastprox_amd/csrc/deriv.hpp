@@ -63,4 +63,23 @@ void deriv_result(const Deriv *d, double *residual, int *iters, int64_t *regions
 // Per-image results of the last derivative (host arrays of B).
 void deriv_image_result(const Deriv *d, double *residual, int *iters);
 
+// The structure's regions (fp_workspace_regions): per entry of the B images
+// (device, B N, C order), its region's number within its image in the order
+// of the regions' first entries (int64 if lab64, else int32), or -1 in an
+// image whose input holds a NaN; sign +1 (0 there). Either may be null.
+// nregions (host, B): the images' region counts, -1 for a NaN image, or
+// null. Waits for the stream when nregions is given. An fp_status.
+int deriv_regions(Deriv *d, hipStream_t s, void *labels, bool lab64, int8_t *sign, int64_t *nregions);
+
+// Kinks (fp_workspace_kinks) at the solution the structure was built from,
+// with q its C-form dual (device, image b's plane a at q + b q_img + a
+// q_plane): per entry the code into mask (device, B N int8; iso: 1 where
+// |(W D z)_p| <= tol and |q_p| >= r_p - tol with r_p > 0; aniso: bit a where
+// the edge (p, p + e_a) has |z_{p+e_a} - z_p| <= tol and its 1D dual, the
+// running sum of q_a along the fiber, is within tol of its bound gamma w_a
+// beta_e > 0), and the entries with a nonzero code into *count. Waits for the
+// stream. An fp_status.
+int deriv_kinks(Deriv *d, hipStream_t s, const void *q, int64_t q_img, int64_t q_plane, double tol, int8_t *mask,
+                int64_t *count);
+
 }  // namespace fpr

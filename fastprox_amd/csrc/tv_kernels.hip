@@ -2017,6 +2017,10 @@ void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
 FPR_INST(float)
 FPR_INST(double)
 #undef FPR_INST
+// Integer outputs of the structure queries (labels, signs, states, masks).
+template void launch_scatter<int8_t>(const int8_t *, int8_t *, int64_t, int, const StridedShape &, hipStream_t);
+template void launch_scatter<int32_t>(const int32_t *, int32_t *, int64_t, int, const StridedShape &, hipStream_t);
+template void launch_scatter<int64_t>(const int64_t *, int64_t *, int64_t, int, const StridedShape &, hipStream_t);
 template void launch_convert<float, double>(const float *, double *, int64_t, hipStream_t);
 template void launch_convert<double, float>(const double *, float *, int64_t, hipStream_t);
 template void launch_convert<float, float>(const float *, float *, int64_t, hipStream_t);

@@ -155,6 +155,35 @@ FP_API fp_status fp_workspace_image_info(const fp_workspace *ws, double *gaps, i
 FP_API fp_status fp_vjp_device(const fp_op *op, const fp_workspace *ws, const fp_array *g, fp_array *gx,
                                fp_array *ggamma, fp_array *gparams);
 
+/* Structure of the derivative (include/fastprox.h:514-579) and kinks
+ * (:440-511) at the point kept by the last prox call on a keep_structure
+ * workspace (FP_EINVAL without one), computed on the device. The units are
+ * the images (count: the batch's images, size: their entries); every entry is
+ * FP_ENTRY_FREE (sign +1) with the label of its region within its image, in
+ * the order of the regions' first entries; an image whose input holds a NaN
+ * has -1 regions, labels -1 and sign 0; no unit has a constraint. Output
+ * arrays have x's shape (labels FP_I32 or FP_I64, state, sign and mask FP_I8,
+ * coef x's dtype), host or device, any strides; each may be NULL. Kinks: iso,
+ * 1 at a pixel with |(W D z)_p| <= tol and |u_p| >= gamma alpha_p - tol;
+ * aniso, bit a at p where the edge (p, p + e_a) has |jump| <= tol and its 1D
+ * dual within tol of gamma w_a beta (edges and pixels of zero bound are never
+ * kinks). fp_workspace_set_branch is not provided. */
+typedef enum { FP_ENTRY_FREE = 0, FP_ENTRY_ZERO = 1, FP_ENTRY_LOWER = 2, FP_ENTRY_UPPER = 3 } fp_entry_state;
+typedef enum { FP_STRUCT_NONE = 0, FP_STRUCT_PROJECTOR = 1, FP_STRUCT_WEIGHTED = 2, FP_STRUCT_CURVED = 3 } fp_struct_kind;
+/* include/fastprox.h:534-550: iso TV is FP_STRUCT_CURVED, aniso FP_STRUCT_PROJECTOR. */
+FP_API fp_status fp_op_struct_kind(const fp_op *op, int *kind);
+/* include/fastprox.h:552-554 */
+FP_API fp_status fp_workspace_units(const fp_workspace *ws, int64_t *count, int64_t *size);
+/* include/fastprox.h:558 */
+FP_API fp_status fp_workspace_states(const fp_workspace *ws, fp_array *state, int64_t *nfixed);
+/* include/fastprox.h:569-570 */
+FP_API fp_status fp_workspace_regions(const fp_workspace *ws, fp_array *labels, fp_array *sign,
+                                      int64_t *nregions, int64_t count);
+/* include/fastprox.h:578-579 */
+FP_API fp_status fp_workspace_constraint(const fp_workspace *ws, fp_array *coef, int8_t *active, int64_t count);
+/* include/fastprox.h:510-511 */
+FP_API fp_status fp_workspace_kinks(const fp_workspace *ws, double tol, fp_array *mask, int64_t *count);
+
 /* include/fastprox.h:782-789 */
 FP_API const char *fp_strerror(fp_status s);
 FP_API int         fp_abi_version(void);
