@@ -84,6 +84,21 @@ __device__ __forceinline__ double wave_sum(double v) {  // fixed butterfly: the 
     return v;
 }
 
+// The images' buffers (input, output, dual, weights) are device memory: access
+// through global-address-space pointers compiles to global rather than flat
+// instructions (a flat access also counts against the LDS counter, so every
+// LDS wait would wait for the row loads in flight as well).
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<const T> gin(const void *p) {
+    return (gptr<const T>)p;
+}
+template <class T>
+__device__ __forceinline__ gptr<T> gout(void *p) {
+    return (gptr<T>)p;
+}
+
 template <class T>
 __device__ __forceinline__ T scale_in(T v, int e) {
     return e ? (T)ldexp((double)v, -e) : v;
@@ -347,7 +362,7 @@ __global__ __launch_bounds__(kProBlock) void prologue_pass(const ImgDesc *desc, 
     const int64_t lo = (int64_t)blk * kProChunk;
     if (lo >= N) return;
     const int64_t hi = lo + kProChunk < N ? lo + kProChunk : N;
-    const T *x = (const T *)D.d;
+    const auto x = gin<T>(D.d);
     __shared__ double sh[4];
     double *out = part + ((int64_t)img * nblk + blk) * 4;
     if (pass == 0) {
@@ -454,7 +469,7 @@ __global__ __launch_bounds__(256) void weight_max_kernel(Problem P, const ImgDes
     double am = 0.0;
     if (P.sw && P.sw_per_image && P.iso) {
         const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
-        const T *al = (const T *)D.sw;
+        const auto al = gin<T>(D.sw);
         for (int64_t i = threadIdx.x; i < N; i += 256) am = fmax(am, (double)al[i]);
         am = block_max(am, sh);
     } else {
@@ -634,7 +649,7 @@ __global__ void warm_iso_kernel(Problem P, T *arena, const ImgDesc *desc,
             nn += v * v;
         }
         double rp = S.gamma;
-        if (P.sw) rp = iso_radius(S.gamma, (double)((const T *)D.sw)[p], S.bound);
+        if (P.sw) rp = iso_radius(S.gamma, (double)gin<T>(D.sw)[p], S.bound);
         if (sqrt(nn) > rp) {
             const double sc = rp / sqrt(nn);
             for (int a = 0; a < k; ++a) u[a] *= sc;
@@ -675,7 +690,7 @@ __global__ void warm_aniso_kernel(Problem P, T *arena, const ImgDesc *desc,
                 cs += (double)((const T *)D.q)[a * D.q_cstride + o];
                 double v = P.w[a] > 0.0 ? -cs / P.w[a] : 0.0;
                 double r = S.gamma;
-                if (P.sw) r = S.gamma * (double)((const T *)D.sw)[a * D.sw_cstride + o];
+                if (P.sw) r = S.gamma * (double)gin<T>(D.sw)[a * D.sw_cstride + o];
                 v = r > 0.0 ? fmin(fmax(v, -r), r) : 0.0;
                 if (i + 1 == L || !isfinite(v)) v = 0.0;
                 U[a * N + o] = (T)v;
@@ -845,9 +860,9 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
         for (int r = 0; r < RY; ++r)
             if ((intr >> r) & 1u) {
                 const int64_t o = (int64_t)(y0 + r) * W + x;
-                ((T *)D.z)[o] = nanv;
-                ((T *)D.q)[o] = nanv;
-                ((T *)D.q)[D.q_cstride + o] = nanv;
+                gout<T>(D.z)[o] = nanv;
+                gout<T>(D.q)[o] = nanv;
+                gout<T>(D.q)[D.q_cstride + o] = nanv;
             }
         return;
     }
@@ -859,7 +874,7 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
     const T tau = (T)S.tau;
     const double gamd = S.gamma, boundd = S.bound;
     const T gam = (T)gamd;
-    const T *dp = (const T *)D.d;
+    const auto dp = gin<T>(D.d);
     const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
     const T *V = S.vsrc_u ? U : U + 2 * N;
 
@@ -879,7 +894,7 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
             v1[r] = in ? V[N + o] : (T)0;
         }
         if constexpr (SW) {
-            const T *sw = (const T *)D.sw;
+            const auto sw = gin<T>(D.sw);
             if constexpr (ISO) {
                 ra[r] = in ? (T)iso_radius(gamd, (double)sw[o], boundd) : (T)0;
             } else {
@@ -985,13 +1000,13 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
             const double zz = ((double)d[r] + w0d * ((double)u0[r] - up)) + w1d * ((double)u1[r] - ul);
             if ((intr >> r) & 1u) {
                 const int64_t o = (int64_t)(y0 + r) * W + x;
-                ((T *)D.z)[o] = (T)ldexp(zz, e);
+                gout<T>(D.z)[o] = (T)ldexp(zz, e);
                 if constexpr (ISO) {
-                    ((T *)D.q)[o] = (T)ldexp((double)u0[r] / wmax, e);
-                    ((T *)D.q)[D.q_cstride + o] = (T)ldexp((double)u1[r] / wmax, e);
+                    gout<T>(D.q)[o] = (T)ldexp((double)u0[r] / wmax, e);
+                    gout<T>(D.q)[D.q_cstride + o] = (T)ldexp((double)u1[r] / wmax, e);
                 } else {
-                    ((T *)D.q)[o] = (T)(w0d * (up - (double)u0[r]));
-                    ((T *)D.q)[D.q_cstride + o] = (T)(w1d * (ul - (double)u1[r]));
+                    gout<T>(D.q)[o] = (T)(w0d * (up - (double)u0[r]));
+                    gout<T>(D.q)[D.q_cstride + o] = (T)(w1d * (ul - (double)u1[r]));
                 }
             }
         }
@@ -1117,9 +1132,9 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
         if (lane_in)
             for (int r = y0; r < yend; ++r) {
                 const int64_t o = (int64_t)r * W + x;
-                ((T *)D.z)[o] = (T)NAN;
-                ((T *)D.q)[o] = (T)NAN;
-                ((T *)D.q)[D.q_cstride + o] = (T)NAN;
+                gout<T>(D.z)[o] = (T)NAN;
+                gout<T>(D.q)[o] = (T)NAN;
+                gout<T>(D.q)[D.q_cstride + o] = (T)NAN;
             }
         return;
     }
@@ -1135,11 +1150,11 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     const T w0 = (T)w0d, w1 = (T)w1d;
     const bool act0 = w0d != 0.0, act1 = w1d != 0.0;
     const bool out_mode = mode == kOutput;
-    const T *dp = (const T *)D.d;
+    const auto dp = gin<T>(D.d);
     const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
     const T *V = S.vsrc_u ? U : U + 2 * N;
     T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N;
-    const T *sw = SW ? (const T *)D.sw : nullptr;
+    const auto sw = SW ? gin<T>(D.sw) : gptr<const T>(nullptr);
     T beta[TT + 1];
 #pragma unroll
     for (int m = 1; m <= TT; ++m) beta[m] = (T)S.beta[m];
@@ -1301,13 +1316,13 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                     const double zz =
                         ((double)dw[TT] + w0d * ((double)u0n - (double)uf0)) + w1d * ((double)u1n - ul);
                     if (st) {
-                        ((T *)D.z)[so] = (T)ldexp(zz, e);
+                        gout<T>(D.z)[so] = (T)ldexp(zz, e);
                         if constexpr (ISO) {
-                            ((T *)D.q)[so] = (T)ldexp((double)u0n / S.wmax, e);
-                            ((T *)D.q)[D.q_cstride + so] = (T)ldexp((double)u1n / S.wmax, e);
+                            gout<T>(D.q)[so] = (T)ldexp((double)u0n / S.wmax, e);
+                            gout<T>(D.q)[D.q_cstride + so] = (T)ldexp((double)u1n / S.wmax, e);
                         } else {
-                            ((T *)D.q)[so] = (T)(w0d * ((double)uf0 - (double)u0n));
-                            ((T *)D.q)[D.q_cstride + so] = (T)(w1d * (ul - (double)u1n));
+                            gout<T>(D.q)[so] = (T)(w0d * ((double)uf0 - (double)u0n));
+                            gout<T>(D.q)[D.q_cstride + so] = (T)(w1d * (ul - (double)u1n));
                         }
                     }
                 }
@@ -1410,8 +1425,8 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
         double *part = A.part + (int64_t)(D.tile_base + tl.pad_) * kSlots;
         if (mode == kOutputNaN) {
             if (in) {
-                ((T *)D.z)[p] = (T)NAN;
-                for (int a = 0; a < k; ++a) ((T *)D.q)[a * D.q_cstride + p] = (T)NAN;
+                gout<T>(D.z)[p] = (T)NAN;
+                for (int a = 0; a < k; ++a) gout<T>(D.q)[a * D.q_cstride + p] = (T)NAN;
             }
             continue;
         }
@@ -1425,11 +1440,11 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
             w[a] = (T)wd[a];
         }
         for (int a = 0; a < k; ++a) has[a] = in && c[3 - k + a] + 1 < D.L[3 - k + a] && wd[a] != 0.0;
-        const T *dp = (const T *)D.d;
+        const T *dp = (const T *)D.d;  // G.zval reads through a generic pointer
         const bool fresh = S.fresh;
         const T *U = A.arena + D.st_off + (int64_t)S.cur * 2 * k * N;
         const T *V = S.vsrc_u ? U : U + k * N;
-        const T *sw = SW ? (const T *)D.sw : nullptr;
+        const auto sw = SW ? gin<T>(D.sw) : gptr<const T>(nullptr);
         const double gamd = S.gamma;
         double rad[3] = {gamd, gamd, gamd};
         if (SW && in) {
@@ -1547,9 +1562,9 @@ __global__ __launch_bounds__(256) void step1_kernel(StepArgs<T> A) {
                 double qv;
                 if (ISO) qv = ldexp(fc / S.wmax, e);
                 else qv = wd[a] * (fp - fc);
-                ((T *)D.q)[a * D.q_cstride + p] = (T)qv;
+                gout<T>(D.q)[a * D.q_cstride + p] = (T)qv;
             }
-            ((T *)D.z)[p] = (T)ldexp(zz, e);
+            gout<T>(D.z)[p] = (T)ldexp(zz, e);
             continue;
         }
         if (!step) continue;
@@ -1605,8 +1620,8 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
             if (upd)
                 for (int t = z0; t < z1; ++t) {
                     const int64_t o = t * SZ + (int64_t)y * SY + x;
-                    ((T *)D.z)[o] = (T)NAN;
-                    for (int a = 0; a < 3; ++a) ((T *)D.q)[a * D.q_cstride + o] = (T)NAN;
+                    gout<T>(D.z)[o] = (T)NAN;
+                    for (int a = 0; a < 3; ++a) gout<T>(D.q)[a * D.q_cstride + o] = (T)NAN;
                 }
             continue;
         }
@@ -1625,11 +1640,11 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
         }
         const T tau = (T)S.tau, beta = (T)S.beta[1];
         const double gamd = S.gamma;
-        const T *dp = (const T *)D.d;
+        const auto dp = gin<T>(D.d);
         const T *U = A.arena + D.st_off + (int64_t)S.cur * 6 * N;
         const T *V = S.vsrc_u ? U : U + 3 * N;
         T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 6 * N;
-        const T *sw = SW ? (const T *)D.sw : nullptr;
+        const auto sw = SW ? gin<T>(D.sw) : gptr<const T>(nullptr);
         const int64_t oxy = in ? (int64_t)y * SY + x : 0;
         const bool yup = in && y > 0;
         // The previous slice's state at this voxel, and z at it.
@@ -1732,9 +1747,9 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
                         double qv;
                         if (ISO) qv = ldexp(fc / S.wmax, e);
                         else qv = wd[a] * (fpv[a] - fc);
-                        ((T *)D.q)[a * D.q_cstride + o] = (T)qv;
+                        gout<T>(D.q)[a * D.q_cstride + o] = (T)qv;
                     }
-                    ((T *)D.z)[o] = (T)ldexp(zup, e);
+                    gout<T>(D.z)[o] = (T)ldexp(zup, e);
                 } else if (step) {
                     T g[3] = {0, 0, 0};
                     if (has0) g[0] = w[0] * (zv - zvp);
