@@ -383,3 +383,76 @@ def test_torch_backward_float32_near_float64(cuda):
     gx64, gg64 = out[torch.float64]
     assert np.abs(gx32 - gx64).max() <= 0.05 * np.abs(gx64).max()
     assert abs(gg32 - gg64) <= 0.05 * abs(gg64)
+
+
+def test_c3_vjp_against_central_differences_and_the_reference(cuda):
+    """The differentiable configuration at its size: x (16, 3, 256, 256) from
+    the rectangle generator with noise 0.1, gamma 0.1, loss mean((z -
+    clean)^2). The VJP in x and gamma (float64, tight tolerances) against
+    float64 central differences (h = 1e-6, 32 random directions) and against
+    the reference library's fp_vjp at the same solution."""
+    from fastprox_amd.synth import rectangles
+    rng = np.random.default_rng(31)
+    shape = (16, 3, 256, 256)
+    x0 = rectangles(48, shape[2:], n=12, sigma=0.1, seed=0, dtype=np.float64).reshape(shape)
+    clean = torch.tensor(rectangles(48, shape[2:], n=12, sigma=0.0, seed=0, dtype=np.float64).reshape(shape),
+                         device="cuda")
+    tol = 1e-12
+
+    def loss(x, g, warm=None):
+        z, info = tv_iso(x, g, tol=tol, max_iter=10 ** 6, deriv_tol=1e-12, deriv_max_iter=100000, warm_start=warm,
+                         return_info=True)
+        return ((z - clean) ** 2).mean(), z, info
+
+    x = torch.tensor(x0, device="cuda", requires_grad=True)
+    gam = torch.tensor(0.1, dtype=torch.float64, device="cuda", requires_grad=True)
+    L0, z, info = loss(x, gam)
+    L0.backward()
+    gx, gg = x.grad.detach(), float(gam.grad)
+    dual = info.dual.detach()
+    # The perturbed solves start from the solution's dual (a warm start), so
+    # both sides of a difference carry nearly the same solve error.
+    h = 1e-6
+    errs, ans = [], []
+    with torch.no_grad():
+        for i in range(32):
+            v = torch.tensor(rng.normal(size=shape), device="cuda")
+            fd = (float(loss(x + h * v, gam, dual)[0]) - float(loss(x - h * v, gam, dual)[0])) / (2 * h)
+            an = float((gx * v).sum())
+            errs.append(abs(fd - an))
+            ans.append(abs(an))
+        fdg = (float(loss(x, gam + h, dual)[0]) - float(loss(x, gam - h, dual)[0])) / (2 * h)
+    # Errors on the scale of the directional derivatives: random directions
+    # give |<gx, v>| over a range of 40x here, and the prox is only piecewise
+    # smooth (a perturbation of 3 M pixels can cross a kink somewhere, which
+    # central differences see and the solution's Jacobian does not); the
+    # reference's own bar on small images is 3e-3 (test_tv_iso.py:126-130).
+    worst = max(errs) / max(ans)
+    print(f"worst FD gap over 32 directions {worst:.2e} of max |<gx, v>| (|<gx, v>| {min(ans):.3e}.."
+          f"{max(ans):.3e}); gamma {gg:.6e} vs FD {fdg:.6e}", flush=True)
+    assert worst <= 3e-3
+    assert abs(gg - fdg) <= 1e-3 * abs(fdg)
+    # The reference's VJP at the same solution (the same bits of z), on four
+    # of the images (each image's derivative is its own; the reference's CPU
+    # solve of all 48 to this tolerance takes minutes): the cotangent of the
+    # loss, x's gradient and gamma's for those images.
+    refbind = pytest.importorskip("refbind")
+    if not refbind.available():
+        pytest.skip("the reference library is not built")
+    xs = x0.reshape((48,) + shape[2:])[:4].copy()
+    gzs = ((2.0 / z.numel()) * (z.detach() - clean)).reshape((48,) + shape[2:])[:4].cpu().numpy()
+    print("reference solve of 4 images", flush=True)
+    rop = refbind.Op(True, 2)
+    zr, dual, gaps, its, zerrs, st, rws = rop.solve(xs, 0.1, tol=1e-10, max_iter=10 ** 6, threads=4, keep=True)
+    try:
+        op, ws, xt, zd, dd, gd = _kept(xs, 0.1, [1.0, 1.0], None, 1e-10)
+        np.testing.assert_array_equal(zd.cpu().numpy(), zr)
+        print("reference VJP", flush=True)
+        rgx, rgg, _ = rop.vjp(rws, gzs, deriv_tol=1e-12, deriv_max_iter=100000)
+        ws.set_deriv_tol(1e-12, 100000)
+        gxd = torch.empty_like(xt)
+        ggd = ws.vjp(torch.tensor(gzs, device="cuda"), gxd)
+        assert _rel(gxd.cpu().numpy(), rgx) <= 1e-7
+        assert ggd == pytest.approx(rgg, rel=1e-7)
+    finally:
+        refbind.Op.free_ws(rws)
