@@ -78,7 +78,10 @@ public:
     bool release_dual(int64_t id);
     bool evict_one_dual();
     const KernelChoice &kernel() const { return kc_; }
-    int steps_per_sync = 8;
+    // Launch pairs per chunk: a finished request is seen when its chunk's
+    // snapshot is read, so shorter chunks cut latency (4 against 8: served
+    // p50 0.67 against 0.78 ms, the same p99; 2 lengthens the p99).
+    int steps_per_sync = 4;
     hipStream_t stream = nullptr;
     int64_t launches = 0;
 
