@@ -2009,6 +2009,12 @@ FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *e
     return FP_OK;
 }
 
+FP_API fp_status fpr_batch_busy(const fpr_batch *b, double *busy_ms) {
+    if (!b || !b->eng || !busy_ms) return FP_EINVAL;
+    *busy_ms = b->eng->stats.busy_ms;
+    return FP_OK;
+}
+
 FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs) {
     if (!ws) return FP_EINVAL;
     if (launches) *launches = ws->eng ? ws->eng->stats.launches : 0;

@@ -8,12 +8,17 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 namespace fpr {
 
 Engine::~Engine() {
     release();
+    for (auto &x : half_ev_)
+        if (x) (void)hipEventDestroy(x);
+    if (half_stream_) (void)hipStreamDestroy(half_stream_);
+    if (ref_ev_) (void)hipEventDestroy(ref_ev_);
     for (auto &x : tev) (void)hipEventDestroy(x);
     for (auto &x : done_ev)
         if (x) (void)hipEventDestroy(x);
@@ -53,6 +58,7 @@ hipStream_t Engine::capture_stream() {
 
 void Engine::release() {
     drop_graphs();
+    half_.reset();
     auto f = [](void *p) {
         if (p) (void)hipFree(p);
     };
@@ -90,14 +96,48 @@ void Engine::release() {
     hgam_ = nullptr;
 }
 
-int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std::vector<ImageSpec> &imgs) {
+int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std::vector<ImageSpec> &all) {
     release();
     tsize_ = tsize;
     P_ = P;
     kc_ = kc;
     P_.TT = kc.iters();
-    specs_ = imgs;
+    // The split: the first images up to half of the batch's tiles stay here.
+    static const bool split_env = [] {
+        const char *e = std::getenv("FASTPROX_ROCM_SPLIT");
+        return !(e && e[0] == '0');
+    }();
+    size_t keep = all.size();
+    if (allow_split && split_env && all.size() >= 2) {
+        std::vector<int64_t> t(all.size());
+        int64_t tot = 0;
+        for (size_t b = 0; b < all.size(); ++b) {
+            const ImageSpec &s = all[b];
+            t[b] = ((s.L[2] + kc.tile_x() - 1) / kc.tile_x()) * ((s.L[1] + kc.tile_y() - 1) / kc.tile_y()) *
+                   kc.tiles_z(s.L[0]);
+            tot += t[b];
+        }
+        if (tot >= kSplitTiles) {
+            int64_t acc = 0;
+            keep = 0;
+            while (keep + 1 < all.size() && 2 * (acc + t[keep]) <= tot) acc += t[keep++];
+            if (keep == 0) keep = 1;
+        }
+    }
+    if (keep < all.size()) {
+        half_.reset(new Engine());
+        half_->allow_split = false;
+        const std::vector<ImageSpec> rest(all.begin() + (std::ptrdiff_t)keep, all.end());
+        if (int e = half_->setup(tsize, P, kc, rest)) return e;
+        hipError_t e;
+        if (!half_stream_ && (e = hipStreamCreateWithFlags(&half_stream_, hipStreamNonBlocking))) return e;
+        for (auto &x : half_ev_)
+            if (!x && (e = hipEventCreateWithFlags(&x, hipEventDisableTiming))) return e;
+    }
+    specs_.assign(all.begin(), all.begin() + (std::ptrdiff_t)keep);
+    const std::vector<ImageSpec> &imgs = specs_;
     const int B = (int)imgs.size();
+    const int Ball = (int)all.size();
     hdesc_.assign((size_t)B, ImgDesc{});
     int64_t arena = 0;
     int tiles = 0;
@@ -145,7 +185,7 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
     if ((e = hipMalloc((void **)&propart_,
                        sizeof(double) * 4 * (size_t)std::max(B, 1) * (size_t)prologue_blocks(std::max<int64_t>(maxN_, 1)))))
         return e;
-    if ((e = hipHostMalloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(B, 1), 0))) return e;
+    if ((e = hipHostMalloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(Ball, 1), 0))) return e;
     if ((e = hipHostMalloc((void **)&hdesc_pinned_, sizeof(ImgDesc) * (size_t)std::max(B, 1), 0))) return e;
     if ((e = hipHostMalloc((void **)&hcount_, 64, 0))) return e;
     if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)std::max(B, 1), 0))) return e;
@@ -168,6 +208,8 @@ int Engine::tile_layout(const KernelChoice &kc, std::vector<ImgDesc> &descs) con
 }
 
 int Engine::set_tail(const KernelChoice &kc, double frac) {
+    if (half_)
+        if (int e = half_->set_tail(kc, frac)) return e;
     drop_graphs();
     tail_frac_ = 0.0;
     if (!(frac > 0.0) || kc.generic != kc_.generic || P_.k != 2) return hipSuccess;
@@ -332,6 +374,9 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
             E.stats.step_ms += a;
             E.stats.epi_ms += b;
             ++E.stats.timed;
+            float t0 = 0.f, t1 = 0.f;
+            if (E.ref_ && !hipEventElapsedTime(&t0, E.ref_, ev[eb]) && !hipEventElapsedTime(&t1, E.ref_, ev[eb + 1]))
+                E.spans_.emplace_back(t0, t1);
         }
     };
     int slot = 0;
@@ -357,9 +402,86 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     return err == (hipError_t)-1 ? -1 : err;
 }
 
+namespace {
+
+// Length of the union of the intervals.
+double union_ms(std::vector<std::pair<float, float>> &v) {
+    std::sort(v.begin(), v.end());
+    double tot = 0.0, lo = 0.0, hi = -1.0;
+    bool open = false;
+    for (const auto &x : v) {
+        if (open && x.first <= hi) {
+            hi = std::max(hi, (double)x.second);
+            continue;
+        }
+        if (open) tot += hi - lo;
+        lo = x.first;
+        hi = x.second;
+        open = true;
+    }
+    if (open) tot += hi - lo;
+    return tot;
+}
+
+}  // namespace
+
 int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol,
                   int max_iter) {
-    const int B = nimg();
+    spans_.clear();
+    ref_ = nullptr;
+    if (timing) {
+        hipError_t e = ref_ev_ ? hipSuccess : hipEventCreate(&ref_ev_);
+        if (!e) e = hipEventRecord(ref_ev_, s);
+        if (e) return e;
+        ref_ = ref_ev_;
+    }
+    if (!half_) {
+        const int r = solve_own(s, ptrs, gammas, warm, tol, max_iter);
+        stats.busy_ms = union_ms(spans_);
+        return r;
+    }
+    const int B0 = own();
+    half_->spans_.clear();
+    half_->ref_ = ref_;
+    half_->timing = timing;
+    half_->graphs = graphs;
+    half_->steps_per_sync = steps_per_sync;
+    // The second half's stream starts after the caller's earlier work on s.
+    hipError_t e = hipEventRecord(half_ev_[0], s);
+    if (!e) e = hipStreamWaitEvent(half_stream_, half_ev_[0], 0);
+    if (e) return e;
+    int dev = 0;
+    if ((e = hipGetDevice(&dev))) return e;
+    int r1 = 0;
+    std::thread worker([&] {
+        r1 = hipSetDevice(dev);
+        if (!r1) r1 = half_->solve_own(half_stream_, ptrs + B0, gammas + B0, warm, tol, max_iter);
+    });
+    const int r0 = solve_own(s, ptrs, gammas, warm, tol, max_iter);
+    worker.join();
+    // Later work on s follows the second half's (which has finished: its
+    // solve synchronises its stream).
+    if (!r1) {
+        e = hipEventRecord(half_ev_[1], half_stream_);
+        if (!e) e = hipStreamWaitEvent(s, half_ev_[1], 0);
+        if (e) r1 = e;
+    }
+    if (r0 || r1) return r0 ? r0 : r1;
+    std::memcpy(hstate_ + B0, half_->state(), sizeof(ImgState) * (size_t)half_->nimg());
+    const SolveStats &h = half_->stats;
+    stats.launches += h.launches;
+    stats.host_syncs += h.host_syncs;
+    stats.step_ms += h.step_ms;
+    stats.epi_ms += h.epi_ms;
+    stats.timed += h.timed;
+    spans_.insert(spans_.end(), half_->spans_.begin(), half_->spans_.end());
+    stats.busy_ms = union_ms(spans_);
+    return hipSuccess;
+}
+
+int Engine::solve_own(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol,
+                      int max_iter) {
+    const int B = own();
     stats = SolveStats{};
     if (B == 0) return hipSuccess;
     P_.warm = warm ? 1 : 0;
@@ -407,11 +529,15 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
 int Engine::copy_dual_iterate(hipStream_t s, void *dst, int64_t img_stride) const {
     if (!arena_) return hipErrorInvalidValue;
     if (tsize_ == 4)
-        launch_copy_u<float>((const float *)arena_, desc_, state_, nimg(), P_.k, maxN_, (float *)dst, img_stride, s);
+        launch_copy_u<float>((const float *)arena_, desc_, state_, own(), P_.k, maxN_, (float *)dst, img_stride, s);
     else
-        launch_copy_u<double>((const double *)arena_, desc_, state_, nimg(), P_.k, maxN_, (double *)dst, img_stride,
+        launch_copy_u<double>((const double *)arena_, desc_, state_, own(), P_.k, maxN_, (double *)dst, img_stride,
                               s);
-    return hipGetLastError();
+    if (hipError_t e = hipGetLastError()) return e;
+    if (half_)
+        return half_->copy_dual_iterate(s, (char *)dst + (size_t)own() * (size_t)img_stride * (size_t)tsize_,
+                                        img_stride);
+    return hipSuccess;
 }
 
 }  // namespace fpr

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "tv_common.hpp"
@@ -35,6 +37,10 @@ struct SolveStats {
     double step_ms = 0.0;  // summed step-kernel time (HIP events), when timing is on
     double epi_ms = 0.0;   // summed epilogue time
     int timed = 0;         // step launches timed
+    // Time at least one step kernel ran, over both streams of a split
+    // batch (the union of the timed step intervals); equal to step_ms
+    // when one stream runs them.
+    double busy_ms = 0.0;
 };
 
 class Engine {
@@ -68,8 +74,7 @@ public:
     int copy_dual_iterate(hipStream_t s, void *dst, int64_t img_stride) const;
 
     const ImgState *state() const { return hstate_; }
-    ImgState *device_state() const { return state_; }
-    int nimg() const { return (int)specs_.size(); }
+    int nimg() const { return own() + (half_ ? half_->nimg() : 0); }
     int total_tiles() const { return total_tiles_; }
     const KernelChoice &kernel() const { return kc_; }
     const Problem &problem() const { return P_; }
@@ -100,8 +105,27 @@ public:
     int ensure_events(int steps, bool timed);
     std::vector<hipEvent_t> tev;
     hipEvent_t done_ev[2] = {nullptr, nullptr};
+    // With timing: the event every step interval is measured from (recorded
+    // on the caller's stream before either half starts), and the intervals
+    // [start, end) in ms of this engine's timed step launches.
+    hipEvent_t ref_ = nullptr;
+    std::vector<std::pair<float, float>> spans_;
+
+    // A batch of at least kSplitTiles tiles is solved as two halves on two
+    // streams (the second half by a worker thread): while one half waits
+    // for its epilogue, a launch or its host sync, the other's step kernel
+    // fills the GPU. Every image's result is the same either way.
+    // FASTPROX_ROCM_SPLIT=0 turns it off.
+    static constexpr int kSplitTiles = 8192;
+    bool allow_split = true;
 
 private:
+    int own() const { return (int)specs_.size(); }  // the images of this engine itself
+    int solve_own(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
+    std::unique_ptr<Engine> half_;      // images [own(), nimg()) when split
+    hipStream_t half_stream_ = nullptr;
+    hipEvent_t half_ev_[2] = {nullptr, nullptr};
+    hipEvent_t ref_ev_ = nullptr;  // owned reference event of timed solves
     void release();
     void drop_graphs();
     std::vector<ChunkGraph> graphs_;

@@ -350,6 +350,14 @@ class Batch:
         _check(lib().fpr_batch_timing(self.h, C.byref(a), C.byref(b), C.byref(c)), "fpr_batch_timing")
         return a.value, b.value, c.value
 
+    def busy_ms(self):
+        """Time at least one step kernel of the last timed solve ran
+        (fpr_batch_busy): below the summed step time when the batch's two
+        halves ran on two streams at once."""
+        a = C.c_double()
+        _check(lib().fpr_batch_busy(self.h, C.byref(a)), "fpr_batch_busy")
+        return a.value
+
     def __del__(self):
         try:
             if getattr(self, "h", None) and self.h.value:

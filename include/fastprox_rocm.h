@@ -258,6 +258,12 @@ FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_sy
  * step launches timed. */
 FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on);
 FP_API fp_status fpr_batch_timing(const fpr_batch *b, double *step_ms, double *epilogue_ms, int *launches);
+/* With timing on: the time, in ms, during which at least one step kernel of
+ * the last solve ran. A batch of many tiles runs as two halves on two
+ * streams whose step kernels overlap, so this is the device's step time,
+ * below the summed step_ms of fpr_batch_timing; with one stream the two
+ * agree. */
+FP_API fp_status fpr_batch_busy(const fpr_batch *b, double *busy_ms);
 /* Each chunk of a solve's step and epilogue launches is captured once as a
  * hipGraph and replayed (on by default; timing turns it off for the timed
  * solves); on = 0 launches every kernel on its own. Results are the same

@@ -404,3 +404,32 @@ def test_3d_streaming_kernel_matches_generic(cuda, dtype, iso):
         for z, dual, its in outs[1:]:
             np.testing.assert_array_equal(its, outs[0][2])
             assert torch.equal(z, outs[0][0]) and torch.equal(dual, outs[0][1])
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_split_batch_matches_per_image_solves(cuda, dtype):
+    """A batch of more than 8192 tiles runs as two halves on two streams
+    (Engine::kSplitTiles): every image has the bits of its own plain solve on
+    the same kernel, through a tail switch, and the kept duals and results
+    are in image order."""
+    n, shape = 120, (96, 128)
+    rng = np.random.default_rng(77)
+    x = torch.tensor(rectangles(n, shape, n=6, sigma=0.05, seed=77), dtype=dtype, device=cuda)
+    gam = list(np.exp(rng.uniform(np.log(0.02), np.log(0.3), n)))
+    bt = Batch(Op(True, 2), dtype, [shape] * n, kernel=(0, 2, 4))
+    bt.set_tail(4, 4, 0.5)
+    zs = [torch.empty_like(v) for v in x]
+    ds = [torch.empty((2,) + shape, dtype=dtype, device=cuda) for _ in range(n)]
+    bt.solve(list(x), zs, ds, gam, tol=1e-7, max_iter=100000)
+    gaps, its, zerrs, conv = bt.image_info()
+    for i in range(0, n, 7):
+        ws = Workspace(Op(True, 2), x[i:i + 1].contiguous())
+        ws.set_kernel(0, 2, 4)
+        ws.set_tail(4, 4, 0.5)
+        z = torch.empty_like(x[i:i + 1])
+        ws.prox(x[i:i + 1].contiguous(), float(gam[i]), z, tol=1e-7, max_iter=100000)
+        g1, i1, _ = ws.image_info(1)
+        d1 = torch.empty((2, 1) + shape, dtype=dtype, device=cuda)
+        ws.get_dual(d1)
+        assert int(its[i]) == int(i1[0]), f"image {i}"
+        assert torch.equal(zs[i], z[0]) and torch.equal(ds[i], d1[:, 0])
