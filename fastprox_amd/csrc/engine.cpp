@@ -102,13 +102,17 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
     P_ = P;
     kc_ = kc;
     P_.TT = kc.iters();
-    // The split: the first images up to half of the batch's tiles stay here.
-    static const bool split_env = [] {
+    // The split: the first images up to 1/parts of the batch's tiles stay
+    // here, the others go to a second engine (which splits them again into
+    // parts - 1 when parts > 2).
+    static const int split_env = [] {
         const char *e = std::getenv("FASTPROX_ROCM_SPLIT");
-        return !(e && e[0] == '0');
+        const int v = e ? std::atoi(e) : 2;
+        return std::min(std::max(v, 1), kMaxParts);
     }();
+    const int parts = split_parts_ > 0 ? split_parts_ : split_env;
     size_t keep = all.size();
-    if (allow_split && split_env && all.size() >= 2) {
+    if (allow_split && parts >= 2 && all.size() >= 2) {
         std::vector<int64_t> t(all.size());
         int64_t tot = 0;
         for (size_t b = 0; b < all.size(); ++b) {
@@ -117,16 +121,17 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
                    kc.tiles_z(s.L[0]);
             tot += t[b];
         }
-        if (tot >= kSplitTiles) {
+        if (split_parts_ > 0 || tot >= kSplitTiles) {
             int64_t acc = 0;
             keep = 0;
-            while (keep + 1 < all.size() && 2 * (acc + t[keep]) <= tot) acc += t[keep++];
+            while (keep + 1 < all.size() && parts * (acc + t[keep]) <= tot) acc += t[keep++];
             if (keep == 0) keep = 1;
         }
     }
     if (keep < all.size()) {
         half_.reset(new Engine());
-        half_->allow_split = false;
+        half_->split_parts_ = parts - 1;
+        half_->allow_split = parts > 2;
         const std::vector<ImageSpec> rest(all.begin() + (std::ptrdiff_t)keep, all.end());
         if (int e = half_->setup(tsize, P, kc, rest)) return e;
         hipError_t e;
@@ -435,11 +440,14 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
         if (e) return e;
         ref_ = ref_ev_;
     }
-    if (!half_) {
-        const int r = solve_own(s, ptrs, gammas, warm, tol, max_iter);
-        stats.busy_ms = union_ms(spans_);
-        return r;
-    }
+    const int r = solve_parts(s, ptrs, gammas, warm, tol, max_iter);
+    if (!r) stats.busy_ms = union_ms(spans_);
+    return r;
+}
+
+int Engine::solve_parts(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol,
+                        int max_iter) {
+    if (!half_) return solve_own(s, ptrs, gammas, warm, tol, max_iter);
     const int B0 = own();
     half_->spans_.clear();
     half_->ref_ = ref_;
@@ -455,7 +463,7 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
     int r1 = 0;
     std::thread worker([&] {
         r1 = hipSetDevice(dev);
-        if (!r1) r1 = half_->solve_own(half_stream_, ptrs + B0, gammas + B0, warm, tol, max_iter);
+        if (!r1) r1 = half_->solve_parts(half_stream_, ptrs + B0, gammas + B0, warm, tol, max_iter);
     });
     const int r0 = solve_own(s, ptrs, gammas, warm, tol, max_iter);
     worker.join();
@@ -475,7 +483,6 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
     stats.epi_ms += h.epi_ms;
     stats.timed += h.timed;
     spans_.insert(spans_.end(), half_->spans_.begin(), half_->spans_.end());
-    stats.busy_ms = union_ms(spans_);
     return hipSuccess;
 }
 

@@ -111,17 +111,21 @@ public:
     hipEvent_t ref_ = nullptr;
     std::vector<std::pair<float, float>> spans_;
 
-    // A batch of at least kSplitTiles tiles is solved as two halves on two
-    // streams (the second half by a worker thread): while one half waits
-    // for its epilogue, a launch or its host sync, the other's step kernel
-    // fills the GPU. Every image's result is the same either way.
-    // FASTPROX_ROCM_SPLIT=0 turns it off.
+    // A batch of at least kSplitTiles tiles is solved as parts (two by
+    // default) on as many streams, each part after the first driven by a
+    // worker thread: while one part waits for its epilogue, a launch or its
+    // host sync, another's step kernel fills the GPU. Every image's result is
+    // the same either way. FASTPROX_ROCM_SPLIT=n asks for n parts (at most
+    // kMaxParts; 0 or 1 turns the split off).
     static constexpr int kSplitTiles = 8192;
+    static constexpr int kMaxParts = 4;
     bool allow_split = true;
 
 private:
     int own() const { return (int)specs_.size(); }  // the images of this engine itself
     int solve_own(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
+    int solve_parts(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
+    int split_parts_ = 0;               // parts this engine's images are split into (0: the environment's)
     std::unique_ptr<Engine> half_;      // images [own(), nimg()) when split
     hipStream_t half_stream_ = nullptr;
     hipEvent_t half_ev_[2] = {nullptr, nullptr};
