@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB five times (the HBM counters' scale for the traffic figure)")
-    ap.add_argument("--legs", default="b256,uneven,aniso,3d,backward,serve",
+    ap.add_argument("--legs", default="b256,uneven,aniso,3d,backward,serve512,serve",
                     help="the other configurations to measure after the batched one: any of b256 (the same "
                          "images, 256 per batch), uneven (256 uneven "
                          "images with pixel weights), aniso (64 x 512^2 f32 and f64), 3d (16 x 128^3), "
@@ -199,6 +199,8 @@ def main():
         extra["volumes_3d"] = leg_3d(args)
     if "backward" in legs:
         extra["backward"] = leg_backward(args)
+    if "serve512" in legs and S == 512:
+        extra["served_512"] = leg_serve512(args, xs, plain)
     if "serve" in legs:
         extra["served"] = leg_serve(args)
 
@@ -453,6 +455,48 @@ def leg_serve(args):
         st["cpu"] = serve_cpu_baseline(trace, a, b)
         log(f"served cpu: {st['cpu']}")
     return st
+
+
+def leg_serve512(args, xs, plain, n=2048):
+    """Served capacity at 512^2: n requests of the headline images (each of
+    the batch's images n / B times), gamma 0.1, tol 1e-6, all submitted at
+    once, so the offered load exceeds what the server completes; images/s
+    from the first submission to the last completion, through the same
+    continuous-batching loop (slots, pool admission, chunks in flight) as
+    the request trace."""
+    import torch
+
+    from fastprox_amd import Op
+    from fastprox_amd.serve import Server
+    B = xs.shape[0]
+    srv = Server(Op(True, 2), None, pool_bytes=int(args.serve_pool_gib * (1 << 30)), slots=512)
+    outs = [torch.empty_like(xs[0]) for _ in range(n)]
+
+    def run(count):
+        t0 = srv.clock()
+        for i in range(count):
+            srv.submit(xs[i % B], args.gamma, tol=args.tol, max_iter=args.max_iter, out=outs[i], t_submit=t0)
+        res = srv.drain()
+        return res, srv.clock() - t0
+    run(64)  # warm-up, not recorded
+    res, sec = run(n)
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res]) * 1e3
+    ok = all(r["converged"] for r in res) and len(res) == n
+    # the steady state: completions between the first and the last quarter
+    td = np.sort([r["t_done"] for r in res])
+    steady = (n // 2) / (td[3 * n // 4] - td[n // 4])
+    out = {"workload": f"{n} requests of the headline 512^2 float32 images, gamma {args.gamma}, tol {args.tol}, "
+                       f"all submitted at t = 0 (offered load above capacity), 512 slots, "
+                       f"{args.serve_pool_gib:.0f} GiB pool",
+           "images_per_s": n / sec, "seconds": sec, "converged": bool(ok),
+           "steady_images_per_s": float(steady),
+           "steady_speedup_vs_plain": (float(steady) / plain) if plain else None,
+           "kernel": "stream:TT={1},SEG={2}".format(*srv.kernel(2)),
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "speedup_vs_plain": (n / sec / plain) if plain else None}
+    log(f"served 512: {out}")
+    del srv
+    return out
 
 
 def serve_cpu_baseline(trace, src2d, src3d, budget_s=25.0):

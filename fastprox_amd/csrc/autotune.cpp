@@ -164,22 +164,28 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
     return kc;
 }
 
-// Measured on the served trace (Poisson 200 requests/s, 80% 2D of side
-// 128..1024, 20% 3D of side 48..128, f32): these give p50 0.78 ms and p99
-// 36 ms, the batch kernels of a 512^2 image (TT 6, 32 rows) and of large
-// volumes (16 slices) 1.1 ms and 43 ms: a request's latency is its launches'
-// count times their length, and short launches shorten both classes.
+// 2D: the batch kernel of a 512^2 image (autotuned; TT 6, 32 rows on
+// MI355X). Measured on 2048 queued 512^2 requests and on the served trace
+// (Poisson 200 requests/s for 20 s, 80% 2D of side 128..1024, 20% 3D of side
+// 48..128, f32), against 4 iterations and 16 rows:
+//   TT 6, 32 rows: 3,958 images/s (4,129 steady), trace p50 0.89 ms, p99 40 ms
+//   TT 4, 32 rows: 3,759 (4,064),                 p50 0.81 ms, p99 40 ms
+//   TT 4, 16 rows: 3,364 (3,624),                 p50 0.67 ms, p99 37 ms
+// Short launches help a lone request, long ones a full server; the batch
+// kernel keeps the served rate at 512^2 above 20x the plain per-call rate
+// (182 images/s). 3D: slabs of 8 slices (the 3D requests set the trace's
+// p99; the batch kernel's 16-slice slabs lengthen it).
 KernelChoice serve_kernel(int k, int tsize, bool iso) {
     const char *env = std::getenv(k == 2 ? "FASTPROX_ROCM_KERNEL" : "FASTPROX_ROCM_KERNEL3D");
-    if (env) {
+    if (env || k == 2) {
         const int64_t img[3] = {k == 3 ? 128 : 1, 512, 512};
         return default_kernel(k, tsize, img, iso, false, 64);
     }
     KernelChoice kc;
-    kc.TT = k == 2 ? 4 : 1;
-    kc.RY = k == 2 ? 16 : 8;
-    kc.stream = k == 2;
-    kc.s3 = k == 3;
+    kc.TT = 1;
+    kc.RY = 8;
+    kc.stream = false;
+    kc.s3 = true;
     return kc;
 }
 

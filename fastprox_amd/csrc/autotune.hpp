@@ -18,10 +18,9 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
 // set_tail); frac 0 when there is none.
 KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, double *frac);
 
-// The step kernel of a serving class: the shortest launches that still fill
-// the GPU under load (2D: 4 iterations per launch, segments of 16 rows; 3D:
-// slabs of 8 slices), unless FASTPROX_ROCM_KERNEL / FASTPROX_ROCM_KERNEL3D
-// name one.
+// The step kernel of a serving class (2D: the batch kernel of a 512^2
+// image; 3D: slabs of 8 slices), unless FASTPROX_ROCM_KERNEL /
+// FASTPROX_ROCM_KERNEL3D name one.
 KernelChoice serve_kernel(int k, int tsize, bool iso);
 
 }  // namespace fpr

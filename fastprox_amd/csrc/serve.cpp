@@ -83,10 +83,14 @@ void RangeAlloc::free(int64_t off, int64_t n) {
 SlotPool::~SlotPool() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void *p : {pool_, (void *)desc_, (void *)state_, (void *)part_, (void *)gam_, (void *)propart_,
-                    (void *)list_[0], (void *)list_[1], (void *)counts_})
+                    (void *)list_[0], (void *)list_[1], (void *)counts_, (void *)dstage_, (void *)sstate_,
+                    (void *)gpart_})
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)hdesc_, (void *)hgam_, (void *)hsnap_[0], (void *)hsnap_[1]})
+    for (void *p : {(void *)hdesc_, (void *)hgam_, (void *)hsnap_[0], (void *)hsnap_[1], (void *)hstage_[0],
+                    (void *)hstage_[1]})
         if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : stage_ev_)
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -115,6 +119,13 @@ int SlotPool::setup(const Problem &P, const KernelChoice &kc, int tsize, int slo
     for (int i = 0; i < 2; ++i)
         if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)tile_cap))) return e;
     if ((e = hipMalloc((void **)&counts_, 64))) return e;
+    if ((e = hipMalloc((void **)&dstage_, sizeof(Stage)))) return e;
+    if ((e = hipMalloc((void **)&sstate_, sizeof(ImgState) * kGroupMax))) return e;
+    if ((e = hipMalloc((void **)&gpart_, sizeof(double) * 4 * (size_t)kGroupBlocks))) return e;
+    for (int i = 0; i < 2; ++i) {
+        if ((e = hipHostMalloc((void **)&hstage_[i], sizeof(Stage), 0))) return e;
+        if ((e = hipEventCreateWithFlags(&stage_ev_[i], hipEventDisableTiming))) return e;
+    }
     if ((e = hipHostMalloc((void **)&hdesc_, sizeof(ImgDesc) * (size_t)slots, 0))) return e;
     if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)slots, 0))) return e;
     for (int i = 0; i < 2; ++i)
@@ -167,8 +178,9 @@ static void launch_setup(const Problem &Pi, void *pool, ImgDesc *desc, ImgState 
     if (tv1d) launch_tv1d<T>(Pi, desc, state, 1, N, u_round, eta, s);
 }
 
-bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
-    *err = 0;
+// A free slot, the request's pool memory (x, z, state; the dual) and its
+// tiles; its descriptor. False when any of them is not free now.
+bool SlotPool::reserve(const ServeRequest &r, int *slot, ImgDesc *Dout, int64_t *offs) {
     int i = -1;
     for (int j = 0; j < nslots_; ++j)
         if (!slots_[(size_t)j].used) {
@@ -198,7 +210,6 @@ bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
         mem_.free(off_dual, n_dual);
         return false;
     }
-    hipStream_t s = stream;
     char *base = (char *)pool_;
     const size_t ts = (size_t)tsize_;
     D.d = base + (size_t)off_main * ts;
@@ -209,14 +220,145 @@ bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
     D.q_cstride = N;
     D.sw_cstride = N;
     D.tile_base = (int32_t)tile_off;
-    // the input, gathered from its strides
-    StridedShape sh{};
-    int64_t st[3] = {r.xs[0], r.xs[1], r.xs[2]};
+    *slot = i;
+    *Dout = D;
+    offs[0] = off_main;
+    offs[1] = n_main;
+    offs[2] = off_dual;
+    offs[3] = n_dual;
+    offs[4] = tile_off;
+    return true;
+}
+
+void SlotPool::occupy(int i, const ServeRequest &r, const ImgDesc &D, const int64_t *offs, double now) {
+    Slot &S = slots_[(size_t)i];
+    S.used = true;
+    S.done = false;
+    S.req = r;
+    S.t_admit = now;
+    S.off_main = offs[0];
+    S.n_main = offs[1];
+    S.off_dual = offs[2];
+    S.n_dual = offs[3];
+    S.tile_off = offs[4];
+    S.ntiles = D.ntiles;
+    S.N = r.L[0] * r.L[1] * r.L[2];
+    S.admit_seq = seq_;
+    ++live_;
+    live_tiles_ += D.ntiles;
+    top_ = std::max(top_, i + 1);
+}
+
+// x's strides, or the contiguous ones when none are given.
+static void input_strides(const ServeRequest &r, int64_t st[3]) {
+    for (int a = 0; a < 3; ++a) st[a] = r.xs[a];
     if (!st[0] && !st[1] && !st[2]) {
         st[2] = 1;
         st[1] = r.L[2];
         st[0] = r.L[1] * r.L[2];
     }
+}
+
+// Whether the prologue will solve r as exact 1D TV (one active axis, the
+// rule of prologue_combine2), which needs the 1D kernels after it.
+bool SlotPool::exact_1d(const ServeRequest &r) const {
+    if (!P_.iso || P_.sw || !(r.gamma > 0.0)) return false;
+    const int k = P_.k;
+    double wmaxL = 0.0;
+    for (int a = 0; a < k; ++a)
+        if (r.L[3 - k + a] > 1) wmaxL = std::max(wmaxL, P_.w[a]);
+    int active = 0;
+    for (int a = 0; a < k; ++a)
+        if (P_.w[a] > 0.0 && r.L[3 - k + a] > 1 && P_.w[a] >= 1e-100 * wmaxL) ++active;
+    return active == 1;
+}
+
+bool SlotPool::groupable(const ServeRequest &r) const {
+    if (r.warm_from >= 0 || exact_1d(r)) return false;
+    const int64_t N = r.L[0] * r.L[1] * r.L[2];
+    return prologue_blocks(N) <= kGroupBlocks / kGroupMax;
+}
+
+bool SlotPool::stage(const ServeRequest &r, double now, int *err) {
+    *err = 0;
+    if (staged_ == kGroupMax)
+        if ((*err = flush_group())) return false;
+    int i = -1;
+    ImgDesc D{};
+    int64_t offs[5];
+    if (!reserve(r, &i, &D, offs)) return false;
+    if (staged_ == 0) {
+        // the staging buffer's previous copy has been read
+        if ((*err = hipEventSynchronize(stage_ev_[stage_buf_]))) return false;
+        staged_maxN_ = 0;
+    }
+    Stage &G = *hstage_[stage_buf_];
+    const int j = staged_++;
+    const int64_t N = r.L[0] * r.L[1] * r.L[2];
+    G.desc[j] = D;
+    G.gam[j] = r.gamma;
+    G.tol[j] = r.tol;
+    G.maxit[j] = r.max_iter;
+    G.map[j] = i;
+    GatherEntry &g = G.gat[j];
+    int64_t st[3];
+    input_strides(r, st);
+    g.src = r.x;
+    g.dst = (void *)D.d;
+    g.n = N;
+    for (int a = 0; a < 3; ++a) {
+        g.shape[a] = r.L[a];
+        g.stride[a] = st[a];
+    }
+    g.contiguous = st[2] == 1 && st[1] == r.L[2] && st[0] == r.L[1] * r.L[2];
+    g.pad_ = 0;
+    staged_maxN_ = std::max(staged_maxN_, N);
+    occupy(i, r, D, offs, now);
+    return true;
+}
+
+int SlotPool::flush_group() {
+    if (staged_ == 0) return 0;
+    const int G = staged_;
+    staged_ = 0;
+    hipStream_t s = stream;
+    const int b = stage_buf_;
+    stage_buf_ ^= 1;
+    hipError_t e = hipMemcpyAsync(dstage_, hstage_[b], sizeof(Stage), hipMemcpyHostToDevice, s);
+    if (e) return e;
+    if ((e = hipEventRecord(stage_ev_[b], s))) return e;
+    Problem Pi = P_;
+    Pi.warm = 0;
+    if (tsize_ == 4) {
+        launch_gather_group<float>(dstage_->gat, G, staged_maxN_, s);
+        launch_prologue<float>(Pi, dstage_->desc, sstate_, dstage_->gam, G, gpart_, staged_maxN_, s, dstage_->tol,
+                               dstage_->maxit);
+    } else {
+        launch_gather_group<double>(dstage_->gat, G, staged_maxN_, s);
+        launch_prologue<double>(Pi, dstage_->desc, sstate_, dstage_->gam, G, gpart_, staged_maxN_, s, dstage_->tol,
+                                dstage_->maxit);
+    }
+    launch_admit_commit(dstage_->desc, sstate_, dstage_->gam, dstage_->map, desc_, state_, gam_, list_[p_],
+                        counts_ + p_, G, s);
+    return hipGetLastError();
+}
+
+bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
+    *err = 0;
+    if ((*err = flush_group())) return false;  // admissions stay in order
+    int i = -1;
+    ImgDesc D{};
+    int64_t offs[5];
+    if (!reserve(r, &i, &D, offs)) return false;
+    const int64_t N = r.L[0] * r.L[1] * r.L[2];
+    const int64_t n_dual = offs[3];
+    hipStream_t s = stream;
+    char *base = (char *)pool_;
+    const size_t ts = (size_t)tsize_;
+    // the input, gathered from its strides
+    StridedShape sh{};
+    int64_t st[3];
+    input_strides(r, st);
     for (int a = 0; a < 3; ++a) {
         sh.shape[a] = r.L[a];
         sh.stride[a] = st[a];
@@ -230,49 +372,46 @@ bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
         if (it != kept_.end() && it->second.L[0] == r.L[0] && it->second.L[1] == r.L[1] &&
             it->second.L[2] == r.L[2]) {
             if ((*err = hipMemcpyAsync(D.q, base + (size_t)it->second.off * ts, (size_t)n_dual * ts,
-                                       hipMemcpyDeviceToDevice, s)))
+                                       hipMemcpyDeviceToDevice, s))) {
+                release_reserved(D, offs);
                 return false;
+            }
             it->second.stamp = ++stamp_;
             warm = true;
         }
     }
     hdesc_[i] = D;
     hgam_[i] = r.gamma;
-    if ((*err = hipMemcpyAsync(desc_ + i, hdesc_ + i, sizeof(ImgDesc), hipMemcpyHostToDevice, s))) return false;
-    if ((*err = hipMemcpyAsync(gam_ + i, hgam_ + i, sizeof(double), hipMemcpyHostToDevice, s))) return false;
+    if ((*err = hipMemcpyAsync(desc_ + i, hdesc_ + i, sizeof(ImgDesc), hipMemcpyHostToDevice, s)) ||
+        (*err = hipMemcpyAsync(gam_ + i, hgam_ + i, sizeof(double), hipMemcpyHostToDevice, s))) {
+        release_reserved(D, offs);
+        return false;
+    }
     Problem Pi = P_;
     Pi.tol = r.tol;
     Pi.max_iter = r.max_iter;
     Pi.warm = warm ? 1 : 0;
-    int active = 0;
-    for (int a = 0; a < k; ++a)
-        if (r.L[3 - k + a] > 1 && P_.w[a] > 0.0) ++active;
-    const bool tv1d = P_.iso && !P_.sw && active == 1;
+    const bool tv1d = exact_1d(r);
     if (tsize_ == 4) launch_setup<float>(Pi, pool_, desc_ + i, state_ + i, gam_ + i, propart_, N, warm, tv1d, s);
     else launch_setup<double>(Pi, pool_, desc_ + i, state_ + i, gam_ + i, propart_, N, warm, tv1d, s);
     launch_initial_list(desc_ + i, state_ + i, list_[p_], counts_ + p_, 1, s, i);
-    if ((*err = hipGetLastError())) return false;
-    Slot &S = slots_[(size_t)i];
-    S.used = true;
-    S.done = false;
-    S.req = r;
-    S.t_admit = now;
-    S.off_main = off_main;
-    S.n_main = n_main;
-    S.off_dual = off_dual;
-    S.n_dual = n_dual;
-    S.tile_off = tile_off;
-    S.ntiles = D.ntiles;
-    S.N = N;
-    S.admit_seq = seq_;
-    ++live_;
-    live_tiles_ += D.ntiles;
-    top_ = std::max(top_, i + 1);
+    if ((*err = hipGetLastError())) {
+        release_reserved(D, offs);
+        return false;
+    }
+    occupy(i, r, D, offs, now);
     return true;
+}
+
+void SlotPool::release_reserved(const ImgDesc &D, const int64_t *offs) {
+    mem_.free(offs[0], offs[1]);
+    mem_.free(offs[2], offs[3]);
+    tiles_.free(offs[4], D.ntiles);
 }
 
 int SlotPool::enqueue_chunk(bool *enqueued) {
     *enqueued = false;
+    if (int e = flush_group()) return e;
     if (live_ == 0 || flight_.size() >= 2) return 0;
     hipStream_t s = stream;
     const int TT = kc_.iters();
@@ -406,10 +545,12 @@ int Server::step(int *live, int *queued) {
     for (int c = 0; c < 2; ++c) {
         SlotPool *P = pools_[c].get();
         if (!P) continue;
-        // admissions, first come first served
+        // admissions, first come first served; most are set up in groups
         while (!queue_[c].empty()) {
             int err = 0;
-            if (P->try_admit(queue_[c].front(), now(), &err)) {
+            const ServeRequest &r = queue_[c].front();
+            const bool ok = P->groupable(r) ? P->stage(r, now(), &err) : P->try_admit(r, now(), &err);
+            if (ok) {
                 queue_[c].pop_front();
                 continue;
             }

@@ -65,6 +65,19 @@ public:
     // Admits r into a free slot if a slot, memory and tiles are free (and
     // enqueues its set-up); false otherwise.
     bool try_admit(const ServeRequest &r, double now, int *err);
+    // Grouped admission: stage() takes r's slot, memory and tiles like
+    // try_admit, and leaves its set-up to flush_group(), which sets up every
+    // staged request at once: one copy of their descriptors, one gather of
+    // their inputs, one prologue, and one kernel that moves the results into
+    // their slots and lists their tiles (instead of about ten small
+    // operations per request). Requests with a warm start, exact 1D TV, or
+    // more than kGroupBlocks / kGroupMax prologue blocks take try_admit.
+    static constexpr int kGroupMax = 64;
+    static constexpr int kGroupBlocks = 64 * 512;
+    bool groupable(const ServeRequest &r) const;
+    bool exact_1d(const ServeRequest &r) const;
+    bool stage(const ServeRequest &r, double now, int *err);
+    int flush_group();
     // Enqueues a chunk of step/epilogue launches and a state snapshot when
     // any slot is live; returns whether one was enqueued.
     int enqueue_chunk(bool *enqueued);
@@ -102,6 +115,27 @@ private:
         uint64_t stamp;
     };
     int finish_slot(int i, const ImgState &S, std::vector<ServeResult> &out, double now);
+    bool reserve(const ServeRequest &r, int *slot, ImgDesc *D, int64_t *offs);
+    void occupy(int i, const ServeRequest &r, const ImgDesc &D, const int64_t *offs, double now);
+    void release_reserved(const ImgDesc &D, const int64_t *offs);
+    // The staged group (pinned, two buffers: one may still be in a copy) and
+    // its device copy; the prologue's states and partials of the group.
+    struct Stage {
+        ImgDesc desc[kGroupMax];
+        double gam[kGroupMax];
+        double tol[kGroupMax];
+        int maxit[kGroupMax];
+        int map[kGroupMax];
+        GatherEntry gat[kGroupMax];
+    };
+    Stage *hstage_[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev_[2] = {nullptr, nullptr};
+    int stage_buf_ = 0;
+    int staged_ = 0;
+    int64_t staged_maxN_ = 0;
+    Stage *dstage_ = nullptr;
+    ImgState *sstate_ = nullptr;
+    double *gpart_ = nullptr;
     Problem P_{};
     KernelChoice kc_{};
     int tsize_ = 4;

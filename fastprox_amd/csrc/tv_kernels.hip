@@ -500,7 +500,7 @@ __global__ void prologue_combine1(const ImgDesc *desc, ImgState *state, const do
 // Pass-2 combine: set the image up (tv_iso.cpp:333-415, 424-483;
 // _tv_torch.py:229-257) and plan its first launch.
 __global__ void prologue_combine2(Problem P, const ImgDesc *desc, ImgState *state, const double *part, int nblk,
-                                  const double *gammas, int nimg) {
+                                  const double *gammas, int nimg, const double *tols, const int *maxits) {
     const int img = blockIdx.x * blockDim.x + threadIdx.x;
     if (img >= nimg) return;
     const ImgDesc &D = desc[img];
@@ -544,7 +544,7 @@ __global__ void prologue_combine2(Problem P, const ImgDesc *desc, ImgState *stat
     S.pending = 0;
     S.after = kAfterNone;
     S.sched = P.sched;
-    S.max_iter = P.max_iter;
+    S.max_iter = maxits ? maxits[img] : P.max_iter;
     S.converged = 1;
     S.t = 1.0;
     S.gp = INFINITY;
@@ -553,7 +553,8 @@ __global__ void prologue_combine2(Problem P, const ImgDesc *desc, ImgState *stat
     S.bound = INFINITY;
     S.wmax = 1.0;
     S.gamma_raw = ldexp(gamma0, -e);
-    S.target = P.tol * 0.5 * s2;
+    const double tol = tols ? tols[img] : P.tol;
+    S.target = tol * 0.5 * s2;
     S.active_axis = -1;
     S.tv1d_lam = 0.0;
     for (int a = 0; a < 3; ++a) S.wa[a] = 0.0;
@@ -603,7 +604,7 @@ __global__ void prologue_combine2(Problem P, const ImgDesc *desc, ImgState *stat
             S.wa[a] = L[a] > 1 ? P.w[a] : 0.0;
             L2 += 4.0 * P.w[a] * P.w[a];
         }
-        S.target = P.tol * s2 / 2.0;
+        S.target = tol * s2 / 2.0;
         if (L2 == 0.0 || !(gamma > 0.0)) {  // _tv_torch.py:247-250: the zero dual, z = x
             S.mode = kOutput;
             S.gp = 0.0;
@@ -1856,6 +1857,69 @@ __global__ void strided_copy_kernel(const T *src, T *dst, int64_t n, int nd, Str
     }
 }
 
+// The inputs of a group of admitted requests, one image per blockIdx.y, into
+// their pool slots (contiguous), each from its own strides.
+template <class T>
+__global__ void gather_group_kernel(const GatherEntry *tab, int G) {
+    const int j = blockIdx.y;
+    if (j >= G) return;
+    const GatherEntry &g = tab[j];
+    const auto src = gin<T>(g.src);
+    const auto dst = gout<T>(g.dst);
+    const int64_t n = g.n;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    if (g.contiguous) {
+        for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += step) dst[i] = src[i];
+        return;
+    }
+    const int64_t L1 = g.shape[1], L2 = g.shape[2];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += step) {
+        const int64_t c2 = i % L2, r = i / L2;
+        const int64_t c1 = r % L1, c0 = r / L1;
+        dst[i] = src[c0 * g.stride[0] + c1 * g.stride[1] + c2 * g.stride[2]];
+    }
+}
+
+// A group's set-up, staged in sdesc/sstate/sgam, moves to its slots map[j]
+// (one block per image), and the tiles of every image the prologue left to
+// iterate or output are appended to the list, as initial_list_kernel does.
+__global__ __launch_bounds__(64) void admit_commit_kernel(const ImgDesc *sdesc, const ImgState *sstate,
+                                                          const double *sgam, const int *map, ImgDesc *desc,
+                                                          ImgState *state, double *gam, Tile *list, int *count,
+                                                          int G) {
+    const int j = blockIdx.x;
+    if (j >= G) return;
+    const int slot = map[j];
+    static_assert(sizeof(ImgDesc) % 4 == 0 && sizeof(ImgState) % 4 == 0, "word copies");
+    {
+        const uint32_t *a = (const uint32_t *)(sdesc + j);
+        uint32_t *b = (uint32_t *)(desc + slot);
+        for (int w = threadIdx.x; w < (int)(sizeof(ImgDesc) / 4); w += 64) b[w] = a[w];
+        const uint32_t *c = (const uint32_t *)(sstate + j);
+        uint32_t *d = (uint32_t *)(state + slot);
+        for (int w = threadIdx.x; w < (int)(sizeof(ImgState) / 4); w += 64) d[w] = c[w];
+        if (threadIdx.x == 0) gam[slot] = sgam[j];
+    }
+    const ImgState &S = sstate[j];
+    const int m = S.mode;
+    if (!(m == kIterate || m == kOutput || m == kOutputNaN)) return;
+    const ImgDesc &D = sdesc[j];
+    __shared__ int base;
+    if (threadIdx.x == 0) base = atomicAdd(count, D.ntiles);
+    __syncthreads();
+    const int nxy = D.ntx * D.nty;
+    for (int i = threadIdx.x; i < D.ntiles; i += 64) {
+        Tile t;
+        t.img = slot;
+        t.tz = i / nxy;
+        const int r = i - t.tz * nxy;
+        t.ty = (int16_t)(r / D.ntx);
+        t.tx = (int16_t)(r - t.ty * D.ntx);
+        t.pad_ = i;
+        list[base + i] = t;
+    }
+}
+
 template <class S, class D>
 __global__ void convert_kernel(const S *src, D *dst, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1876,7 +1940,7 @@ int prologue_blocks(int64_t maxN) { return (int)((maxN + kProChunk - 1) / kProCh
 
 template <class T>
 void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
-                     double *part, int64_t maxN, hipStream_t s) {
+                     double *part, int64_t maxN, hipStream_t s, const double *tols, const int *maxits) {
     const int nblk = prologue_blocks(maxN);
     const dim3 grid(nblk, nimg);
     const int g1 = grid_for(nimg, 64);
@@ -1886,7 +1950,8 @@ void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, con
     hipLaunchKernelGGL(prologue_pass<T>, grid, dim3(kProBlock), 0, s, desc, state, part, nblk, 1);
     hipLaunchKernelGGL(prologue_combine1, dim3(g1), dim3(64), 0, s, desc, state, part, nblk, nimg);
     hipLaunchKernelGGL(prologue_pass<T>, grid, dim3(kProBlock), 0, s, desc, state, part, nblk, 2);
-    hipLaunchKernelGGL(prologue_combine2, dim3(g1), dim3(64), 0, s, P, desc, state, part, nblk, gammas, nimg);
+    hipLaunchKernelGGL(prologue_combine2, dim3(g1), dim3(64), 0, s, P, desc, state, part, nblk, gammas, nimg, tols,
+                       maxits);
 }
 
 void launch_reset_counts(int *counts, hipStream_t s) {
@@ -2013,6 +2078,15 @@ template <class T>
 void launch_scatter(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s) {
     hipLaunchKernelGGL(strided_copy_kernel<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n, nd, sh, 0);
 }
+template <class T>
+void launch_gather_group(const GatherEntry *tab, int G, int64_t maxN, hipStream_t s) {
+    hipLaunchKernelGGL(gather_group_kernel<T>, dim3(grid_for(maxN, 256, 1024), G), dim3(256), 0, s, tab, G);
+}
+void launch_admit_commit(const ImgDesc *sdesc, const ImgState *sstate, const double *sgam, const int *map,
+                         ImgDesc *desc, ImgState *state, double *gam, Tile *list, int *count, int G, hipStream_t s) {
+    hipLaunchKernelGGL(admit_commit_kernel, dim3(G), dim3(64), 0, s, sdesc, sstate, sgam, map, desc, state, gam,
+                       list, count, G);
+}
 template <class S, class D>
 void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL((convert_kernel<S, D>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n);
@@ -2020,7 +2094,8 @@ void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
 
 #define FPR_INST(T)                                                                                              \
     template void launch_prologue<T>(const Problem &, const ImgDesc *, ImgState *, const double *, int, double *, \
-                                     int64_t, hipStream_t);                                                              \
+                                     int64_t, hipStream_t, const double *, const int *);                                 \
+    template void launch_gather_group<T>(const GatherEntry *, int, int64_t, hipStream_t);                             \
     template void launch_warm<T>(const Problem &, T *, const ImgDesc *, const ImgState *, int, int64_t, hipStream_t);                                                         \
     template void launch_tv1d<T>(const Problem &, const ImgDesc *, ImgState *, int, int64_t, double, double,      \
                                  hipStream_t);                                                  \

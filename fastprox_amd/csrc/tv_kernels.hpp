@@ -16,6 +16,18 @@ struct StridedShape {
     int64_t stride[kMaxStrided];
 };
 
+// One image of a grouped gather: n entries of extents shape (3 axes) and
+// element strides stride from src into the contiguous dst.
+struct GatherEntry {
+    const void *src;
+    void *dst;
+    int64_t n;
+    int64_t shape[3];
+    int64_t stride[3];
+    int32_t contiguous;
+    int32_t pad_;
+};
+
 // The step kernel of a solve: the register-tile 2D kernel with TT iterations
 // per launch on tiles of (64 - 2 TT) x (RY - 2 TT) pixels, or the generic
 // one-iteration kernel (2D and 3D).
@@ -41,9 +53,11 @@ bool kernel_available_s3(int RY);
 
 // The prologue's per-block partials: 4 doubles per block, prologue_blocks(maxN) blocks per image.
 int prologue_blocks(int64_t maxN);
+// tols / maxits: per-image tolerance and iteration cap (null: P's).
 template <class T>
 void launch_prologue(const Problem &P, const ImgDesc *desc, ImgState *state, const double *gammas, int nimg,
-                     double *part, int64_t maxN, hipStream_t s);
+                     double *part, int64_t maxN, hipStream_t s, const double *tols = nullptr,
+                     const int *maxits = nullptr);
 void launch_reset_counts(int *counts, hipStream_t s);
 // desc and state may point at image img0 of the arrays the step kernels
 // read: the tiles carry img0 + i.
@@ -68,5 +82,9 @@ void launch_copy_u(const T *arena, const ImgDesc *desc, const ImgState *state, i
                    int64_t img_stride, hipStream_t s);
 template <class S, class D>
 void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s);
+template <class T>
+void launch_gather_group(const GatherEntry *tab, int G, int64_t maxN, hipStream_t s);
+void launch_admit_commit(const ImgDesc *sdesc, const ImgState *sstate, const double *sgam, const int *map,
+                         ImgDesc *desc, ImgState *state, double *gam, Tile *list, int *count, int G, hipStream_t s);
 
 }  // namespace fpr

@@ -11,12 +11,13 @@ import pytest
 import torch
 
 from fastprox_amd import Op, Workspace
+from fastprox_amd import _capi as A
 from fastprox_amd.serve import Server, latency_stats, replay
 from fastprox_amd.synth import crop_of, serving_sources, serving_trace
 
 pytestmark = pytest.mark.gpu
 
-def _alone(x, gamma, tol, warm=None, kernel=None):
+def _alone(x, gamma, tol, warm=None, kernel=None, max_iter=100000):
     """The plain solve of x alone, on the step kernel the server uses for
     x's class (the iterates do not depend on the kernel; the gaps' partial
     sums follow its tiling)."""
@@ -29,7 +30,7 @@ def _alone(x, gamma, tol, warm=None, kernel=None):
         ws.set_dual(warm)
         ws.set_warm(True)
     out = torch.empty_like(xc)
-    st = ws.prox(xc, gamma, out, tol=tol, max_iter=100000)
+    st = ws.prox(xc, gamma, out, tol=tol, max_iter=max_iter)
     gaps, its, _ = ws.image_info(1)
     dual = torch.empty((k,) + tuple(xc.shape), dtype=xc.dtype, device=xc.device)
     ws.get_dual(dual)
@@ -72,6 +73,54 @@ def test_serve_same_alone_and_in_the_mix(cuda):
         assert torch.equal(outs[d["id"]], z)
         admitted_late += d["t_admit"] >= first_done
     assert admitted_late > 0  # continuous batching: admissions after completions
+
+
+def test_serve_grouped_admission(cuda):
+    """150 requests queued at once are set up in groups of up to 64 (one
+    gather, one prologue, one commit per group): each request keeps its own
+    tolerance, iteration cap and strides, and its result has the bits of its
+    plain solve. Exact 1D images and warm starts, which take the single path,
+    are interleaved and keep their order."""
+    src2d, src3d = _sources()
+    rng = np.random.default_rng(21)
+    srv = Server(Op(True, 2), Op(True, 3), pool_bytes=256 << 20, slots=200)
+    kept = torch.empty((60, 70), device="cuda")
+    a = srv.submit(src2d[2][:60, :70], 0.1, tol=1e-5, out=kept, keep_dual=True)
+    srv.drain()
+    cases = {}
+    for j in range(150):
+        if j % 37 == 5:
+            x = src2d[j % 3][7, 10:10 + 30 + j]  # a row: exact 1D TV
+            x = x.reshape(1, -1)
+        elif j % 41 == 7:
+            x = src2d[2][:60, :70]
+        elif j % 5 == 0:
+            x = src3d[j % 2][:8 + j % 7, 3:14, ::2][:, :, :9]  # strided volume
+        else:
+            h, w = (int(v) for v in rng.integers(16, 97, size=2))
+            c0 = (2 * j) % 120
+            x = src2d[j % 3][j:j + h, c0:c0 + 2 * w:2]  # every other column
+        gamma = float(np.exp(rng.uniform(np.log(0.02), np.log(0.3))))
+        tol = float(rng.choice([1e-4, 1e-5, 1e-6]))
+        mi = 7 if j % 11 == 3 else 100000
+        out = torch.empty(tuple(x.shape), device="cuda")
+        warm = a if j % 41 == 7 else -1
+        rid = srv.submit(x, gamma, tol=tol, max_iter=mi, out=out, warm_from=warm)
+        cases[rid] = (x, gamma, tol, mi, out, warm >= 0)
+    res = srv.drain()
+    torch.cuda.synchronize()
+    assert len(res) == 150
+    kern = {2: srv.kernel(2), 3: srv.kernel(3)}
+    _, _, _, _, dual_a = _alone(src2d[2][:60, :70], 0.1, 1e-5, kernel=kern)
+    for d in res:
+        x, gamma, tol, mi, out, warm = cases[d["id"]]
+        z, its, gap, st, _ = _alone(x, gamma, tol, kernel=kern, max_iter=mi,
+                                    warm=dual_a if warm else None)
+        assert d["iters"] == its, (d["id"], tuple(x.shape), mi)
+        assert d["converged"] == (st == A.FP_OK), (d["id"], st)
+        assert d["gap"] == gap and torch.equal(out, z), (d["id"], tuple(x.shape))
+    assert sum(1 for c in cases.values() if c[3] == 7) >= 10
+    assert sum(1 for d in res if not d["converged"]) >= 5  # stopped by their own cap
 
 
 def test_serve_admission_by_memory(cuda):
