@@ -407,7 +407,7 @@ fp_status build_engine(fp_workspace &w) {
     w.eng.reset(new fpr::Engine());
     if (int e = w.eng->setup((int)w.tsize, P, w.kc, specs)) return hip_status(e);
     double frac = 0.0;
-    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)w.tsize, w.img, op.iso, !op.spatial.empty(), &frac);
+    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)w.tsize, w.img, op.iso, !op.spatial.empty(), w.B, &frac);
     if (frac > 0.0 && w.kc.stream && !w.kc.generic && tk.iters() >= w.kc.iters())
         return hip_status(w.eng->set_tail(tk, frac));
     return FP_OK;
@@ -1147,7 +1147,7 @@ fp_status batch_build(fpr_batch &b) {
     for (size_t i = 1; i < b.specs.size() && uniform; ++i)
         for (int a = 0; a < 3; ++a) uniform = uniform && b.specs[i].L[a] == b.specs[0].L[a];
     double frac = 0.0;
-    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)b.tsize, b.rep, op.iso, b.weighted != 0, &frac);
+    const fpr::KernelChoice tk = fpr::default_tail(op.k, (int)b.tsize, b.rep, op.iso, b.weighted != 0, (int64_t)b.specs.size(), &frac);
     if (uniform && frac > 0.0 && b.kc.stream && !b.kc.generic && tk.iters() >= b.kc.iters())
         return hip_status(b.eng->set_tail(tk, frac));
     return FP_OK;

@@ -1,11 +1,14 @@
 // The step-kernel choice (autotune.hpp): a tuning table measured on the
 // device by fastprox_amd/autotune.py, read once from tuning.txt beside this
 // library (or the file FASTPROX_ROCM_TUNING names). Each line is
-//     k dtype_bytes iso spatial max_pixels kind TT RY tail_TT tail_RY tail_frac
+//     k dtype_bytes iso spatial max_pixels kind TT RY tail_TT tail_RY tail_frac [max_images]
 // (kind 0: the streaming 2D kernel, 1: generic, 2: block, 3: the 3D streaming
-// kernel); the entry for an image is the first line of its class whose
-// max_pixels is at least the image's pixel count (the last line of the class
-// beyond that). Without a table the built-in defaults below apply.
+// kernel; max_images, when present and not 0, restricts the line to batches
+// of at most that many images: a lone image fills the GPU only with short
+// tiles); the entry for a batch of B images of n pixels is the first line of
+// its class with n <= max_pixels that admits B (the last line of the class
+// without an image limit beyond that). Without a table the built-in defaults
+// below apply.
 // FASTPROX_ROCM_KERNEL / FASTPROX_ROCM_KERNEL3D / FASTPROX_ROCM_TAIL override
 // everything (for sweeps).
 //
@@ -29,6 +32,7 @@ struct Entry {
     long long max_px;
     int kind, TT, RY, tTT, tRY;
     double tfrac;
+    long long max_images;  // 0: any batch
 };
 
 std::string table_path() {
@@ -50,9 +54,10 @@ std::vector<Entry> load_table() {
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         Entry e{};
-        if (std::sscanf(line, "%d %d %d %d %lld %d %d %d %d %d %lf", &e.k, &e.tsize, &e.iso, &e.sw, &e.max_px,
-                        &e.kind, &e.TT, &e.RY, &e.tTT, &e.tRY, &e.tfrac) == 11)
-            t.push_back(e);
+        const int got = std::sscanf(line, "%d %d %d %d %lld %d %d %d %d %d %lf %lld", &e.k, &e.tsize, &e.iso, &e.sw,
+                                    &e.max_px, &e.kind, &e.TT, &e.RY, &e.tTT, &e.tRY, &e.tfrac, &e.max_images);
+        if (got == 11) e.max_images = 0;
+        if (got >= 11) t.push_back(e);
     }
     std::fclose(f);
     return t;
@@ -63,13 +68,14 @@ const std::vector<Entry> &table() {
     return t;
 }
 
-const Entry *lookup(int k, int tsize, const int64_t *img, bool iso, bool sw) {
+const Entry *lookup(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B) {
     long long n = 1;
     for (int a = 0; a < 3; ++a) n *= img ? (long long)img[a] : 1LL;
     const Entry *last = nullptr;
     for (const Entry &e : table()) {
         if (e.k != k || e.tsize != tsize || e.iso != (iso ? 1 : 0) || e.sw != (sw ? 1 : 0)) continue;
-        last = &e;
+        if (e.max_images > 0 && B > e.max_images) continue;
+        if (e.max_images == 0) last = &e;
         if (n <= e.max_px) return &e;
     }
     return last;
@@ -103,7 +109,6 @@ bool entry_kernel(const Entry &e, int tsize, KernelChoice &kc) {
 }  // namespace
 
 KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B) {
-    (void)B;
     KernelChoice kc;
     if (k != 2) {
         kc.stream = false;
@@ -122,7 +127,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
                 return kc;
             }
         }
-        if (const Entry *t = lookup(k, tsize, img, iso, sw))
+        if (const Entry *t = lookup(k, tsize, img, iso, sw, B))
             if (entry_kernel(*t, tsize, kc)) return kc;
         kc = KernelChoice{};
         kc.stream = false;
@@ -155,7 +160,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
             }
         }
     }
-    if (const Entry *t = lookup(k, tsize, img, iso, sw))
+    if (const Entry *t = lookup(k, tsize, img, iso, sw, B))
         if (entry_kernel(*t, tsize, kc)) return kc;
     kc = KernelChoice{};
     kc.stream = true;
@@ -189,7 +194,7 @@ KernelChoice serve_kernel(int k, int tsize, bool iso) {
     return kc;
 }
 
-KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, double *frac) {
+KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B, double *frac) {
     KernelChoice kc;
     *frac = 0.0;
     if (k != 2) return kc;
@@ -216,7 +221,7 @@ KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool s
         }
         return kc;
     }
-    if (const Entry *t = lookup(k, tsize, img, iso, sw)) {
+    if (const Entry *t = lookup(k, tsize, img, iso, sw, B)) {
         if (t->tfrac > 0.0 && kernel_available(tsize, true, t->tTT, t->tRY)) {
             kc.stream = true;
             kc.TT = t->tTT;

@@ -16,7 +16,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
 
 // The tail kernel and the live-tile fraction that switches to it (Engine::
 // set_tail); frac 0 when there is none.
-KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, double *frac);
+KernelChoice default_tail(int k, int tsize, const int64_t *img, bool iso, bool sw, int64_t B, double *frac);
 
 // The step kernel of a serving class (2D: the batch kernel of a 512^2
 // image; 3D: slabs of 8 slices), unless FASTPROX_ROCM_KERNEL /
