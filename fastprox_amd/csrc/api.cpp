@@ -415,6 +415,15 @@ fp_status build_engine(fp_workspace &w) {
 
 // kind 0: the streaming kernel (tt iterations, segments of ry rows); 1: the
 // generic one-iteration kernel; 2: the block kernel (tt, ry).
+fpr::KernelChoice generic_kernel() {
+    fpr::KernelChoice kc;
+    kc.generic = true;
+    kc.stream = false;
+    kc.TT = 1;
+    kc.RY = 4;
+    return kc;
+}
+
 fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, fpr::KernelChoice &kc) {
     kc = fpr::KernelChoice{};
     if (kind == 1) {
@@ -487,6 +496,7 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     }
     if (fp_status s = check_image(*op, x.shape + (x.ndim - op->k))) return s;
     w->kc = fpr::default_kernel(op->k, (int)w->tsize, w->img, op->iso, !op->spatial.empty(), B);
+    if (w->kc.stream && !fpr::stream_fits((int)w->tsize, w->N)) w->kc = generic_kernel();
     if (fp_status s = build_engine(*w)) return s;
     const size_t bytes = (size_t)w->numel * w->tsize;
     if (fp_status s = alloc(&w->dstage, bytes)) return s;
@@ -1092,6 +1102,7 @@ fp_status vjp_impl(const fp_op *op, fp_workspace *ws, const fp_array *ga, fp_arr
 struct fpr_batch {
     const fp_op *op = nullptr;
     int64_t rep[3] = {1, 1, 1};  // a representative image for the tuning table
+    int64_t maxN = 0;            // the largest image's pixel count
     int dtype = FP_F32;
     size_t tsize = 4;
     int weighted = 0;
@@ -1562,6 +1573,8 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
         if (!ws) return fail(FP_EINVAL, "null workspace");
         fpr::KernelChoice kc;
         if (fp_status st = kernel_choice(generic, tt, ry, ws->k, (int)ws->tsize, kc)) return st;
+        if (kc.stream && !fpr::stream_fits((int)ws->tsize, ws->N))
+            return fail(FP_EINVAL, "images too large for the streaming kernel");
         ws->kc = kc;
         if (ws->numel == 0) return FP_OK;
         ws->dual_valid = false;
@@ -1614,7 +1627,8 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
                 for (int a = 0; a < k; ++a)
                     if (sp.L[3 - k + a] != op->sshape[a]) return FP_ESHAPE;
         }
-        (void)total;
+        b->maxN = 0;
+        for (const fpr::ImageSpec &sp : b->specs) b->maxN = std::max(b->maxN, sp.L[0] * sp.L[1] * sp.L[2]);
         // The tuning entry of a representative image: the batch's mean
         // pixel count, as a square (or cube).
         {
@@ -1623,6 +1637,7 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
             for (int a = 0; a < 3; ++a) b->rep[a] = (a < 3 - k) ? 1 : std::max<int64_t>(1, (int64_t)(side + 0.5));
         }
         b->kc = fpr::default_kernel(k, (int)b->tsize, b->rep, op->iso, b->weighted != 0, count);
+        if (b->kc.stream && !fpr::stream_fits((int)b->tsize, b->maxN)) b->kc = generic_kernel();
         b->ptrs.assign((size_t)count, fpr::ImagePtrs{});
         b->gammas.assign((size_t)count, 0.0);
         b->gaps.assign((size_t)count, 0.0);
@@ -1707,6 +1722,8 @@ FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry)
         if (!b) return FP_EINVAL;
         fpr::KernelChoice kc;
         if (fp_status st = kernel_choice(generic, tt, ry, b->op->k, (int)b->tsize, kc)) return st;
+        if (kc.stream && !fpr::stream_fits((int)b->tsize, b->maxN))
+            return fail(FP_EINVAL, "images too large for the streaming kernel");
         b->kc = kc;
         b->valid = false;
         const bool timing = b->eng && b->eng->timing;

@@ -99,6 +99,19 @@ __device__ __forceinline__ gptr<T> gout(void *p) {
     return (gptr<T>)p;
 }
 
+// Element o of a plane at a uniform base, through a 32-bit byte offset
+// (o * sizeof(T) < 2^32, which kStreamMaxPlaneBytes guarantees for the streaming
+// 2D kernel): the access takes the scalar-base form, with no 64-bit address
+// arithmetic per lane.
+template <class T>
+__device__ __forceinline__ T ld32(gptr<const T> p, uint32_t o) {
+    return *(gptr<const T>)((gptr<const char>)p + o * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(gptr<T> p, uint32_t o, T v) {
+    *(gptr<T>)((gptr<char>)p + o * (uint32_t)sizeof(T)) = v;
+}
+
 template <class T>
 __device__ __forceinline__ T scale_in(T v, int e) {
     return e ? (T)ldexp((double)v, -e) : v;
@@ -1151,11 +1164,15 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     const T w0 = (T)w0d, w1 = (T)w1d;
     const bool act0 = w0d != 0.0, act1 = w1d != 0.0;
     const bool out_mode = mode == kOutput;
+    // Plane bases (uniform); every access below is base + a 32-bit offset.
     const auto dp = gin<T>(D.d);
-    const T *U = A.arena + D.st_off + (int64_t)S.cur * 4 * N;
-    const T *V = S.vsrc_u ? U : U + 2 * N;
-    T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N;
+    const gptr<const T> U0 = (gptr<const T>)(A.arena + D.st_off + (int64_t)S.cur * 4 * N), U1 = U0 + N;
+    const gptr<const T> V0 = S.vsrc_u ? U0 : U0 + 2 * N, V1 = V0 + N;
+    const gptr<T> Uo = (gptr<T>)(A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N);
+    const gptr<T> Uo1 = Uo + N, Uo2 = Uo + 2 * N, Uo3 = Uo + 3 * N;
     const auto sw = SW ? gin<T>(D.sw) : gptr<const T>(nullptr);
+    const auto swb = SW && !ISO ? sw + D.sw_cstride : sw;
+    const auto zo = gout<T>(D.z), qo0 = gout<T>(D.q), qo1 = qo0 + D.q_cstride;
     T beta[TT + 1];
 #pragma unroll
     for (int m = 1; m <= TT; ++m) beta[m] = (T)S.beta[m];
@@ -1194,18 +1211,18 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     T qd = (T)0, qu0 = (T)0, qu1 = (T)0, qv0 = (T)0, qv1 = (T)0, qa = (T)0, qb = (T)0;
     auto load_row = [&](int ys) {
         const bool in = xin && ys >= 0 && ys < H;
-        const int64_t o = in ? (int64_t)ys * W + x : 0;
-        qd = in ? dp[o] : (T)0;
+        const uint32_t o = in ? (uint32_t)(ys * W + x) : 0u;
+        qd = in ? ld32<T>(dp, o) : (T)0;
         qu0 = qu1 = qv0 = qv1 = (T)0;
         if (!fresh && in) {
-            qu0 = U[o];
-            qu1 = U[N + o];
-            qv0 = V[o];
-            qv1 = V[N + o];
+            qu0 = ld32<T>(U0, o);
+            qu1 = ld32<T>(U1, o);
+            qv0 = ld32<T>(V0, o);
+            qv1 = ld32<T>(V1, o);
         }
         if constexpr (SW) {
-            qa = in ? sw[o] : (T)0;
-            if constexpr (!ISO) qb = in ? sw[D.sw_cstride + o] : (T)0;
+            qa = in ? ld32<T>(sw, o) : (T)0;
+            if constexpr (!ISO) qb = in ? ld32<T>(swb, o) : (T)0;
         }
     };
     load_row(ystart);
@@ -1311,27 +1328,27 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             } else {
                 // The final level: row r leaves the wave.
                 const bool st = lane_in && r >= y0 && r < yend;
-                const int64_t so = st ? (int64_t)r * W + x : 0;
+                const uint32_t so = st ? (uint32_t)(r * W + x) : 0u;
                 if (out_mode) {
                     const double ul = from_left((double)u1n);
                     const double zz =
                         ((double)dw[TT] + w0d * ((double)u0n - (double)uf0)) + w1d * ((double)u1n - ul);
                     if (st) {
-                        gout<T>(D.z)[so] = (T)ldexp(zz, e);
+                        st32<T>(zo, so, (T)ldexp(zz, e));
                         if constexpr (ISO) {
-                            gout<T>(D.q)[so] = (T)ldexp((double)u0n / S.wmax, e);
-                            gout<T>(D.q)[D.q_cstride + so] = (T)ldexp((double)u1n / S.wmax, e);
+                            st32<T>(qo0, so, (T)ldexp((double)u0n / S.wmax, e));
+                            st32<T>(qo1, so, (T)ldexp((double)u1n / S.wmax, e));
                         } else {
-                            gout<T>(D.q)[so] = (T)(w0d * ((double)uf0 - (double)u0n));
-                            gout<T>(D.q)[D.q_cstride + so] = (T)(w1d * (ul - (double)u1n));
+                            st32<T>(qo0, so, (T)(w0d * ((double)uf0 - (double)u0n)));
+                            st32<T>(qo1, so, (T)(w1d * (ul - (double)u1n)));
                         }
                     }
                 }
                 if (st && cnt > 0) {  // the iterate (also of an output launch: the kept dual)
-                    Uo[so] = u0n;
-                    Uo[N + so] = u1n;
-                    Uo[2 * N + so] = v0n;
-                    Uo[3 * N + so] = v1n;
+                    st32<T>(Uo, so, u0n);
+                    st32<T>(Uo1, so, u1n);
+                    st32<T>(Uo2, so, v0n);
+                    st32<T>(Uo3, so, v1n);
                 }
                 uf0 = u0n;
             }

@@ -16,6 +16,13 @@ struct StridedShape {
     int64_t stride[kMaxStrided];
 };
 
+// The streaming 2D kernel addresses each plane of an image through 32-bit
+// byte offsets: it takes images of at most kStreamMaxPlaneBytes per plane
+// (2^30 pixels in float32, 2^29 in float64); larger ones take the generic
+// kernel.
+constexpr int64_t kStreamMaxPlaneBytes = int64_t(1) << 32;
+inline bool stream_fits(int tsize, int64_t maxN) { return maxN * (int64_t)tsize <= kStreamMaxPlaneBytes; }
+
 // One image of a grouped gather: n entries of extents shape (3 axes) and
 // element strides stride from src into the contiguous dst.
 struct GatherEntry {
