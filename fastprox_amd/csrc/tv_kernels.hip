@@ -1265,10 +1265,15 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
 #pragma unroll
         for (int m = 1; m <= TT; ++m) {
             const int r = ys - m;  // the row level m produces
+            // Level m's rows above ystart + m feed nothing: the final level's
+            // rows need level m on rows from y0 - (TT - m) on, reached at
+            // ys = ystart + 2m; until then the level is skipped (its values
+            // are never read by a level, a partial sum or a store).
+            const bool need = ys >= ystart + 2 * m;
             const T zc = oz[m - 1];
             const T zr = from_right(zc);
             T v0n, v1n, u0n, u1n;
-            if (m <= cnt) {
+            if (m <= cnt && need) {
                 // An inactive axis has w = 0 and v = 0 (cold start, or zeroed by the
                 // warm start), so its c is 0 as the reference's select makes it.
                 const T g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0 * (pz - zc) : (T)0;
@@ -1308,8 +1313,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             oz[m - 1] = pz;
             ozu[m - 1] = pzu;
             if (m < TT) {
-                if (m < cnt) pz = madd(w1, v1n - from_left(v1n), madd(w0, v0n - ov0[m], dw[m]));
-                if ((gm >> m) & 1u) {
+                if (m < cnt && need) pz = madd(w1, v1n - from_left(v1n), madd(w0, v0n - ov0[m], dw[m]));
+                if (((gm >> m) & 1u) && need) {
                     // The gap of level m on row r - 1.
                     pzu = ((double)dw[m] + w0d * ((double)u0n - (double)ou0[m])) +
                           w1d * ((double)u1n - from_left((double)u1n));
@@ -1374,7 +1379,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
 template <class T, bool ISO, bool SW, int TT>
 constexpr int stream2d_waves() {
     if (sizeof(T) != 4) return 1;
-    return (TT <= 4 || (TT == 6 && !(ISO && SW))) ? 4 : 3;
+    if (!SW) return TT <= 6 ? 4 : 3;
+    return TT <= (ISO ? 2 : 4) ? 4 : 3;
 }
 
 template <class T, bool ISO, bool SW, int TT>
