@@ -93,12 +93,17 @@ struct RBlock {
     int img, c0, c1, pad_;
 };
 
+// Pixel indices of the structure are below B N < 2^31 (checked when it is
+// built), so coordinates and image indices come from 32-bit unsigned
+// division, a short sequence on the GPU where 64-bit division is a long one.
 __device__ __forceinline__ void coords(const Geo &G, int64_t p, int64_t c[3]) {
-    c[2] = p % G.L[2];
-    const int64_t t = p / G.L[2];
-    c[1] = t % G.L[1];
-    c[0] = t / G.L[1];
+    const uint32_t q = (uint32_t)p, L2 = (uint32_t)G.L[2], L1 = (uint32_t)G.L[1];
+    const uint32_t t = q / L2, u = t / L1;
+    c[2] = q - t * L2;
+    c[1] = t - u * L1;
+    c[0] = u;
 }
+__device__ __forceinline__ int img_of(const Geo &G, int64_t gp) { return (int)((uint32_t)gp / (uint32_t)G.N); }
 
 template <class T>
 __device__ __forceinline__ double ld(const T *a, int64_t i) {
@@ -209,7 +214,7 @@ __global__ __launch_bounds__(kThr) void flat_iso_kernel(Geo G, const T *x, const
                                                         double *gz, uint8_t *flat, int *parent) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
     const T *zsrc = D.nan ? x : z;
@@ -250,7 +255,7 @@ __global__ __launch_bounds__(kThr) void flat_aniso_kernel(Geo G, const T *x, con
                                                           int *parent) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
     int64_t c[3];
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(kThr) void union_kernel(Geo G, int iso, const DImg 
     const uint8_t f = flat[gp];
     if (iso && f != 3) return;  // flat and with a neighbour
     if (!iso && f == 0) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
     int64_t c[3];
@@ -366,13 +371,30 @@ __global__ __launch_bounds__(kThr) void inv_kernel(int nc, const int *rstart, do
 // out[C] = the sum of f(p, C) over C's members for every region inside one
 // block of kChunk member positions; head[j] / tail[j]: the parts of the
 // regions that cross into block j from before / out of it after.
+// The block's keys and member values are first loaded into LDS with one
+// member per thread per pass (coalesced), then each thread walks kPerThr
+// consecutive positions from LDS in order; rows of kPerThr + 1 entries keep
+// the walk's LDS accesses on distinct banks. The sums are the same, term for
+// term and in the same order, as a walk over global memory.
 template <class F>
 __global__ __launch_bounds__(kThr) void rsum_kernel(F f, const int *skey, const int *members, const int *rstart,
                                                     int64_t M, double *out, double *head, double *tail) {
+    constexpr int kRow = kPerThr + 1;
+    __shared__ double lv[kThr * kRow];
+    __shared__ int lk[kThr * kRow];
     __shared__ double sv[kThr];
     __shared__ int sf[kThr];
     const int64_t base = (int64_t)blockIdx.x * kChunk;
     const int t = threadIdx.x;
+    const int64_t cend = base + kChunk < M ? base + kChunk : M;
+    const int n = (int)(cend - base);
+    for (int i = t; i < n; i += kThr) {
+        const int C = skey[base + i];
+        const int j = (i / kPerThr) * kRow + i % kPerThr;
+        lk[j] = C;
+        lv[j] = f(members[base + i], C);
+    }
+    __syncthreads();
     const int64_t m0 = base + (int64_t)t * kPerThr;
     const int64_t m1 = m0 + kPerThr < M ? m0 + kPerThr : M;
     double open = 0.0;
@@ -380,13 +402,16 @@ __global__ __launch_bounds__(kThr) void rsum_kernel(F f, const int *skey, const 
     int has_head = 0, head_key = -1, last_key = -1, last_after = 0;
     double head_sum = 0.0;
     if (m0 < m1) {
-        const int kprev = m0 > 0 ? skey[m0 - 1] : -1;
-        const int knext = m1 < M ? skey[m1] : -1;
-        int cur = skey[m0];
+        const int *rk = lk + t * kRow;
+        const double *rv = lv + t * kRow;
+        const int kprev = m0 > 0 ? (t > 0 ? lk[(t - 1) * kRow + kPerThr - 1] : skey[m0 - 1]) : -1;
+        const int knext = m1 < M ? (m1 < cend ? lk[(t + 1) * kRow] : skey[m1]) : -1;
+        const int cnt = (int)(m1 - m0);
+        int cur = rk[0];
         double s = 0.0;
         bool first = true;
-        for (int64_t m = m0; m < m1; ++m) {
-            const int C = skey[m];
+        for (int i = 0; i < cnt; ++i) {
+            const int C = rk[i];
             if (C != cur) {
                 if (first && cur == kprev) {
                     has_head = 1;
@@ -399,7 +424,7 @@ __global__ __launch_bounds__(kThr) void rsum_kernel(F f, const int *skey, const 
                 cur = C;
                 s = 0.0;
             }
-            s += f(members[m], C);
+            s += rv[i];
         }
         const bool before = first && cur == kprev;
         const bool after = cur == knext;
@@ -439,7 +464,6 @@ __global__ __launch_bounds__(kThr) void rsum_kernel(F f, const int *skey, const 
         if (rstart[head_key] >= base) out[head_key] = tot;
         else head[blockIdx.x] = tot;
     }
-    const int64_t cend = base + kChunk < M ? base + kChunk : M;
     const int tl = (int)((cend - base - 1) / kPerThr);
     if (t == tl && last_after) {
         if (rstart[last_key] >= base) tail[blockIdx.x] = sv[t];
@@ -476,7 +500,7 @@ struct FApply {
     const DImg *di;
     const CgState *cg;
     __device__ double operator()(int gp, int C) const {
-        const int b = (int)(gp / G.N);
+        const int b = img_of(G, gp);
         if (!cg[b].active) return 0.0;
         const DImg &D = di[b];
         int64_t c[3];
@@ -525,7 +549,7 @@ __global__ __launch_bounds__(kThr) void hgv_kernel(Geo G, const DImg *di, const 
                                                    const T *sw, int64_t sw_img, double *h) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     if (!cg[b].active) return;
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
@@ -578,7 +602,7 @@ __global__ __launch_bounds__(kThr) void jacobi_own_kernel(Geo G, const DImg *di,
                                                           int64_t sw_img, double *own) {
     const int64_t gq = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gq >= G.BN) return;
-    const int b = (int)(gq / G.N);
+    const int b = img_of(G, gq);
     const int64_t q = gq - (int64_t)b * G.N;
     const DImg &D = di[b];
     int64_t cq[3];
@@ -814,7 +838,7 @@ __global__ __launch_bounds__(kThr) void jvp_rhs_iso_kernel(Geo G, const DImg *di
                                                            const double *dsw, int64_t dsw_img, double *rhs) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
     int64_t c[3];
@@ -860,7 +884,7 @@ __global__ __launch_bounds__(kThr) void jvp_rhs_aniso_kernel(Geo G, const DImg *
                                                              double *rhs) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
     int64_t c[3];
@@ -1437,7 +1461,7 @@ __global__ __launch_bounds__(kThr) void region_labels_kernel(Geo G, const DImg *
                                                              void *lab, int lab64, int8_t *sign) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const bool nan = di[b].nan != 0;
     const int64_t v = nan ? -1 : (int64_t)(label[gp] - rbase[b]);
     if (lab) {
@@ -1460,7 +1484,7 @@ __global__ __launch_bounds__(kThr) void kinks_iso_kernel(Geo G, const DImg *di, 
                                                          int8_t *mask) {
     const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
     if (gp >= G.BN) return;
-    const int b = (int)(gp / G.N);
+    const int b = img_of(G, gp);
     const int64_t p = gp - (int64_t)b * G.N;
     const DImg &D = di[b];
     const double gamma = ldexp(D.gamma_s, D.e);
