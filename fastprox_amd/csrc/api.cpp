@@ -1748,7 +1748,9 @@ FP_API fp_status fpr_batch_set_tail(fpr_batch *b, int tt, int ry, double frac) {
         kc.stream = true;
         kc.TT = tt;
         kc.RY = ry;
-        if (frac > 0.0 && (!b->kc.stream || b->kc.generic || !fpr::kernel_available((int)b->tsize, true, tt, ry)))
+        // the tail runs at least the main kernel's iterations per launch
+        if (frac > 0.0 && (!b->kc.stream || b->kc.generic || !fpr::kernel_available((int)b->tsize, true, tt, ry) ||
+                           kc.iters() < b->kc.iters()))
             return FP_EINVAL;
         DevGuard dg(b->device);
         return hip_status(b->eng->set_tail(kc, frac));
@@ -1762,7 +1764,8 @@ FP_API fp_status fpr_workspace_set_tail(fp_workspace *ws, int tt, int ry, double
         kc.stream = true;
         kc.TT = tt;
         kc.RY = ry;
-        if (frac > 0.0 && (!ws->kc.stream || ws->kc.generic || !fpr::kernel_available((int)ws->tsize, true, tt, ry)))
+        if (frac > 0.0 && (!ws->kc.stream || ws->kc.generic || !fpr::kernel_available((int)ws->tsize, true, tt, ry) ||
+                           kc.iters() < ws->kc.iters()))
             return FP_EINVAL;
         DevGuard dg(ws->device);
         return hip_status(ws->eng->set_tail(kc, frac));

@@ -73,3 +73,20 @@ def test_busy_time_counts_the_two_streams_once(cuda):
     # 20 images: one stream, the union is the sum
     step_ms, busy, launches = run(20)
     assert busy == pytest.approx(step_ms, rel=1e-3)
+
+
+def test_a_tail_shorter_than_the_main_kernel_is_rejected(cuda):
+    """A tail kernel runs at least the main kernel's iterations per launch
+    (the launch before the switch was planned for them): fewer is FP_EINVAL,
+    and the batch keeps solving with its previous tail."""
+    from fastprox_amd import FastproxError
+    from fastprox_amd import _capi as A
+    bt = Batch(Op(True, 2), torch.float32, [(64, 80)] * 3, kernel=(0, 6, 32))
+    with pytest.raises(FastproxError) as err:
+        bt.set_tail(4, 16, 0.5)
+    assert err.value.status == A.FP_EINVAL
+    bt.set_tail(8, 16, 0.5)
+    x = torch.tensor(rectangles(3, (64, 80), n=4, sigma=0.05, seed=2, dtype=np.float32), device=cuda)
+    zs = [torch.empty_like(v) for v in x]
+    ds = [torch.empty((2, 64, 80), device=cuda) for _ in range(3)]
+    assert bt.solve(list(x), zs, ds, [0.1] * 3, tol=1e-6, max_iter=100000) == A.FP_OK
