@@ -100,6 +100,7 @@ def declare(lib):
         "fp_workspace_regions": (I, [P, A, A, C.POINTER(I64), I64]),
         "fp_workspace_constraint": (I, [P, A, C.POINTER(C.c_int8), I64]),
         "fp_workspace_kinks": (I, [P, D, A, C.POINTER(I64)]),
+        "fp_workspace_set_branch": (I, [P, A, D]),
         "fp_device_available": (I, [I]),
         "fp_op_device_check": (I, [P, I, C.POINTER(I)]),
         "fp_workspace_set_stream": (I, [P, P]),

@@ -238,6 +238,14 @@ class Workspace:
                                         C.byref(c)), "fp_workspace_kinks")
         return c.value
 
+    def set_branch(self, side, tol=0.0):
+        """fp_workspace_set_branch (anisotropic TV): side (int8, x's shape,
+        values -1, 0, +1) picks, at every kink edge within tol, whether fp_jvp
+        and fp_vjp join its sides (-1) or keep them apart (+1); None restores
+        the default rule."""
+        _check(lib().fp_workspace_set_branch(self.h, as_array(side).ref if side is not None else None, float(tol)),
+               "fp_workspace_set_branch")
+
     def get_dual(self, out):
         _check(lib().fp_workspace_get_dual(self.h, as_array(out).ref), "fp_workspace_get_dual")
         return out

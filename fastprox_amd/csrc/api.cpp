@@ -659,6 +659,7 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     ws->steps_on = false;
     ws->info_valid = false;
     ws->deriv.reset();
+    ws->branch_on = false;
     if (x.numel() == 0) {
         ws->dual_valid = true;
         ws->info = fp_info{0.0, 0, 0};
@@ -744,6 +745,7 @@ fp_status refine_impl(const fp_op *op, fp_workspace *ws, double tol, int max_ite
     if (info) *info = fp_info{0.0, 0, 0};
     ws->info_valid = false;
     ws->deriv.reset();
+    ws->branch_on = false;
     if (ws->numel == 0) {
         ws->info = fp_info{0.0, 0, 0};
         ws->info_status = FP_OK;
@@ -975,6 +977,13 @@ fp_status ensure_deriv(const fp_op *op, fp_workspace *ws, hipStream_t s) {
     sp.gamma = ws->gammas.data();
     sp.gap = ws->gaps.data();
     sp.e = ws->exps.data();
+    if (!op->iso && ws->branch_on) {
+        sp.side = ws->branch;
+        sp.side_tol = ws->branch_tol;
+        sp.q = ws->q;  // (k, B, N): the C-form dual
+        sp.q_img = ws->N;
+        sp.q_plane = ws->B * ws->N;
+    }
     fpr::Deriv *d = nullptr;
     if (int r = fpr::deriv_create(sp, s, &d)) return (fp_status)r;
     ws->deriv.reset(d);
@@ -1452,6 +1461,59 @@ fp_status constraint_impl(const fp_workspace *ws, fp_array *ca, int8_t *active, 
     return FP_OK;
 }
 
+// fp_workspace_set_branch (src/api/api.cpp:993-1025): anisotropic TV only;
+// every argument is checked before the workspace changes, so a failed call
+// leaves an earlier override in place.
+fp_status set_branch_impl(fp_workspace *ws, const fp_array *sa, double tol) {
+    if (!ws || !ws->op) return FP_EINVAL;
+    if (!sa) {
+        if (ws->branch_on) {
+            ws->branch_on = false;
+            ws->deriv.reset();
+        }
+        return FP_OK;
+    }
+    if (!(tol >= 0.0) || !std::isfinite(tol)) return FP_EPARAM;
+    if (ws->op->iso) return FP_EINVAL;
+    if (fp_status s = struct_ws(ws)) return s;
+    Arr a;
+    if (fp_status s = to_arr(sa, a, false)) return s;
+    if (a.dtype != FP_I8) return FP_EDTYPE;
+    if (a.ndim != ws->ndim) return FP_ESHAPE;
+    for (int d = 0; d < a.ndim; ++d)
+        if (a.shape[d] != ws->shape[d]) return FP_ESHAPE;
+    if (ws->numel == 0) return FP_OK;
+    DevGuard guard(ws->device);
+    hipStream_t s = (hipStream_t)ws->stream;
+    const size_t n = (size_t)ws->numel;
+    std::vector<char> h(n);
+    if (a.device == FP_DEVICE_CPU) {
+        pack_host(a, h.data());
+    } else {
+        QueryBuf tb(s);
+        const void *src = a.data;
+        if (!a.contiguous()) {
+            if (a.ndim > fpr::kMaxStrided) return FP_ESHAPE;
+            if (fp_status st = tb.alloc(n)) return st;
+            fpr::launch_gather<int8_t>((const int8_t *)a.data, (int8_t *)tb.p, (int64_t)n, a.ndim, strided_of(a), s);
+            src = tb.p;
+        }
+        if (hipMemcpyAsync(h.data(), src, n, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s)) return FP_EDEVICE;
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (h[i] < -1 || h[i] > 1) return FP_EPARAM;
+    if (!ws->branch && hipMalloc((void **)&ws->branch, n)) {
+        ws->branch = nullptr;
+        return FP_ENOMEM;
+    }
+    if (hipMemcpyAsync(ws->branch, h.data(), n, hipMemcpyHostToDevice, s) || hipStreamSynchronize(s))
+        return FP_EDEVICE;
+    ws->branch_on = true;
+    ws->branch_tol = tol;
+    ws->deriv.reset();  // rebuilt with the override on first use
+    return FP_OK;
+}
+
 fp_status kinks_impl(const fp_op *op, fp_workspace *ws, double tol, fp_array *ma, int64_t *count) {
     if (!(tol >= 0.0) || !std::isfinite(tol)) return FP_EPARAM;
     if (fp_status s = struct_ws(ws)) return s;
@@ -1507,6 +1569,10 @@ FP_API fp_status fp_workspace_constraint(const fp_workspace *ws, fp_array *coef,
 
 FP_API fp_status fp_workspace_kinks(const fp_workspace *ws, double tol, fp_array *mask, int64_t *count) {
     return guard([&] { return kinks_impl(ws ? ws->op : nullptr, const_cast<fp_workspace *>(ws), tol, mask, count); });
+}
+
+FP_API fp_status fp_workspace_set_branch(fp_workspace *ws, const fp_array *side, double tol) {
+    return guard([&] { return set_branch_impl(ws, side, tol); });
 }
 
 FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, double *smoothing) {

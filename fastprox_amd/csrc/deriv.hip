@@ -1097,6 +1097,11 @@ namespace {
 
 int64_t spatial_len(const Deriv &d) { return d.sp.iso ? d.G.N : d.G.k * d.G.N; }
 
+// The branch overrides on the flat edges and signs of an aniso structure
+// (defined with the kink kernels below).
+template <class T>
+int apply_branch(Deriv &d, hipStream_t s, const T *z, const T *u, const T *sw);
+
 template <class T>
 int build(Deriv &d, hipStream_t s) {
     const DerivSpec &sp = d.sp;
@@ -1192,6 +1197,8 @@ int build(Deriv &d, hipStream_t s) {
     else
         hipLaunchKernelGGL(flat_aniso_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, x, z, u, sp.u_img, sp.u_plane, sw,
                            sp.sw_img, sp.sw_plane, d.di, d.nsg, d.flat, d.parent);
+    if (!sp.iso && sp.side)
+        if (int r = apply_branch<T>(d, s, z, u, sw)) return r;
     hipLaunchKernelGGL(union_kernel, dim3(gpx), dim3(kThr), 0, s, G, (int)sp.iso, d.di, d.flat, d.parent);
     hipLaunchKernelGGL(compress_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.parent, d.isroot);
     hipLaunchKernelGGL(finish_roots_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.parent);
@@ -1574,6 +1581,51 @@ int kinks_t(Deriv &d, hipStream_t s, const void *qv, int64_t q_img, int64_t q_pl
     unsigned long long hc = 0;
     if (hipMemcpyAsync(&hc, cnt, sizeof hc, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s)) return FP_EDEVICE;
     if (count) *count = (int64_t)hc;
+    return hipGetLastError() ? FP_EDEVICE : FP_OK;
+}
+
+// At the kink edges of a set_branch call (tv_aniso.cpp:1008-1011): side -1
+// joins the edge's two sides, +1 keeps them apart with the sign of its dual.
+template <class T>
+__global__ __launch_bounds__(kThr) void branch_aniso_kernel(Geo G, const int8_t *side, const int8_t *kink,
+                                                            const T *u, int64_t u_img, int64_t u_plane, int8_t *nsg,
+                                                            uint8_t *flat) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int sd = side[gp], km = kink[gp];
+    if (sd == 0 || km == 0) return;
+    const int b = img_of(G, gp);
+    const int64_t p = gp - (int64_t)b * G.N;
+    uint8_t f = flat[gp];
+    for (int a = 0; a < G.k; ++a) {
+        if (!((km >> a) & 1)) continue;
+        if (sd < 0) {
+            f = (uint8_t)(f | (1u << a));
+            nsg[a * G.BN + gp] = 0;
+        } else {
+            f = (uint8_t)(f & ~(1u << a));
+            const double ua = ld(u, b * u_img + a * u_plane + p);
+            nsg[a * G.BN + gp] = (int8_t)(ua > 0.0 ? 1 : (ua < 0.0 ? -1 : 0));
+        }
+    }
+    flat[gp] = f;
+}
+
+template <class T>
+int apply_branch(Deriv &d, hipStream_t s, const T *z, const T *u, const T *sw) {
+    const Geo G = d.G;
+    const DerivSpec &sp = d.sp;
+    int8_t *km = nullptr;
+    if (d.alloc(&km, (size_t)G.BN)) return FP_ENOMEM;
+    const int gpx = grid_of(G.BN);
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3(gpx), dim3(kThr), 0, s, G.BN, km);
+    for (int a = 0; a < G.k; ++a) {
+        const int64_t nf = G.BN / G.L[3 - G.k + a];
+        hipLaunchKernelGGL(kinks_aniso_kernel<T>, dim3(grid_of(nf)), dim3(kThr), 0, s, G, a, d.di, z,
+                           (const T *)sp.q, sp.q_img, sp.q_plane, sw, sp.sw_img, sp.sw_plane, sp.side_tol, km);
+    }
+    hipLaunchKernelGGL(branch_aniso_kernel<T>, dim3(gpx), dim3(kThr), 0, s, G, sp.side, km, u, sp.u_img, sp.u_plane,
+                       d.nsg, d.flat);
     return hipGetLastError() ? FP_EDEVICE : FP_OK;
 }
 

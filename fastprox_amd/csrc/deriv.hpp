@@ -36,6 +36,16 @@ struct DerivSpec {
     const double *gamma = nullptr;  // host, B: the steps as given
     const double *gap = nullptr;    // host, B: the gaps at the input's scale (iso)
     const int *e = nullptr;         // host, B: the solves' scale exponents (iso), or null for 0
+    // Aniso branch overrides (fp_workspace_set_branch), or side null: at an
+    // edge (p, p + e_a) at a kink within side_tol (its jump and its 1D dual
+    // within side_tol of 0 and of the bound), side[p] = -1 joins its two
+    // sides and +1 keeps them apart with the sign of its dual; 0 keeps the
+    // default rule. side: device int8, B contiguous images. q: the C-form
+    // dual (k planes) the kink test sums along each fiber.
+    const int8_t *side = nullptr;
+    double side_tol = 0.0;
+    const void *q = nullptr;
+    int64_t q_img = 0, q_plane = 0;
 };
 
 // Builds the structure (one host synchronisation); an fp_status.

@@ -2132,6 +2132,7 @@ FPR_INST(double)
 #undef FPR_INST
 // Integer outputs of the structure queries (labels, signs, states, masks).
 template void launch_scatter<int8_t>(const int8_t *, int8_t *, int64_t, int, const StridedShape &, hipStream_t);
+template void launch_gather<int8_t>(const int8_t *, int8_t *, int64_t, int, const StridedShape &, hipStream_t);
 template void launch_scatter<int32_t>(const int32_t *, int32_t *, int64_t, int, const StridedShape &, hipStream_t);
 template void launch_scatter<int64_t>(const int64_t *, int64_t *, int64_t, int, const StridedShape &, hipStream_t);
 template void launch_convert<float, double>(const float *, double *, int64_t, hipStream_t);

@@ -108,8 +108,13 @@ struct fp_workspace {
     std::unique_ptr<fpr::Deriv, fpr::DerivDeleter> deriv;
     double *dscr = nullptr;  // device doubles for the derivatives' small arrays
     size_t dscr_n = 0;
+    // fp_workspace_set_branch (aniso): the side array (device int8, numel,
+    // C order) and its kink tolerance, until the next prox call
+    int8_t *branch = nullptr;
+    bool branch_on = false;
+    double branch_tol = 0.0;
     ~fp_workspace() {
-        for (void *p : {dstage, zstage, q, xsave, zsave, usave, (void *)dscr})
+        for (void *p : {dstage, zstage, q, xsave, zsave, usave, (void *)dscr, (void *)branch})
             if (p) (void)hipFree(p);
     }
 };

@@ -183,6 +183,14 @@ FP_API fp_status fp_workspace_regions(const fp_workspace *ws, fp_array *labels, 
 FP_API fp_status fp_workspace_constraint(const fp_workspace *ws, fp_array *coef, int8_t *active, int64_t count);
 /* include/fastprox.h:510-511 */
 FP_API fp_status fp_workspace_kinks(const fp_workspace *ws, double tol, fp_array *mask, int64_t *count);
+/* include/fastprox.h:485-512. Anisotropic TV: for the edges at a kink within
+ * tol (the kinks of fp_workspace_kinks), side[p] = -1 joins the two sides of
+ * the edge (p, p + e_a) and +1 keeps them apart, for the fp_jvp and fp_vjp
+ * calls until the next prox call or side = NULL; 0 keeps the default rule.
+ * side: FP_I8, x's shape, any strides, host or device, values -1..1
+ * (FP_EPARAM otherwise). FP_EINVAL for isotropic TV and without a kept point;
+ * a failed call leaves an earlier override in place. */
+FP_API fp_status fp_workspace_set_branch(fp_workspace *ws, const fp_array *side, double tol);
 
 /* include/fastprox.h:782-789 */
 FP_API const char *fp_strerror(fp_status s);

@@ -113,6 +113,14 @@ class Op:
                         A.numpy_array(out).ref), "jvp")
         return out
 
+    def set_branch(self, ws, side, tol):
+        """fp_workspace_set_branch (None: the default rule); the status."""
+        L = lib()
+        if side is None:
+            return L.fp_workspace_set_branch(ws, None, float(tol))
+        sd = np.ascontiguousarray(side, np.int8)
+        return L.fp_workspace_set_branch(ws, A.numpy_array(sd).ref, float(tol))
+
     @staticmethod
     def free_ws(ws):
         lib().fp_workspace_free(ws)
