@@ -96,7 +96,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (images are independent; "
-              f"the multi-GPU form is data-parallel over images, see DESIGN.md)", file=sys.stderr)
+              f"the multi-GPU form is data-parallel over images, one engine or server per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
