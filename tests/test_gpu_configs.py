@@ -78,3 +78,32 @@ def test_c1_images_within_the_certificate(cuda):
         g = aniso_certificate(x[b], z[b], q[:, b], 0.05, w)
         assert g == pytest.approx(gaps[b], rel=1e-6, abs=1e-12)
         assert ((z[b] - z_ref[b]) ** 2).sum() <= 2 * (g + gaps_ref[b]) * 1.0001 + 1e-20
+
+
+def test_c2_images_bitwise_against_the_reference(cuda):
+    """The uneven configuration's first four images (H, W ~ U[64, 1024], 16
+    rectangles + N(0, 0.1^2), pixel weights lognormal(0, 0.3), gamma
+    log-uniform [0.02, 0.3], tol 1e-6; the generator of the bench's uneven
+    leg), solved in float64 as one uneven batch: each image has the
+    reference library's bits, solved alone with its weighted operator."""
+    from fastprox_amd.torch import Batch
+    n = 4
+    rng = np.random.default_rng(0)
+    shapes = [(int(rng.integers(64, 1025)), int(rng.integers(64, 1025))) for _ in range(256)][:n]
+    gam = [float(np.exp(rng.uniform(np.log(0.02), np.log(0.3)))) for _ in range(256)][:n]
+    xs = [rectangles(1, sh, n=16, sigma=0.1, seed=1000 + i)[0].astype(np.float64) for i, sh in enumerate(shapes)]
+    al = [rng.lognormal(0.0, 0.3, size=sh).astype(np.float32).astype(np.float64) for sh in shapes]
+    print("C2 float64: shapes", shapes, flush=True)
+    bt = Batch(Op(True, 2), torch.float64, shapes, weighted=True)
+    xt = [torch.tensor(x, device=cuda) for x in xs]
+    zt = [torch.empty_like(x) for x in xt]
+    dt = [torch.empty((2,) + x.shape, dtype=torch.float64, device=cuda) for x in xt]
+    wt = [torch.tensor(a, device=cuda) for a in al]
+    assert bt.solve(xt, zt, dt, gam, weights=wt, tol=1e-6, max_iter=100000) == A.FP_OK
+    gaps, its, zerrs, conv = bt.image_info()
+    for i in range(n):
+        z_ref, dual_ref, _, its_ref, _, _, _ = refbind.Op(True, 2, spatial=al[i]).solve(
+            xs[i], gam[i], tol=1e-6, max_iter=100000, threads=16)
+        assert int(its[i]) == int(its_ref[0]), i
+        np.testing.assert_array_equal(zt[i].cpu().numpy(), z_ref)
+        np.testing.assert_array_equal(dt[i].cpu().numpy(), dual_ref)
