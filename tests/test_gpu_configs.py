@@ -107,3 +107,25 @@ def test_c2_images_bitwise_against_the_reference(cuda):
         assert int(its[i]) == int(its_ref[0]), i
         np.testing.assert_array_equal(zt[i].cpu().numpy(), z_ref)
         np.testing.assert_array_equal(dt[i].cpu().numpy(), dual_ref)
+
+
+@pytest.mark.parametrize("gamma,tol", [(0.05, 1e-6), (0.2, 1e-5)])
+def test_volumes_bitwise_against_the_reference(cuda, gamma, tol):
+    """Two 64^3 volumes of 16 boxes + N(0, 0.1^2) (the serving trace's 3D
+    kind), float64, on the 3D streaming kernel: the reference library's bits
+    (z, dual, iterations)."""
+    x = rectangles(2, (64, 64, 64), n=16, sigma=0.1, seed=7, dtype=np.float64)
+    print(f"3D float64: 2 x 64^3, gamma {gamma}, tol {tol}", flush=True)
+    xt = torch.tensor(x, device=cuda)
+    out = torch.empty_like(xt)
+    ws = Workspace(Op(True, 3), xt)
+    assert ws.prox(xt, gamma, out, tol=tol, max_iter=100000) == A.FP_OK
+    gaps, its, _ = ws.image_info(2)
+    dual = torch.empty((3,) + tuple(xt.shape), dtype=torch.float64, device=cuda)
+    ws.get_dual(dual)
+    z_ref, dual_ref, gaps_ref, its_ref, _, _, _ = refbind.Op(True, 3).solve(x, gamma, tol=tol, max_iter=100000,
+                                                                            threads=16)
+    np.testing.assert_array_equal(its, its_ref)
+    np.testing.assert_array_equal(out.cpu().numpy(), z_ref)
+    np.testing.assert_array_equal(dual.cpu().numpy(), dual_ref)
+    np.testing.assert_allclose(gaps, gaps_ref, rtol=1e-9)
