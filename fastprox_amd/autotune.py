@@ -5,7 +5,7 @@ size, and writes the tuning table that libfastprox_rocm reads at start-up
 
 For every class and size a batch of synthetic images (rectangles plus noise,
 about 16 Mpx in all: at 512^2 the 64 images of the headline batch, whose
-8,192 tiles run as two halves on two streams) is solved to a gap tolerance of 1e-6 with each candidate:
+tiles run as two halves on two streams) is solved to a gap tolerance of 1e-6 with each candidate:
 the 2D streaming kernel with TT iterations per launch and segments of SEG rows,
 with or without a tail kernel; the 3D streaming kernel with slabs of SLICES
 slices. The fastest candidate (wall time of the whole solve) is kept. Results

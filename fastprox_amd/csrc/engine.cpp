@@ -121,7 +121,12 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
                    kc.tiles_z(s.L[0]);
             tot += t[b];
         }
-        if (split_parts_ > 0 || tot >= kSplitTiles) {
+        static const int64_t split_tiles = [] {
+            const char *e = std::getenv("FASTPROX_ROCM_SPLIT_TILES");
+            const long long v = e ? std::atoll(e) : 0;
+            return v > 0 ? (int64_t)v : (int64_t)kSplitTiles;
+        }();
+        if (split_parts_ > 0 || tot >= split_tiles) {
             int64_t acc = 0;
             keep = 0;
             while (keep + 1 < all.size() && parts * (acc + t[keep]) <= tot) acc += t[keep++];

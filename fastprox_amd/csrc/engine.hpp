@@ -116,8 +116,9 @@ public:
     // worker thread: while one part waits for its epilogue, a launch or its
     // host sync, another's step kernel fills the GPU. Every image's result is
     // the same either way. FASTPROX_ROCM_SPLIT=n asks for n parts (at most
-    // kMaxParts; 0 or 1 turns the split off).
-    static constexpr int kSplitTiles = 8192;
+    // kMaxParts; 0 or 1 turns the split off), FASTPROX_ROCM_SPLIT_TILES
+    // another threshold.
+    static constexpr int kSplitTiles = 2048;
     static constexpr int kMaxParts = 4;
     bool allow_split = true;
 

@@ -408,7 +408,7 @@ def test_3d_streaming_kernel_matches_generic(cuda, dtype, iso):
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_split_batch_matches_per_image_solves(cuda, dtype):
-    """A batch of more than 8192 tiles runs as two halves on two streams
+    """A batch of more than 2048 tiles runs as two halves on two streams
     (Engine::kSplitTiles): every image has the bits of its own plain solve on
     the same kernel, through a tail switch, and the kept duals and results
     are in image order."""
