@@ -187,6 +187,11 @@ def main():
         plain = P / (time.perf_counter() - tp0)
         log(f"plain per-call: {plain:.1f} images/s over {P} images")
 
+    torch_pc = None
+    if args.plain > 0:
+        torch_pc = torch_per_call(xs[:min(4, B)], args.gamma, args.tol, args.max_iter, zs)
+        log(f"plain per-call with torch ops: {torch_pc}")
+
     legs = [v for v in args.legs.split(",") if v]
     extra = {}
     if "b256" in legs and S == 512:
@@ -230,6 +235,8 @@ def main():
                    "input_digest": digest(x)},
         "plain_images_per_s": plain,
         "speedup_vs_plain": (value / plain) if plain else None,
+        "plain_torch_per_call": torch_pc,
+        "speedup_vs_torch_per_call": (value / torch_pc["images_per_s"]) if torch_pc else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "hbm_gbs_measured": hbm_gbs,
@@ -260,6 +267,71 @@ def _sync_time(fn, steps):
         r = fn()
     torch.cuda.synchronize()
     return (time.perf_counter() - t0) / steps, r
+
+
+def torch_per_call(xs, gamma, tol, max_iter, zref):
+    """The reference's per-call GPU solve as it runs there: its torch device
+    loop (_tv_torch.py:229-283; operators :93-186) restated with torch ops,
+    isotropic 2D, one image per call, the restart test and the gap (with a
+    host sync) every 4 iterations. A baseline to compare against, not a
+    path of this library; its z is checked against the library's."""
+    import torch
+    n = xs.shape[0]
+
+    def primal(x, v):  # x - G^T v (w = 1)
+        z = x.clone()
+        z[:, :-1, :].add_(v[0][:, :-1, :])
+        z[:, 1:, :].sub_(v[0][:, :-1, :])
+        z[:, :, :-1].add_(v[1][:, :, :-1])
+        z[:, :, 1:].sub_(v[1][:, :, :-1])
+        return z
+
+    def grad(z):
+        g = torch.zeros((2,) + tuple(z.shape), dtype=z.dtype, device=z.device)
+        torch.sub(z[:, 1:, :], z[:, :-1, :], out=g[0][:, :-1, :])
+        torch.sub(z[:, :, 1:], z[:, :, :-1], out=g[1][:, :, :-1])
+        return g
+
+    def solve(x):
+        tau = 1.0 / 8.0
+        u = torch.zeros((2,) + tuple(x.shape), dtype=x.dtype, device=x.device)
+        v = u.clone()
+        t = torch.ones(1, dtype=x.dtype, device=x.device)
+        target = tol * float((x.double() ** 2).sum()) / 2
+        it = 0
+        while it < max_iter:
+            y = v + tau * grad(primal(x, v))
+            up = y.div_(torch.hypot(y[0], y[1]).div_(gamma).clamp_(min=1.0))
+            it += 1
+            tn = (1 + torch.sqrt(1 + 4 * t * t)) / 2
+            coef = (t - 1) / tn
+            check = it % 4 == 0 or it == max_iter
+            if check:
+                restart = ((v - up) * (up - u)).sum() > 0
+                coef = torch.where(restart, torch.zeros_like(coef), coef)
+                tn = torch.where(restart, torch.ones_like(tn), tn)
+            v = torch.addcmul(up, coef, up - u)
+            u, t = up, tn
+            if check:
+                z = primal(x, u)
+                g = grad(z)
+                gap = gamma * torch.hypot(g[0], g[1]).sum() - (g * u).sum()
+                if float(gap) <= target:
+                    break
+        return primal(x, u), it
+
+    solve(xs[:1])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, dz = [], 0.0
+    for i in range(n):
+        z, it = solve(xs[i:i + 1])
+        its.append(it)
+        dz = max(dz, float((z[0] - zref[i]).abs().max()))
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    return {"images_per_s": n / sec, "images": n, "iters": its, "max_abs_z_vs_library": dz,
+            "method": "the reference's torch device loop restated with torch ops, one image per call"}
 
 
 def leg_b256(args, plain, B=256, S=512):
