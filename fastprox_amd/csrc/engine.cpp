@@ -312,12 +312,13 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
             A.list_in = list[p];
             A.count_in = counts + p;
             A.count_out = counts + (p ^ 1);
+            // With timing, the step kernel's start and end and the epilogue's
+            // end are stamped by the launches themselves.
             const size_t eb = (size_t)(3 * (slot * steps_per_sync + i));
-            if (timing) (void)hipEventRecord(ev[eb], st);
-            launch_step<T>(cur_kc, P.iso != 0, P.sw != 0, A, grid, st);
-            if (timing) (void)hipEventRecord(ev[eb + 1], st);
-            launch_epilogue(cur_desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, st);
-            if (timing) (void)hipEventRecord(ev[eb + 2], st);
+            launch_step<T>(cur_kc, P.iso != 0, P.sw != 0, A, grid, st, timing ? ev[eb] : nullptr,
+                           timing ? ev[eb + 1] : nullptr);
+            launch_epilogue(cur_desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, st,
+                            timing ? ev[eb + 2] : nullptr);
             p ^= 1;
         }
         return hipMemcpyAsync(hcount + slot, counts + p, sizeof(int), hipMemcpyDeviceToHost, st);

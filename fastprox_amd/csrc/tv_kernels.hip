@@ -30,6 +30,7 @@
 // their summation order.
 //
 // SPDX-License-Identifier: MIT
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -2006,28 +2007,41 @@ void launch_tv1d(const Problem &P, const ImgDesc *desc, ImgState *state, int nim
                        eta);
 }
 
-void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out, int *count_out,
-                     int TT, double u_round, double eta, int nimg, hipStream_t s) {
-    hipLaunchKernelGGL(epilogue_kernel, dim3(nimg), dim3(64), 0, s, desc, state, part, list_out, count_out, TT,
-                       u_round, eta, nimg);
+// A launch that stamps t0 when the kernel starts and t1 when it completes
+// (either may be null) through its own dispatch, with no separate event
+// record in the stream; without events, a plain launch (the form graph
+// capture records).
+template <class F, class... Args>
+static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t t0, hipEvent_t t1, Args... args) {
+    if (t0 || t1) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, t0, t1, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
+// t1 (optional): an event stamped when the epilogue completes.
+void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out, int *count_out,
+                     int TT, double u_round, double eta, int nimg, hipStream_t s, hipEvent_t t1) {
+    launch_timed(epilogue_kernel, dim3(nimg), dim3(64), s, nullptr, t1, desc, state, part, list_out, count_out, TT,
+                 u_round, eta, nimg);
+}
+
+// t0, t1 (optional): events stamped when the step kernel starts and when it
+// completes, by its own dispatch.
 template <class T, bool ISO, bool SW>
-static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid, hipStream_t s) {
+static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid, hipStream_t s, hipEvent_t t0,
+                          hipEvent_t t1) {
     if (kc.s3) {
-        hipLaunchKernelGGL((stream3d_kernel<T, ISO, SW, kS3Waves>), dim3(grid), dim3(64 * kS3Waves), 0, s, A,
-                           kc.RY);
+        launch_timed(stream3d_kernel<T, ISO, SW, kS3Waves>, dim3(grid), dim3(64 * kS3Waves), s, t0, t1, A, kc.RY);
         return;
     }
     if (kc.generic) {
-        hipLaunchKernelGGL((step1_kernel<T, ISO, SW>), dim3(grid), dim3(256), 0, s, A);
+        launch_timed(step1_kernel<T, ISO, SW>, dim3(grid), dim3(256), s, t0, t1, A);
         return;
     }
     if (kc.stream) {
-#define FPR_SCASE(TTv)                                                                                   \
-    if (kc.TT == TTv) {                                                                                  \
-        hipLaunchKernelGGL((stream2d_kernel<T, ISO, SW, TTv>), dim3(grid), dim3(256), 0, s, A, kc.RY);   \
-        return;                                                                                          \
+#define FPR_SCASE(TTv)                                                                                          \
+    if (kc.TT == TTv) {                                                                                         \
+        launch_timed(stream2d_kernel<T, ISO, SW, TTv>, dim3(grid), dim3(256), s, t0, t1, A, kc.RY);            \
+        return;                                                                                                 \
     }
         FPR_SCASE(2)
         FPR_SCASE(4)
@@ -2038,7 +2052,7 @@ static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid
     }
 #define FPR_CASE(TTv, RYv)                                                                              \
     if (kc.TT == TTv && kc.RY == RYv) {                                                                 \
-        hipLaunchKernelGGL((step2d_kernel<T, ISO, SW, TTv, RYv>), dim3(grid), dim3(256), 0, s, A);      \
+        launch_timed(step2d_kernel<T, ISO, SW, TTv, RYv>, dim3(grid), dim3(256), s, t0, t1, A);          \
         return;                                                                                         \
     }
     if constexpr (sizeof(T) == 4) {
@@ -2051,13 +2065,14 @@ static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid
 }
 
 template <class T>
-void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s) {
+void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s,
+                 hipEvent_t t0, hipEvent_t t1) {
     if (iso) {
-        if (sw) step_dispatch<T, true, true>(kc, A, grid, s);
-        else step_dispatch<T, true, false>(kc, A, grid, s);
+        if (sw) step_dispatch<T, true, true>(kc, A, grid, s, t0, t1);
+        else step_dispatch<T, true, false>(kc, A, grid, s, t0, t1);
     } else {
-        if (sw) step_dispatch<T, false, true>(kc, A, grid, s);
-        else step_dispatch<T, false, false>(kc, A, grid, s);
+        if (sw) step_dispatch<T, false, true>(kc, A, grid, s, t0, t1);
+        else step_dispatch<T, false, false>(kc, A, grid, s, t0, t1);
     }
 }
 
@@ -2122,7 +2137,8 @@ void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
     template void launch_warm<T>(const Problem &, T *, const ImgDesc *, const ImgState *, int, int64_t, hipStream_t);                                                         \
     template void launch_tv1d<T>(const Problem &, const ImgDesc *, ImgState *, int, int64_t, double, double,      \
                                  hipStream_t);                                                  \
-    template void launch_step<T>(const KernelChoice &, bool, bool, const StepArgs<T> &, int, hipStream_t);       \
+    template void launch_step<T>(const KernelChoice &, bool, bool, const StepArgs<T> &, int, hipStream_t,        \
+                                 hipEvent_t, hipEvent_t);                                                        \
     template void launch_gather<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);            \
     template void launch_scatter<T>(const T *, T *, int64_t, int, const StridedShape &, hipStream_t);            \
     template void launch_copy_u<T>(const T *, const ImgDesc *, const ImgState *, int, int, int64_t, T *, int64_t,  \

@@ -76,10 +76,13 @@ void launch_warm(const Problem &P, T *arena, const ImgDesc *desc, const ImgState
 template <class T>
 void launch_tv1d(const Problem &P, const ImgDesc *desc, ImgState *state, int nimg, int64_t maxN, double u_round,
                  double eta, hipStream_t s);
+// t0 / t1 (optional): events the launch stamps when the kernel starts /
+// completes (hipExtLaunchKernel), for timing without separate records.
 template <class T>
-void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s);
+void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A, int grid, hipStream_t s,
+                 hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 void launch_epilogue(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out, int *count_out,
-                     int TT, double u_round, double eta, int nimg, hipStream_t s);
+                     int TT, double u_round, double eta, int nimg, hipStream_t s, hipEvent_t t1 = nullptr);
 template <class T>
 void launch_gather(const T *src, T *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s);
 template <class T>
