@@ -158,11 +158,11 @@ bool SlotPool::release_dual(int64_t id) {
     return true;
 }
 
-bool SlotPool::evict_one_dual() {
-    if (kept_.empty()) return false;
-    auto old = kept_.begin();
+bool SlotPool::evict_one_dual(int64_t keep) {
+    auto old = kept_.end();
     for (auto it = kept_.begin(); it != kept_.end(); ++it)
-        if (it->second.stamp < old->second.stamp) old = it;
+        if (it->first != keep && (old == kept_.end() || it->second.stamp < old->second.stamp)) old = it;
+    if (old == kept_.end()) return false;
     mem_.free(old->second.off, old->second.n);
     kept_.erase(old);
     return true;
@@ -509,12 +509,18 @@ Server::~Server() = default;
 int Server::setup(const Problem *probs, const KernelChoice *kcs, int tsize, int slots, size_t pool_bytes) {
     for (int c = 0; c < 2; ++c) {
         if (probs[c].k == 0) continue;
-        pools_[c].reset(new SlotPool());
-        const int64_t elems = (int64_t)(pool_bytes / (size_t)tsize);
+        const bool two = c == 0 && probs[1].k == 0 && pool_bytes >= kLanePoolBytes && slots >= kLaneSlots;
+        const int nl = two ? 2 : 1;
+        const int64_t elems = (int64_t)(pool_bytes / (size_t)tsize) / nl;
         const int k = probs[c].k;
         const int64_t max_pixels = std::min<int64_t>(elems / (2 + 5 * k), (int64_t)1 << 28);
         const int64_t tile_cap = std::max<int64_t>(elems / 1024, 4096);
-        if (int e = pools_[c]->setup(probs[c], kcs[c], tsize, slots, elems, tile_cap, max_pixels)) return e;
+        for (int i = 0; i < nl; ++i) {
+            std::unique_ptr<SlotPool> &P = i == 0 ? pools_[c] : pools_[2];
+            P.reset(new SlotPool());
+            if (int e = P->setup(probs[c], kcs[c], tsize, (slots + nl - 1) / nl, elems, tile_cap, max_pixels))
+                return e;
+        }
     }
     return 0;
 }
@@ -543,25 +549,48 @@ int Server::submit(ServeRequest r, int64_t *id) {
 
 int Server::step(int *live, int *queued) {
     for (int c = 0; c < 2; ++c) {
-        SlotPool *P = pools_[c].get();
-        if (!P) continue;
-        // admissions, first come first served; most are set up in groups
+        if (!pools_[c]) continue;
+        const int nl = lanes(c);
+        // admissions, first come first served; most are set up in groups. A
+        // warm start goes to the lane that keeps its dual; others to the first
+        // lane while it holds fewer than kLaneSpill live tiles (a light load
+        // runs on one stream, as without lanes), then to the lane with fewer;
+        // the second lane also takes what the first has no room for.
         while (!queue_[c].empty()) {
             int err = 0;
             const ServeRequest &r = queue_[c].front();
-            const bool ok = P->groupable(r) ? P->stage(r, now(), &err) : P->try_admit(r, now(), &err);
+            SlotPool *order[2] = {lane(c, 0), nl > 1 ? lane(c, 1) : nullptr};
+            if (nl > 1) {
+                const bool swap = r.warm_from >= 0 ? order[1]->has_kept(r.warm_from)
+                                                   : (order[0]->live_tiles() >= kLaneSpill &&
+                                                      order[1]->live_tiles() < order[0]->live_tiles());
+                if (swap) std::swap(order[0], order[1]);
+                if (r.warm_from >= 0 && order[0]->has_kept(r.warm_from)) order[1] = nullptr;
+            }
+            bool ok = false;
+            for (SlotPool *P : order) {
+                if (!P) continue;
+                ok = P->groupable(r) ? P->stage(r, now(), &err) : P->try_admit(r, now(), &err);
+                if (err) return err;
+                if (ok) break;
+            }
             if (ok) {
                 queue_[c].pop_front();
                 continue;
             }
-            if (err) return err;
-            if (P->evict_one_dual()) continue;
+            // room from kept duals, never from the one this request warms from
+            bool evicted = false;
+            for (SlotPool *P : order)
+                if (P && !evicted) evicted = P->evict_one_dual(r.warm_from);
+            if (evicted) continue;
             break;
         }
-        bool enq = false;
-        if (int e = P->enqueue_chunk(&enq)) return e;
+        for (int i = 0; i < nl; ++i) {
+            bool enq = false;
+            if (int e = lane(c, i)->enqueue_chunk(&enq)) return e;
+        }
     }
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 3; ++c) {
         SlotPool *P = pools_[c].get();
         if (!P) continue;
         // keep one chunk in flight while the older one is retired
@@ -569,10 +598,9 @@ int Server::step(int *live, int *queued) {
             if (int e = P->retire(done_, server_clock, this)) return e;
     }
     int l = 0, q = 0;
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 3; ++c)
         if (pools_[c]) l += pools_[c]->live() + pools_[c]->in_flight();
-        q += (int)queue_[c].size();
-    }
+    for (int c = 0; c < 2; ++c) q += (int)queue_[c].size();
     if (live) *live = l;
     if (queued) *queued = q;
     return 0;

@@ -86,10 +86,13 @@ public:
     int retire(std::vector<ServeResult> &out, double (*clock)(const void *), const void *ctx);
     int in_flight() const { return (int)flight_.size(); }
     int live() const { return live_; }
+    int64_t live_tiles() const { return live_tiles_; }
+    bool has_kept(int64_t id) const { return kept_.count(id) != 0; }
     bool fits_ever(const ServeRequest &r) const;
     int64_t footprint(const ServeRequest &r) const;
     bool release_dual(int64_t id);
-    bool evict_one_dual();
+    // Frees the oldest kept dual other than keep's; false if there is none.
+    bool evict_one_dual(int64_t keep = -1);
     const KernelChoice &kernel() const { return kc_; }
     // Launch pairs per chunk: a finished request is seen when its chunk's
     // snapshot is read, so shorter chunks cut latency (4 against 8: served
@@ -186,7 +189,22 @@ public:
     const KernelChoice *kernel(int c) const { return pools_[c] ? &pools_[c]->kernel() : nullptr; }
 
 private:
-    std::unique_ptr<SlotPool> pools_[2];
+    // pools_[0]: 2D, [1]: 3D, [2]: a second 2D lane. A server of 2D requests
+    // alone, with a pool of at least kLanePoolBytes and kLaneSlots slots,
+    // runs them as two lanes, each with half the slots and the memory, on its
+    // own stream: while one lane waits for its epilogue or its chunk's
+    // snapshot, the other's step kernel runs (as the batch engine's two
+    // halves do). Measured on 2048 queued 512^2 requests: 4,368 -> 4,838
+    // images/s. A server of both classes keeps one 2D lane: on the mixed
+    // 2D/3D trace (Poisson 200 requests/s) a second 2D lane raised the 3D
+    // requests' p99 from 59 to 77 ms, whether or not it took work while 3D
+    // requests were live.
+    static constexpr size_t kLanePoolBytes = size_t(1) << 30;
+    static constexpr int kLaneSlots = 32;
+    static constexpr int64_t kLaneSpill = 4096;
+    std::unique_ptr<SlotPool> pools_[3];
+    int lanes(int c) const { return c == 0 && pools_[2] ? 2 : 1; }
+    SlotPool *lane(int c, int i) const { return i == 0 ? pools_[c].get() : pools_[2].get(); }
     std::deque<ServeRequest> queue_[2];
     std::vector<ServeResult> done_;
     int64_t next_id_ = 0;

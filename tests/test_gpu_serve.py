@@ -75,15 +75,21 @@ def test_serve_same_alone_and_in_the_mix(cuda):
     assert admitted_late > 0  # continuous batching: admissions after completions
 
 
-def test_serve_grouped_admission(cuda):
+@pytest.mark.parametrize("mixed", [True, False], ids=["2d+3d", "2d-two-lanes"])
+def test_serve_grouped_admission(cuda, mixed):
     """150 requests queued at once are set up in groups of up to 64 (one
     gather, one prologue, one commit per group): each request keeps its own
     tolerance, iteration cap and strides, and its result has the bits of its
     plain solve. Exact 1D images and warm starts, which take the single path,
-    are interleaved and keep their order."""
+    are interleaved and keep their order. A server of 2D requests alone with
+    a pool of 1 GiB runs two 2D lanes: a warm start goes to the lane that
+    keeps its dual, and the results are the same."""
     src2d, src3d = _sources()
     rng = np.random.default_rng(21)
-    srv = Server(Op(True, 2), Op(True, 3), pool_bytes=256 << 20, slots=200)
+    if mixed:
+        srv = Server(Op(True, 2), Op(True, 3), pool_bytes=256 << 20, slots=200)
+    else:
+        srv = Server(Op(True, 2), None, pool_bytes=1 << 30, slots=200)
     kept = torch.empty((60, 70), device="cuda")
     a = srv.submit(src2d[2][:60, :70], 0.1, tol=1e-5, out=kept, keep_dual=True)
     srv.drain()
@@ -94,7 +100,7 @@ def test_serve_grouped_admission(cuda):
             x = x.reshape(1, -1)
         elif j % 41 == 7:
             x = src2d[2][:60, :70]
-        elif j % 5 == 0:
+        elif j % 5 == 0 and mixed:
             x = src3d[j % 2][:8 + j % 7, 3:14, ::2][:, :, :9]  # strided volume
         else:
             h, w = (int(v) for v in rng.integers(16, 97, size=2))
@@ -110,7 +116,7 @@ def test_serve_grouped_admission(cuda):
     res = srv.drain()
     torch.cuda.synchronize()
     assert len(res) == 150
-    kern = {2: srv.kernel(2), 3: srv.kernel(3)}
+    kern = {2: srv.kernel(2), 3: srv.kernel(3) if mixed else None}
     _, _, _, _, dual_a = _alone(src2d[2][:60, :70], 0.1, 1e-5, kernel=kern)
     for d in res:
         x, gamma, tol, mi, out, warm = cases[d["id"]]
