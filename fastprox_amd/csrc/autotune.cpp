@@ -169,17 +169,17 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
     return kc;
 }
 
-// 2D: the batch kernel of a 512^2 image (autotuned; TT 6, 32 rows on
-// MI355X). Measured on 2048 queued 512^2 requests and on the served trace
-// (Poisson 200 requests/s for 20 s, 80% 2D of side 128..1024, 20% 3D of side
-// 48..128, f32), against 4 iterations and 16 rows:
+// 2D: the batch kernel of 64 images of 512^2 (autotuned; on MI355X TT 6 with
+// 40-row segments). Measured on 2048 queued 512^2 requests and on the served
+// trace (Poisson 200 requests/s for 20 s, 80% 2D of side 128..1024, 20% 3D of
+// side 48..128, f32) when the table chose TT 6 with 32 rows, against 4
+// iterations and 16 rows:
 //   TT 6, 32 rows: 3,958 images/s (4,129 steady), trace p50 0.89 ms, p99 40 ms
 //   TT 4, 32 rows: 3,759 (4,064),                 p50 0.81 ms, p99 40 ms
 //   TT 4, 16 rows: 3,364 (3,624),                 p50 0.67 ms, p99 37 ms
-// Short launches help a lone request, long ones a full server; the batch
-// kernel keeps the served rate at 512^2 above 20x the plain per-call rate
-// (182 images/s). 3D: slabs of 8 slices (the 3D requests set the trace's
-// p99; the batch kernel's 16-slice slabs lengthen it).
+// Short launches help a lone request, long ones a full server. 3D: slabs of
+// 8 slices (the 3D requests set the trace's p99; the batch kernel's 16-slice
+// slabs lengthen it).
 KernelChoice serve_kernel(int k, int tsize, bool iso) {
     const char *env = std::getenv(k == 2 ? "FASTPROX_ROCM_KERNEL" : "FASTPROX_ROCM_KERNEL3D");
     if (env || k == 2) {
