@@ -15,7 +15,7 @@ from fastprox_amd import FastproxError, Op, Workspace
 from fastprox_amd import _capi as A
 from fastprox_amd.synth import disks, rectangles
 from fastprox_amd.torch import Batch
-from helpers import aniso_certificate, golden, image, iso_certificate
+from helpers import aniso_certificate, fdiff, fdiff_t, golden, image, iso_certificate
 
 pytestmark = pytest.mark.gpu
 
@@ -49,14 +49,16 @@ def _run(c, dev, kernel=None, dtype=torch.float64):
     return st, out.cpu().numpy(), dual.cpu().numpy(), gaps, its, zerrs, ws
 
 
-@pytest.mark.parametrize("kernel", KERNELS_2D, ids=lambda k: "default" if k is None else f"{k}")
-@pytest.mark.parametrize("name", ISO)
+# every float64 fixture on the default kernel; the 2D ones also on each tiled
+# kernel (the tiled kernels are 2D; float32 fixtures are checked below)
+BITWISE = [(n, kern) for n in ISO if G[n]["x"].dtype == np.float64
+           for kern in (KERNELS_2D if len(G[n]["w"]) == 2 else [None])]
+
+
+@pytest.mark.parametrize("name,kernel", BITWISE,
+                         ids=[f"{n}-{'default' if kern is None else kern}" for n, kern in BITWISE])
 def test_iso_golden_float64_bitwise(cuda, name, kernel):
     c = G[name]
-    if c["x"].dtype != np.float64:
-        pytest.skip("float64 cases")
-    if kernel is not None and len(c["w"]) != 2:
-        pytest.skip("the tiled kernels are 2D")
     st, z, dual, gaps, its, zerrs, _ = _run(c, cuda, kernel)
     assert st == int(c["status"])
     np.testing.assert_array_equal(its, c["iters"])
@@ -93,6 +95,41 @@ def test_iso_float32_within_1e4_of_the_reference(cuda, kernel):
         assert np.abs(z[b] - r.z).max() <= 1e-4
         assert abs(int(its[b]) - r.iters) <= 8
         assert gaps[b] <= 1e-6 * 0.5 * (x[b].astype(np.float64) ** 2).sum()
+
+
+F32 = [n for n in ISO if G[n]["x"].dtype == np.float32]
+
+
+@pytest.mark.parametrize("kernel", KERNELS_2D, ids=lambda k: "default" if k is None else f"{k}")
+@pytest.mark.parametrize("name", F32)
+def test_iso_golden_float32_within_1e4(cuda, name, kernel):
+    """The reference's float32 fixtures (its solve promotes to float64 and
+    rounds z to float32), solved in float32 on the device: the same status,
+    |z - z_ref|_inf <= 1e-4, a feasible dual whose recomputed gap meets the
+    target, and an iteration count near the reference's."""
+    c = G[name]
+    k = len(c["w"])
+    assert k == 2
+    st, z, dual, gaps, its, _, _ = _run(c, cuda, kernel, dtype=torch.float32)
+    assert st == int(c["status"])
+    np.testing.assert_allclose(z, c["z"], rtol=0, atol=1e-4)
+    xs = c["x"].reshape((-1,) + c["x"].shape[-k:]).astype(np.float64)
+    zs = z.reshape(xs.shape).astype(np.float64)
+    us = np.moveaxis(dual.reshape((k,) + xs.shape), 0, 1).astype(np.float64)
+    gamma, w = float(c["gamma"]), c["w"]
+    for b in range(xs.shape[0]):
+        target = float(c["tol"]) * 0.5 * (xs[b] ** 2).sum()
+        assert gaps[b] <= target
+        # the certificate of the rounded dual, in float64: z(u) = x - D^T W u
+        # and the gap of (z(u), u)
+        u = us[b]
+        assert (np.sqrt((u ** 2).sum(0)) <= gamma * (1 + 1e-6)).all()
+        zu = xs[b] - sum(w[a] * fdiff_t(u[a], a) for a in range(k))
+        assert np.abs(zs[b] - zu).max() <= 1e-5
+        g = np.stack([w[a] * fdiff(zu, a) for a in range(k)])
+        cert = float((gamma * np.sqrt((g ** 2).sum(0)) - (g * u).sum(0)).sum())
+        assert cert <= 2 * target
+        assert abs(int(its[b]) - int(c["iters"][b])) <= 12
 
 
 def _aniso_cases():
