@@ -578,10 +578,16 @@ int Server::step(int *live, int *queued) {
                 queue_[c].pop_front();
                 continue;
             }
-            // room from kept duals, never from the one this request warms from
+            // Room from kept duals other than the one this request warms from;
+            // that one goes too when nothing else would ever free room (its
+            // lane has no request live or in flight): the request then starts
+            // cold instead of waiting for good.
             bool evicted = false;
             for (SlotPool *P : order)
                 if (P && !evicted) evicted = P->evict_one_dual(r.warm_from);
+            if (!evicted && r.warm_from >= 0)
+                for (SlotPool *P : order)
+                    if (P && !evicted && P->live() == 0 && P->in_flight() == 0) evicted = P->evict_one_dual();
             if (evicted) continue;
             break;
         }

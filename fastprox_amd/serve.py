@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _capi as A
-from .core import _check, lib
+from .core import FastproxError, _check, lib
 
 
 class _Request(C.Structure):
@@ -145,13 +145,17 @@ class Server:
         return a.value, b.value, c.value
 
     def drain(self):
-        """Steps until every submitted request is done; their results."""
+        """Steps until every submitted request is done; their results. Raises
+        FastproxError if queued requests remain with nothing live or in flight
+        after a step (no later step could admit them)."""
         out = []
         while True:
             b, q = self.step()
             out += self.poll()
             if b == 0 and q == 0:
                 return out + self.poll()
+            if b == 0:
+                raise FastproxError(A.FP_ENOMEM, f"fpr_server_step: {q} queued request(s) can never be admitted")
 
     def __del__(self):
         if getattr(self, "h", None) and self.h.value:

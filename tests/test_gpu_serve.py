@@ -129,6 +129,26 @@ def test_serve_grouped_admission(cuda, mixed):
     assert sum(1 for d in res if not d["converged"]) >= 5  # stopped by their own cap
 
 
+def test_serve_warm_start_that_cannot_fit_beside_its_dual(cuda):
+    """A pool that holds one 200^2 request but not that request and a kept
+    dual of its shape: a warm start from the kept dual finds nothing else to
+    evict and no request that would free room, so its dual is evicted and it
+    runs cold (the plain cold solve's bits) instead of waiting for good."""
+    src2d, _ = _sources()
+    x = src2d[0][:200, :200]
+    srv = Server(Op(True, 2), None, pool_bytes=48 * 200 * 200 + 4096, slots=4)
+    kern = {2: srv.kernel(2)}
+    a = srv.submit(x, 0.1, tol=1e-5, out=torch.empty((200, 200), device="cuda"), keep_dual=True)
+    assert len(srv.drain()) == 1
+    x2 = src2d[1][:200, :200]
+    out = torch.empty((200, 200), device="cuda")
+    srv.submit(x2, 0.1, tol=1e-5, out=out, warm_from=a)
+    res = srv.drain()
+    assert len(res) == 1 and res[0]["converged"]
+    z, its, _, _, _ = _alone(x2, 0.1, 1e-5, kernel=kern)
+    assert res[0]["iters"] == its and torch.equal(out, z)
+
+
 def test_serve_admission_by_memory(cuda):
     """A pool that holds two 200^2 requests at a time: the rest wait, are
     admitted as memory frees, and all finish with the plain results."""
