@@ -200,6 +200,8 @@ def replay(srv, trace, src2d, src3d, warmup=0.0, progress=None):
             wait = trace[i].t - (srv.clock() - t0)
             if wait > 0:
                 time.sleep(min(wait, 0.01))
+        elif b == 0:
+            raise FastproxError(A.FP_ENOMEM, f"fpr_server_step: {q} queued request(s) can never be admitted")
 
 
 def latency_stats(results):
