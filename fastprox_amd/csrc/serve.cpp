@@ -15,7 +15,8 @@
 //   2. enqueues a chunk of step/epilogue launch pairs and a snapshot of the
 //      slot states, keeping two chunks in flight so the GPU never waits on
 //      the host;
-//   3. waits for the older chunk and retires the requests its snapshot shows
+//   3. waits for a class's older chunk (with both classes served, of the
+//      first class to finish one) and retires the requests its snapshot shows
 //      done: their output is copied out, their memory and tiles return to
 //      the pools, their dual stays resident when asked (a warm start for a
 //      later request of the same shape).
@@ -30,6 +31,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 #include "../../include/fastprox_rocm.h"
 
@@ -483,6 +485,11 @@ int SlotPool::finish_slot(int i, const ImgState &St, std::vector<ServeResult> &o
     return 0;
 }
 
+bool SlotPool::oldest_done() const {
+    // an error counts as done: retire() then reports it
+    return !flight_.empty() && hipEventQuery(flight_.front().ev) != hipErrorNotReady;
+}
+
 int SlotPool::retire(std::vector<ServeResult> &out, double (*clock)(const void *), const void *ctx) {
     if (flight_.empty()) return 0;
     const Flight f = flight_.front();
@@ -596,12 +603,28 @@ int Server::step(int *live, int *queued) {
             if (int e = lane(c, i)->enqueue_chunk(&enq)) return e;
         }
     }
-    for (int c = 0; c < 3; ++c) {
-        SlotPool *P = pools_[c].get();
-        if (!P) continue;
-        // keep one chunk in flight while the older one is retired
-        if (P->in_flight() >= 2 || (P->in_flight() > 0 && P->live() == 0))
+    // Retirement keeps one chunk in flight per pool while the older one is
+    // retired (both, once nothing is live). In a server of both classes a
+    // pool whose older chunk is done is retired at once, and while none is,
+    // every pool due is polled until one is, so the 2D class's long chunks
+    // (TT iterations per launch) do not hold back the 3D class's short ones
+    // (one per launch): on the mixed trace (Poisson 200 requests/s) p99 37.3
+    // -> 34.6 ms, the 3D requests' 55 -> 52 ms. Pools of one class (the two
+    // 2D lanes) have alike chunks and are retired in turn, waiting for each:
+    // polling them cost 2.5% of the served rate on 2048 queued 512^2 requests.
+    const bool poll = pools_[0] && pools_[1];
+    for (;;) {
+        bool due = false, retired = false;
+        for (int c = 0; c < 3; ++c) {
+            SlotPool *P = pools_[c].get();
+            if (!P || !(P->in_flight() >= 2 || (P->in_flight() > 0 && P->live() == 0))) continue;
+            due = true;
+            if (poll && !P->oldest_done()) continue;
             if (int e = P->retire(done_, server_clock, this)) return e;
+            retired = true;
+        }
+        if (!due || retired) break;
+        std::this_thread::yield();
     }
     int l = 0, q = 0;
     for (int c = 0; c < 3; ++c)

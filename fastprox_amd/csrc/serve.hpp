@@ -84,6 +84,8 @@ public:
     // Waits for the oldest chunk in flight and finishes the requests it shows
     // done (results appended to out).
     int retire(std::vector<ServeResult> &out, double (*clock)(const void *), const void *ctx);
+    // Whether the oldest chunk in flight has finished (without waiting).
+    bool oldest_done() const;
     int in_flight() const { return (int)flight_.size(); }
     int live() const { return live_; }
     int64_t live_tiles() const { return live_tiles_; }
