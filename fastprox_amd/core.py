@@ -147,7 +147,7 @@ class Workspace:
     def kernel(self):
         g, t, r = C.c_int(), C.c_int(), C.c_int()
         _check(lib().fpr_workspace_get_kernel(self.h, C.byref(g), C.byref(t), C.byref(r)), "get_kernel")
-        return bool(g.value), t.value, r.value
+        return g.value, t.value, r.value  # kind: 0 stream 2D, 1 generic, 2 block 2D, 3 stream 3D
 
     def stats(self):
         a, b = C.c_int(), C.c_int()

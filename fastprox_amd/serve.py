@@ -166,12 +166,13 @@ class Server:
             self.h = C.c_void_p()
 
 
-def replay(srv, trace, src2d, src3d, warmup=0.0, progress=None):
+def replay(srv, trace, src2d, src3d, warmup=0.0, progress=None, keep_outputs=False):
     """Replays a serving trace (synth.serving_trace) against srv in real time:
     each request is submitted at its arrival time (recorded as its submit
     time, so queueing behind a late host loop counts in its latency).
     src2d / src3d: cuda tensors of source images the requests crop from.
-    Returns the results (dicts, with the request's class, side and pixels)."""
+    Returns the results (dicts, with the request's class, side and pixels;
+    with keep_outputs, also "z": the request's output tensor)."""
     from .synth import crop_of
     import time
     t0 = srv.clock() + warmup
@@ -184,12 +185,15 @@ def replay(srv, trace, src2d, src3d, warmup=0.0, progress=None):
         while i < n and trace[i].t <= now:
             r = trace[i]
             x = crop_of(r, src2d, src3d)
-            rid = srv.submit(x, r.gamma, tol=r.tol, max_iter=100000, t_submit=t0 + r.t)
-            meta[rid] = (r.k, r.side, r.side ** r.k)
+            out = torch.empty(tuple(x.shape), dtype=x.dtype, device=x.device) if keep_outputs else None
+            rid = srv.submit(x, r.gamma, tol=r.tol, max_iter=100000, out=out, t_submit=t0 + r.t)
+            meta[rid] = (r.k, r.side, r.side ** r.k, out)
             i += 1
         b, q = srv.step()
         for d in srv.poll():
-            d["k"], d["side"], d["pixels"] = meta.pop(d["id"])
+            d["k"], d["side"], d["pixels"], out = meta.pop(d["id"])
+            if keep_outputs:
+                d["z"] = out
             res.append(d)
         if progress and now - last_report >= 5.0:
             progress(f"served {len(res)} of {n} requests, {i} submitted, t = {now:.1f} s")
