@@ -2007,11 +2007,12 @@ FP_API fp_status fpr_server_submit(fpr_server *s, const fpr_request *r, double t
         fpr::ServeRequest q;
         q.k = r->ndim;
         if (q.k != 2 && q.k != 3) return FP_EINVAL;
+        q.has_xs = r->strides != nullptr;
         for (int a = 0; a < q.k; ++a) {
             q.L[3 - q.k + a] = r->shape[a];
-            q.xs[3 - q.k + a] = r->strides ? r->strides[a] : 0;
+            q.xs[3 - q.k + a] = q.has_xs ? r->strides[a] : 0;
         }
-        if (r->strides && q.k == 2) q.xs[0] = 0;
+        q.caller = (hipStream_t)r->stream;
         q.x = r->x;
         q.z = r->z;
         q.gamma = r->gamma;
@@ -2036,11 +2037,20 @@ FP_API fp_status fpr_server_step(fpr_server *s, int *busy, int *queued) {
 
 FP_API int64_t fpr_server_poll(fpr_server *s, fpr_result *out, int64_t max) {
     if (!s || !out || max <= 0) return 0;
-    std::vector<fpr::ServeResult> tmp((size_t)max);
-    const int64_t n = s->s->poll(tmp.data(), max);
-    for (int64_t i = 0; i < n; ++i) {
-        const fpr::ServeResult &R = tmp[(size_t)i];
-        out[i] = fpr_result{R.id, R.status, R.converged, R.iters, R.gap, R.zerr, R.t_submit, R.t_admit, R.t_done};
+    // results move through a small buffer on the stack: nothing allocates
+    // and nothing can throw across the boundary
+    fpr::ServeResult buf[64];
+    int64_t n = 0;
+    while (n < max) {
+        const int64_t want = std::min<int64_t>(64, max - n);
+        const int64_t got = s->s->poll(buf, want);
+        for (int64_t i = 0; i < got; ++i) {
+            const fpr::ServeResult &R = buf[i];
+            out[n + i] = fpr_result{R.id, R.status, R.converged, R.iters, R.gap, R.zerr, R.t_submit, R.t_admit,
+                                    R.t_done};
+        }
+        n += got;
+        if (got < want) break;
     }
     return n;
 }

@@ -60,6 +60,13 @@ int64_t RangeAlloc::alloc(int64_t n) {
     return -1;
 }
 
+bool RangeAlloc::fits(int64_t n) const {
+    if (n <= 0) return false;
+    for (const auto &r : free_)
+        if (r.second >= n) return true;
+    return false;
+}
+
 void RangeAlloc::free(int64_t off, int64_t n) {
     if (off < 0 || n <= 0) return;
     auto it = free_.emplace(off, n).first;
@@ -94,6 +101,7 @@ SlotPool::~SlotPool() {
     for (hipEvent_t e : stage_ev_)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    if (copy_ev_) (void)hipEventDestroy(copy_ev_);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -139,6 +147,7 @@ int SlotPool::setup(const Problem &P, const KernelChoice &kc, int tsize, int slo
     events_.resize(4);
     for (auto &ev : events_)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return e;
+    if ((e = hipEventCreateWithFlags(&copy_ev_, hipEventDisableTiming))) return e;
     return hipStreamSynchronize(stream);
 }
 
@@ -181,15 +190,17 @@ static void launch_setup(const Problem &Pi, void *pool, ImgDesc *desc, ImgState 
 }
 
 // A free slot, the request's pool memory (x, z, state; the dual) and its
-// tiles; its descriptor. False when any of them is not free now.
-bool SlotPool::reserve(const ServeRequest &r, int *slot, ImgDesc *Dout, int64_t *offs) {
+// tiles; its descriptor. kFits, or the first of them (slot, tiles, memory)
+// that is not free now; only a shortage of memory can be cured by evicting
+// kept duals.
+int SlotPool::reserve(const ServeRequest &r, int *slot, ImgDesc *Dout, int64_t *offs) {
     int i = -1;
     for (int j = 0; j < nslots_; ++j)
         if (!slots_[(size_t)j].used) {
             i = j;
             break;
         }
-    if (i < 0) return false;
+    if (i < 0) return kShortSlot;
     const int k = P_.k;
     const int64_t N = r.L[0] * r.L[1] * r.L[2];
     ImgDesc D{};
@@ -199,19 +210,15 @@ bool SlotPool::reserve(const ServeRequest &r, int *slot, ImgDesc *Dout, int64_t 
     D.ntz = (int32_t)kc_.tiles_z(r.L[0]);
     D.ntiles = D.ntx * D.nty * D.ntz;
     const int64_t n_main = (2 + 4 * (int64_t)k) * N, n_dual = (int64_t)k * N;
+    if (!tiles_.fits(D.ntiles)) return kShortTiles;
     const int64_t off_main = mem_.alloc(n_main);
-    if (off_main < 0) return false;
+    if (off_main < 0) return kShortMem;
     const int64_t off_dual = mem_.alloc(n_dual);
     if (off_dual < 0) {
         mem_.free(off_main, n_main);
-        return false;
+        return kShortMem;
     }
-    const int64_t tile_off = tiles_.alloc(D.ntiles);
-    if (tile_off < 0) {
-        mem_.free(off_main, n_main);
-        mem_.free(off_dual, n_dual);
-        return false;
-    }
+    const int64_t tile_off = tiles_.alloc(D.ntiles);  // fits: checked above
     char *base = (char *)pool_;
     const size_t ts = (size_t)tsize_;
     D.d = base + (size_t)off_main * ts;
@@ -229,7 +236,7 @@ bool SlotPool::reserve(const ServeRequest &r, int *slot, ImgDesc *Dout, int64_t 
     offs[2] = off_dual;
     offs[3] = n_dual;
     offs[4] = tile_off;
-    return true;
+    return kFits;
 }
 
 void SlotPool::occupy(int i, const ServeRequest &r, const ImgDesc &D, const int64_t *offs, double now) {
@@ -251,14 +258,12 @@ void SlotPool::occupy(int i, const ServeRequest &r, const ImgDesc &D, const int6
     top_ = std::max(top_, i + 1);
 }
 
-// x's strides, or the contiguous ones when none are given.
+// x's strides when given (any values, zero for a broadcast axis), else the
+// C-order ones. An axis of extent 1 is never stepped along, so it takes its
+// C-order stride, which lets a contiguous input be recognised as one.
 static void input_strides(const ServeRequest &r, int64_t st[3]) {
-    for (int a = 0; a < 3; ++a) st[a] = r.xs[a];
-    if (!st[0] && !st[1] && !st[2]) {
-        st[2] = 1;
-        st[1] = r.L[2];
-        st[0] = r.L[1] * r.L[2];
-    }
+    const int64_t c[3] = {r.L[1] * r.L[2], r.L[2], 1};
+    for (int a = 0; a < 3; ++a) st[a] = (r.has_xs && r.L[a] > 1) ? r.xs[a] : c[a];
 }
 
 // Whether the prologue will solve r as exact 1D TV (one active axis, the
@@ -281,14 +286,20 @@ bool SlotPool::groupable(const ServeRequest &r) const {
     return prologue_blocks(N) <= kGroupBlocks / kGroupMax;
 }
 
-bool SlotPool::stage(const ServeRequest &r, double now, int *err) {
+bool SlotPool::stage(const ServeRequest &r, double now, int *err, int *why) {
     *err = 0;
     if (staged_ == kGroupMax)
         if ((*err = flush_group())) return false;
     int i = -1;
     ImgDesc D{};
     int64_t offs[5];
-    if (!reserve(r, &i, &D, offs)) return false;
+    if ((*why = reserve(r, &i, &D, offs)) != kFits) return false;
+    // the group's gather reads x after the work on its caller's stream
+    // before the submission
+    if (r.ready && (*err = hipStreamWaitEvent(stream, r.ready, 0))) {
+        release_reserved(D, offs);
+        return false;
+    }
     if (staged_ == 0) {
         // the staging buffer's previous copy has been read
         if ((*err = hipEventSynchronize(stage_ev_[stage_buf_]))) return false;
@@ -345,13 +356,17 @@ int SlotPool::flush_group() {
     return hipGetLastError();
 }
 
-bool SlotPool::try_admit(const ServeRequest &r, double now, int *err) {
+bool SlotPool::try_admit(const ServeRequest &r, double now, int *err, int *why) {
     *err = 0;
     if ((*err = flush_group())) return false;  // admissions stay in order
     int i = -1;
     ImgDesc D{};
     int64_t offs[5];
-    if (!reserve(r, &i, &D, offs)) return false;
+    if ((*why = reserve(r, &i, &D, offs)) != kFits) return false;
+    if (r.ready && (*err = hipStreamWaitEvent(stream, r.ready, 0))) {
+        release_reserved(D, offs);
+        return false;
+    }
     const int64_t N = r.L[0] * r.L[1] * r.L[2];
     const int64_t n_dual = offs[3];
     hipStream_t s = stream;
@@ -464,12 +479,6 @@ int SlotPool::finish_slot(int i, const ImgState &St, std::vector<ServeResult> &o
     R.t_submit = S.req.t_submit;
     R.t_admit = S.t_admit;
     R.t_done = now;
-    if (S.req.z) {
-        const char *zsrc = (const char *)pool_ + (size_t)(S.off_main + S.N) * (size_t)tsize_;
-        if (hipError_t e = hipMemcpyAsync(S.req.z, zsrc, (size_t)S.N * (size_t)tsize_, hipMemcpyDeviceToDevice,
-                                          stream))
-            return e;
-    }
     mem_.free(S.off_main, S.n_main);
     tiles_.free(S.tile_off, S.ntiles);
     if (S.req.keep_dual && !R.status) {
@@ -490,6 +499,32 @@ bool SlotPool::oldest_done() const {
     return !flight_.empty() && hipEventQuery(flight_.front().ev) != hipErrorNotReady;
 }
 
+int SlotPool::copy_out(const std::vector<int> &done, hipEvent_t chunk) {
+    // One pass per distinct caller stream: it waits for the chunk that wrote
+    // the outputs, copies them, and this pool's stream waits for the copies
+    // before any later work can reuse the memory they are read from.
+    std::vector<hipStream_t> streams;
+    for (int i : done) {
+        const Slot &S = slots_[(size_t)i];
+        if (S.req.z && std::find(streams.begin(), streams.end(), S.req.caller) == streams.end())
+            streams.push_back(S.req.caller);
+    }
+    for (hipStream_t cs : streams) {
+        if (hipError_t e = hipStreamWaitEvent(cs, chunk, 0)) return e;
+        for (int i : done) {
+            const Slot &S = slots_[(size_t)i];
+            if (!S.req.z || S.req.caller != cs) continue;
+            const char *zsrc = (const char *)pool_ + (size_t)(S.off_main + S.N) * (size_t)tsize_;
+            if (hipError_t e = hipMemcpyAsync(S.req.z, zsrc, (size_t)S.N * (size_t)tsize_, hipMemcpyDeviceToDevice,
+                                              cs))
+                return e;
+        }
+        if (hipError_t e = hipEventRecord(copy_ev_, cs)) return e;
+        if (hipError_t e = hipStreamWaitEvent(stream, copy_ev_, 0)) return e;
+    }
+    return 0;
+}
+
 int SlotPool::retire(std::vector<ServeResult> &out, double (*clock)(const void *), const void *ctx) {
     if (flight_.empty()) return 0;
     const Flight f = flight_.front();
@@ -497,13 +532,18 @@ int SlotPool::retire(std::vector<ServeResult> &out, double (*clock)(const void *
     if (hipError_t e = hipEventSynchronize(f.ev)) return e;
     const double now = clock(ctx);
     const ImgState *snap = hsnap_[f.buf];
+    std::vector<int> &done = retire_buf_;
+    done.clear();
     for (int i = 0; i < f.top; ++i) {
-        Slot &S = slots_[(size_t)i];
+        const Slot &S = slots_[(size_t)i];
         // a slot admitted after the chunk was enqueued is not in its snapshot
         if (!S.used || S.admit_seq >= f.seq) continue;
         if (snap[i].mode != kDone) continue;
-        if (int e = finish_slot(i, snap[i], out, now)) return e;
+        done.push_back(i);
     }
+    if (int e = copy_out(done, f.ev)) return e;
+    for (int i : done)
+        if (int e = finish_slot(i, snap[i], out, now)) return e;
     return 0;
 }
 
@@ -511,7 +551,10 @@ int SlotPool::retire(std::vector<ServeResult> &out, double (*clock)(const void *
 // Server
 // ---------------------------------------------------------------------------
 
-Server::~Server() = default;
+Server::~Server() {
+    for (auto &p : pools_) p.reset();  // each waits for its stream
+    for (hipEvent_t e : ev_all_) (void)hipEventDestroy(e);
+}
 
 int Server::setup(const Problem *probs, const KernelChoice *kcs, int tsize, int slots, size_t pool_bytes) {
     for (int c = 0; c < 2; ++c) {
@@ -547,6 +590,17 @@ int Server::submit(ServeRequest r, int64_t *id) {
     if (!(r.gamma >= 0.0) || !std::isfinite(r.gamma) || !(r.tol >= 0.0) || !std::isfinite(r.tol) || r.max_iter < 1)
         return FP_EPARAM;
     if (!pools_[c]->fits_ever(r)) return FP_ENOMEM;
+    // the admission's gather waits for the work on the caller's stream
+    // before this point (the work that produces x)
+    if (ev_free_.empty()) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming)) return FP_EDEVICE;
+        ev_all_.push_back(ev);
+        ev_free_.push_back(ev);
+    }
+    if (hipEventRecord(ev_free_.back(), r.caller)) return FP_EDEVICE;
+    r.ready = ev_free_.back();
+    ev_free_.pop_back();
     r.id = next_id_++;
     if (r.t_submit < 0.0) r.t_submit = now();
     queue_[c].push_back(r);
@@ -575,26 +629,34 @@ int Server::step(int *live, int *queued) {
                 if (r.warm_from >= 0 && order[0]->has_kept(r.warm_from)) order[1] = nullptr;
             }
             bool ok = false;
-            for (SlotPool *P : order) {
+            int why[2] = {SlotPool::kShortSlot, SlotPool::kShortSlot};
+            for (int j = 0; j < 2 && !ok; ++j) {
+                SlotPool *P = order[j];
                 if (!P) continue;
-                ok = P->groupable(r) ? P->stage(r, now(), &err) : P->try_admit(r, now(), &err);
+                ok = P->groupable(r) ? P->stage(r, now(), &err, &why[j]) : P->try_admit(r, now(), &err, &why[j]);
                 if (err) return err;
-                if (ok) break;
             }
             if (ok) {
+                ev_free_.push_back(r.ready);  // its wait is enqueued
                 queue_[c].pop_front();
                 continue;
             }
-            // Room from kept duals other than the one this request warms from;
-            // that one goes too when nothing else would ever free room (its
-            // lane has no request live or in flight): the request then starts
-            // cold instead of waiting for good.
+            // Room from kept duals, in a lane whose only shortage was memory
+            // (a slot and tiles were free): first duals other than the one
+            // this request warms from; that one goes too when nothing else
+            // would ever free room (its lane has no request live or in
+            // flight): the request then starts cold instead of waiting for
+            // good. A lane short of slots or tiles waits for completions and
+            // keeps its duals.
             bool evicted = false;
-            for (SlotPool *P : order)
-                if (P && !evicted) evicted = P->evict_one_dual(r.warm_from);
+            for (int j = 0; j < 2 && !evicted; ++j)
+                if (order[j] && why[j] == SlotPool::kShortMem) evicted = order[j]->evict_one_dual(r.warm_from);
             if (!evicted && r.warm_from >= 0)
-                for (SlotPool *P : order)
-                    if (P && !evicted && P->live() == 0 && P->in_flight() == 0) evicted = P->evict_one_dual();
+                for (int j = 0; j < 2 && !evicted; ++j) {
+                    SlotPool *P = order[j];
+                    if (P && why[j] == SlotPool::kShortMem && P->live() == 0 && P->in_flight() == 0)
+                        evicted = P->evict_one_dual();
+                }
             if (evicted) continue;
             break;
         }

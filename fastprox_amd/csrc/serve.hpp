@@ -23,6 +23,7 @@ class RangeAlloc {
 public:
     void reset(int64_t capacity);
     int64_t alloc(int64_t n);  // offset, or -1
+    bool fits(int64_t n) const;  // whether alloc(n) would succeed now
     void free(int64_t off, int64_t n);
     int64_t free_total() const { return free_total_; }
 
@@ -36,7 +37,10 @@ struct ServeRequest {
     int k = 2;
     int64_t L[3] = {1, 1, 1};    // extents, padded in front with 1
     const void *x = nullptr;     // device, the server's dtype
+    bool has_xs = false;         // xs holds x's strides (else x is in C order)
     int64_t xs[3] = {0, 0, 0};   // element strides of x (padded like L)
+    hipStream_t caller = nullptr;  // x is produced and z consumed on this stream
+    hipEvent_t ready = nullptr;    // recorded on caller at the submit
     void *z = nullptr;           // device output (contiguous) or null
     double gamma = 0.0, tol = 1e-6;
     int max_iter = 100000;
@@ -62,9 +66,11 @@ public:
     ~SlotPool();
     int setup(const Problem &P, const KernelChoice &kc, int tsize, int slots, int64_t pool_elems,
               int64_t tile_cap, int64_t max_pixels);
+    // What an admission found short: a slot, tile range or memory.
+    enum : int { kFits = 0, kShortSlot = 1, kShortTiles = 2, kShortMem = 3 };
     // Admits r into a free slot if a slot, memory and tiles are free (and
-    // enqueues its set-up); false otherwise.
-    bool try_admit(const ServeRequest &r, double now, int *err);
+    // enqueues its set-up); false otherwise, with *why what was short.
+    bool try_admit(const ServeRequest &r, double now, int *err, int *why);
     // Grouped admission: stage() takes r's slot, memory and tiles like
     // try_admit, and leaves its set-up to flush_group(), which sets up every
     // staged request at once: one copy of their descriptors, one gather of
@@ -76,7 +82,7 @@ public:
     static constexpr int kGroupBlocks = 64 * 512;
     bool groupable(const ServeRequest &r) const;
     bool exact_1d(const ServeRequest &r) const;
-    bool stage(const ServeRequest &r, double now, int *err);
+    bool stage(const ServeRequest &r, double now, int *err, int *why);
     int flush_group();
     // Enqueues a chunk of step/epilogue launches and a state snapshot when
     // any slot is live; returns whether one was enqueued.
@@ -120,7 +126,11 @@ private:
         uint64_t stamp;
     };
     int finish_slot(int i, const ImgState &S, std::vector<ServeResult> &out, double now);
-    bool reserve(const ServeRequest &r, int *slot, ImgDesc *D, int64_t *offs);
+    // Copies the outputs of the finished slots to their callers' buffers on
+    // the callers' streams (after the chunk's event), and orders this
+    // pool's later work after those copies.
+    int copy_out(const std::vector<int> &done, hipEvent_t chunk);
+    int reserve(const ServeRequest &r, int *slot, ImgDesc *D, int64_t *offs);  // kFits or what is short
     void occupy(int i, const ServeRequest &r, const ImgDesc &D, const int64_t *offs, double now);
     void release_reserved(const ImgDesc &D, const int64_t *offs);
     // The staged group (pinned, two buffers: one may still be in a copy) and
@@ -166,12 +176,14 @@ private:
     int64_t seq_ = 0;
     std::deque<Flight> flight_;
     std::vector<hipEvent_t> events_;
+    hipEvent_t copy_ev_ = nullptr;  // recorded on a caller's stream after its copies
     int p_ = 0;       // the list the next step reads
     int live_ = 0;
     int top_ = 0;     // 1 + the highest slot ever used
     int64_t live_tiles_ = 0;
     std::map<int64_t, Kept> kept_;
     uint64_t stamp_ = 0;
+    std::vector<int> retire_buf_;  // the slots a retirement finishes
 };
 
 class Server {
@@ -209,6 +221,9 @@ private:
     SlotPool *lane(int c, int i) const { return i == 0 ? pools_[c].get() : pools_[2].get(); }
     std::deque<ServeRequest> queue_[2];
     std::vector<ServeResult> done_;
+    // Events recorded on the callers' streams at submission, waited on by
+    // the admission; recycled once the wait is enqueued.
+    std::vector<hipEvent_t> ev_free_, ev_all_;
     int64_t next_id_ = 0;
     std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
 };

@@ -24,7 +24,7 @@ from .core import FastproxError, _check, lib
 class _Request(C.Structure):
     _fields_ = [("ndim", C.c_int), ("shape", C.c_int64 * 3), ("x", C.c_void_p), ("strides", C.POINTER(C.c_int64)),
                 ("z", C.c_void_p), ("gamma", C.c_double), ("tol", C.c_double), ("max_iter", C.c_int),
-                ("keep_dual", C.c_int), ("warm_from", C.c_int64)]
+                ("keep_dual", C.c_int), ("warm_from", C.c_int64), ("stream", C.c_void_p)]
 
 
 class _Result(C.Structure):
@@ -81,10 +81,13 @@ class Server:
     def clock(self):
         return float(lib().fpr_server_clock(self.h))
 
-    def submit(self, x, gamma, tol=1e-6, max_iter=100000, out=None, keep_dual=False, warm_from=-1, t_submit=-1.0):
+    def submit(self, x, gamma, tol=1e-6, max_iter=100000, out=None, keep_dual=False, warm_from=-1, t_submit=-1.0,
+               stream=None):
         """Queues the image x (a 2D or 3D cuda tensor of the server's dtype,
-        any strides); out: a contiguous cuda tensor for z, or None. Returns
-        the request id."""
+        any strides); out: a contiguous cuda tensor for z, or None. stream
+        (default: torch's current stream): x is read after the work queued
+        on it so far, and out is written before the work queued on it after
+        the poll() that returns the request. Returns the request id."""
         if x.dtype != self.dtype or not x.is_cuda or x.dim() not in (2, 3):
             raise ValueError("x must be a 2D or 3D cuda tensor of the server's dtype")
         r = _Request()
@@ -101,6 +104,9 @@ class Server:
         r.gamma, r.tol, r.max_iter = float(gamma), float(tol), int(max_iter)
         r.keep_dual = int(bool(keep_dual))
         r.warm_from = int(warm_from)
+        if stream is None:
+            stream = torch.cuda.current_stream(x.device)
+        r.stream = C.c_void_p(stream.cuda_stream or None)
         rid = C.c_int64()
         _check(lib().fpr_server_submit(self.h, C.byref(r), float(t_submit), C.byref(rid)), "fpr_server_submit")
         self._held[rid.value] = (x, out, st)

@@ -355,6 +355,11 @@ typedef struct {
     int keep_dual;            /* keep the dual resident after completion */
     int64_t warm_from;        /* start from the kept dual of that request
                                  (same shape), or -1 */
+    void *stream;             /* the caller's hipStream_t (NULL: the null
+                                 stream): x is read after the work enqueued on
+                                 it before the submit, and z is written before
+                                 the work enqueued on it after the poll that
+                                 returns the request's result */
 } fpr_request;
 typedef struct {
     int64_t id;
@@ -375,7 +380,10 @@ FP_API fp_status fpr_server_submit(fpr_server *s, const fpr_request *r, double t
 /* One round of the loop (admissions, launches, retirements); busy: live
  * requests plus chunks in flight, queued: requests waiting. */
 FP_API fp_status fpr_server_step(fpr_server *s, int *busy, int *queued);
-/* Moves up to max finished requests' results to out; returns the count. */
+/* Moves up to max finished requests' results to out; returns the count. A
+ * result's z is ordered on the request's stream: work enqueued there after
+ * this call sees it (a host read needs that stream synchronised first).
+ * Allocates nothing. */
 FP_API int64_t   fpr_server_poll(fpr_server *s, fpr_result *out, int64_t max);
 FP_API double    fpr_server_clock(const fpr_server *s);
 FP_API fp_status fpr_server_release(fpr_server *s, int64_t id);

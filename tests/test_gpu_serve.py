@@ -217,3 +217,74 @@ def test_replay_reports_latency(cuda):
     assert len(res) == len(trace)
     st = latency_stats(res)
     assert st["p99_ms"] >= st["p50_ms"] > 0 and st["converged"] == len(trace)
+
+
+def test_serve_full_slots_keep_the_kept_dual(cuda):
+    """Slots, not memory, are what is short: with both slots busy and a
+    request queued, the kept dual stays resident, and a later warm start
+    from it has the bits of the warm plain solve."""
+    src2d, _ = _sources()
+    srv = Server(Op(True, 2), None, pool_bytes=64 << 20, slots=2)
+    kern = {2: srv.kernel(2)}
+    xa = src2d[0][:120, :130]
+    a = srv.submit(xa, 0.1, tol=1e-5, out=torch.empty((120, 130), device="cuda"), keep_dual=True)
+    assert len(srv.drain()) == 1
+    busy = [src2d[1][:300, :300], src2d[2][:300, :300]]
+    for x in busy:  # long solves that hold both slots
+        srv.submit(x, 0.3, tol=1e-7, out=torch.empty((300, 300), device="cuda"))
+    srv.submit(src2d[1][:100, :100], 0.1, tol=1e-5, out=torch.empty((100, 100), device="cuda"))
+    xe = src2d[0][1:121, 2:132]
+    ze = torch.empty((120, 130), device="cuda")
+    e = srv.submit(xe, 0.12, tol=1e-6, out=ze, warm_from=a)
+    for _ in range(4):  # rounds in which the queued requests find no slot
+        srv.step()
+    res = {d["id"]: d for d in srv.drain()}
+    torch.cuda.synchronize()
+    _, _, _, _, dual_a = _alone(xa, 0.1, 1e-5, kernel=kern)
+    z, its, gap, st, _ = _alone(xe, 0.12, 1e-6, warm=dual_a, kernel=kern)
+    assert res[e]["iters"] == its and torch.equal(ze, z)
+    srv.release(a)  # still kept
+
+
+def test_serve_broadcast_inputs(cuda):
+    """Inputs with a zero stride (expanded views) are gathered from their
+    strides: the results equal the plain solves of their contiguous copies."""
+    srv = Server(Op(True, 2), None, pool_bytes=32 << 20, slots=4)
+    kern = {2: srv.kernel(2)}
+    g = torch.Generator(device="cpu").manual_seed(3)
+    row = torch.rand((1, 64), generator=g).cuda()
+    col = torch.rand((48, 1), generator=g).cuda()
+    xs = [row.expand(48, 64), col.expand(48, 64), torch.full((1, 1), 0.7, device="cuda").expand(40, 30)]
+    outs = [torch.empty(tuple(x.shape), device="cuda") for x in xs]
+    ids = [srv.submit(x, 0.05, tol=1e-6, out=o) for x, o in zip(xs, outs)]
+    res = {d["id"]: d for d in srv.drain()}
+    torch.cuda.synchronize()
+    for x, o, i in zip(xs, outs, ids):
+        z, its, gap, st, _ = _alone(x.contiguous(), 0.05, 1e-6, kernel=kern)
+        assert res[i]["iters"] == its and torch.equal(o, z), tuple(x.stride())
+
+
+def test_serve_orders_on_the_callers_stream(cuda):
+    """A request submitted on a side stream right after the work that
+    produces its input, and read on that stream after the poll that returns
+    it, without a host synchronisation in between: the server reads x after
+    it is produced and writes z before the stream's later work reads it."""
+    src2d, _ = _sources()
+    srv = Server(Op(True, 2), None, pool_bytes=64 << 20, slots=4)
+    kern = {2: srv.kernel(2)}
+    side = torch.cuda.Stream()
+    a = torch.randn((2048, 2048), device="cuda")
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        x = torch.zeros((200, 210), device="cuda")
+        for _ in range(20):  # keeps the side stream busy before x is written
+            a = (a @ a) * 1e-3
+        x.copy_(src2d[0][:200, :210] + 0.0 * a[:200, :210])
+        out = torch.empty_like(x)
+        rid = srv.submit(x, 0.1, tol=1e-6, out=out)
+        res = srv.drain()
+        got = out.clone()  # on the side stream, after the poll
+    torch.cuda.synchronize()
+    z, its, gap, st, _ = _alone(src2d[0][:200, :210], 0.1, 1e-6, kernel=kern)
+    assert res[0]["id"] == rid and res[0]["iters"] == its
+    assert torch.equal(got, z)
