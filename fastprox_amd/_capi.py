@@ -108,6 +108,7 @@ def declare(lib):
         "fp_strerror": (C.c_char_p, [I]),
         "fp_abi_version": (I, []),
         "fp_version_string": (C.c_char_p, []),
+        "fpr_alloc_count": (I64, []),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name, None)

@@ -19,6 +19,7 @@
 #include <new>
 #include <vector>
 
+#include "alloc.hpp"
 #include "../../include/fastprox_rocm.h"
 #include "autotune.hpp"
 #include "deriv.hpp"
@@ -454,7 +455,43 @@ fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, fpr::KernelC
 }
 
 fp_status alloc(void **p, size_t bytes) {
-    if (hipMalloc(p, std::max<size_t>(bytes, 16))) return FP_ENOMEM;
+    if (fpr::dev_alloc(p, std::max<size_t>(bytes, 16))) return FP_ENOMEM;
+    return FP_OK;
+}
+
+// keep_structure: every buffer the derivatives and the structure queries of a
+// kept call use, so that fp_prox, fp_prox_steps, fp_jvp, fp_vjp, the
+// queries and fp_workspace_set_branch allocate no device memory: the saved
+// point, the derivative structure's buffers (fpr::deriv_reserve), the small
+// per-image arrays of the derivatives, the queries' output and the branch
+// overrides (the reference sizes these here too, api.cpp:265-289).
+fp_status reserve_kept(fp_workspace &w) {
+    const fp_op *op = w.op;
+    const size_t bytes = (size_t)w.numel * w.tsize;
+    if (fp_status s = alloc(&w.xsave, bytes)) return s;
+    if (fp_status s = alloc(&w.zsave, bytes)) return s;
+    if (!op->iso)
+        if (fp_status s = alloc(&w.usave, bytes * (size_t)op->k)) return s;
+    w.exps.assign((size_t)w.B, 0);
+    fpr::DerivSpec sp;
+    sp.iso = op->iso;
+    sp.k = op->k;
+    for (int a = 0; a < 3; ++a) sp.L[a] = w.img[a];
+    sp.B = w.B;
+    sp.tsize = (int)w.tsize;
+    for (int a = 0; a < op->k; ++a) sp.w[a] = op->w[a];
+    hipError_t err;
+    sp.sw = op->device_weights(w.device, (int)w.tsize, err);
+    if (err) return hip_status(err);
+    sp.sw_img = 0;
+    fpr::Deriv *d = nullptr;
+    if (int r = fpr::deriv_reserve(sp, &d)) return (fp_status)r;
+    w.deriv.reset(d);
+    w.dscr_n = (size_t)(w.B + op->k + (int64_t)op->spatial.size());
+    if (fp_status s = alloc((void **)&w.dscr, std::max<size_t>(w.dscr_n, 16) * sizeof(double))) return s;
+    if (fp_status s = alloc(&w.qbuf, 9 * (size_t)w.numel)) return s;
+    if (!op->iso)
+        if (fp_status s = alloc((void **)&w.branch, (size_t)w.numel)) return s;
     return FP_OK;
 }
 
@@ -502,16 +539,13 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     if (fp_status s = alloc(&w->dstage, bytes)) return s;
     if (fp_status s = alloc(&w->zstage, bytes)) return s;
     if (fp_status s = alloc(&w->q, bytes * (size_t)op->k)) return s;
-    if (w->keep) {
-        if (fp_status s = alloc(&w->xsave, bytes)) return s;
-        if (fp_status s = alloc(&w->zsave, bytes)) return s;
-        if (!op->iso)
-            if (fp_status s = alloc(&w->usave, bytes * (size_t)op->k)) return s;
-        w->exps.assign((size_t)B, 0);
-    }
     hipError_t err;
     (void)op->device_weights(w->device, (int)w->tsize, err);
     if (err) return hip_status(err);
+    if (w->keep)
+        if (fp_status s = reserve_kept(*w)) return s;
+    // host arrays are packed through host_stage: sized here for x's dtype
+    w->host_stage.reserve(bytes * (size_t)op->k);
     *out = w.release();
     return FP_OK;
 }
@@ -658,7 +692,7 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     ws->valid = false;
     ws->steps_on = false;
     ws->info_valid = false;
-    ws->deriv.reset();
+    ws->deriv_built = false;
     ws->branch_on = false;
     if (x.numel() == 0) {
         ws->dual_valid = true;
@@ -744,7 +778,7 @@ fp_status refine_impl(const fp_op *op, fp_workspace *ws, double tol, int max_ite
     }
     if (info) *info = fp_info{0.0, 0, 0};
     ws->info_valid = false;
-    ws->deriv.reset();
+    ws->deriv_built = false;
     ws->branch_on = false;
     if (ws->numel == 0) {
         ws->info = fp_info{0.0, 0, 0};
@@ -831,17 +865,14 @@ fp_status dual_impl(fp_workspace *ws, const fp_array *aa, bool to_ws) {
             if (a.device == FP_DEVICE_CPU) {
                 unpack_host(packed.data(), a);
             } else {
-                void *tmp = nullptr;
-                if (hipMalloc(&tmp, packed.size())) return FP_ENOMEM;
-                fp_status st = FP_OK;
-                if (hipMemcpy(tmp, packed.data(), packed.size(), hipMemcpyHostToDevice)) st = FP_EDEVICE;
-                const size_t saved = ws->tsize;
-                ws->tsize = a.itemsize();
-                if (!st) st = stage_out(*ws, tmp, a, s);
-                ws->tsize = saved;
-                if (!st && hipStreamSynchronize(s)) st = FP_EDEVICE;
-                (void)hipFree(tmp);
-                if (st) return st;
+                // the dual converted on the device straight into a's dtype and strides
+                if (a.ndim > fpr::kMaxStrided) return FP_ESHAPE;
+                const fpr::StridedShape sh = strided_of(a);
+                if (ws->tsize == 4)
+                    fpr::launch_convert_scatter<float, double>((const float *)ws->q, (double *)a.data, n, a.ndim, sh, s);
+                else
+                    fpr::launch_convert_scatter<double, float>((const double *)ws->q, (float *)a.data, n, a.ndim, sh, s);
+                if (hipGetLastError() || hipStreamSynchronize(s)) return FP_EDEVICE;
             }
             return FP_OK;
         }
@@ -936,21 +967,18 @@ fp_status write_doubles(const Arr &x, const double *v) {
     return FP_OK;
 }
 
+// The derivatives' small device arrays (sized by reserve_kept).
 fp_status scratch(fp_workspace *ws, size_t n, double **out) {
-    if (ws->dscr_n < n) {
-        if (ws->dscr) (void)hipFree(ws->dscr);
-        ws->dscr = nullptr;
-        ws->dscr_n = 0;
-        if (hipMalloc((void **)&ws->dscr, std::max<size_t>(n, 16) * sizeof(double))) return FP_ENOMEM;
-        ws->dscr_n = n;
-    }
+    if (!ws->dscr || ws->dscr_n < n) return FP_EINTERNAL;
     *out = ws->dscr;
     return FP_OK;
 }
 
-// The structure of the kept solve, built on first use.
+// The structure of the kept solve, built into its reserved buffers on first
+// use after each kept call.
 fp_status ensure_deriv(const fp_op *op, fp_workspace *ws, hipStream_t s) {
-    if (ws->deriv) return FP_OK;
+    if (ws->deriv_built) return FP_OK;
+    if (!ws->deriv) return FP_EINVAL;
     fpr::DerivSpec sp;
     sp.iso = op->iso;
     sp.k = op->k;
@@ -984,9 +1012,8 @@ fp_status ensure_deriv(const fp_op *op, fp_workspace *ws, hipStream_t s) {
         sp.q_img = ws->N;
         sp.q_plane = ws->B * ws->N;
     }
-    fpr::Deriv *d = nullptr;
-    if (int r = fpr::deriv_create(sp, s, &d)) return (fp_status)r;
-    ws->deriv.reset(d);
+    if (int r = fpr::deriv_build(ws->deriv.get(), sp, s)) return (fp_status)r;
+    ws->deriv_built = true;
     return FP_OK;
 }
 
@@ -1360,19 +1387,6 @@ fp_status struct_out(const fp_workspace &ws, const fp_array *a, Arr &out, OutKin
     return check_output(nullptr, out);
 }
 
-// A device buffer for one query, freed after the stream has finished with it.
-struct QueryBuf {
-    void *p = nullptr;
-    hipStream_t s = nullptr;
-    explicit QueryBuf(hipStream_t st) : s(st) {}
-    ~QueryBuf() {
-        if (p) {
-            (void)hipStreamSynchronize(s);
-            (void)hipFree(p);
-        }
-    }
-    fp_status alloc(size_t bytes) { return hipMalloc(&p, std::max<size_t>(bytes, 16)) ? FP_ENOMEM : FP_OK; }
-};
 
 // The contiguous device entries src (C order, out's item size) into out
 // (host or device, any strides).
@@ -1397,13 +1411,14 @@ fp_status put_struct(const void *src, const Arr &a, hipStream_t s) {
     return hipGetLastError() ? FP_EDEVICE : FP_OK;
 }
 
-// Zeros into out (a state array of FREE entries, or a coefficient array).
-fp_status put_zeros(const Arr &a, hipStream_t s) {
-    QueryBuf z(s);
+// Zeros into out (a state array of FREE entries, or a coefficient array):
+// through the workspace's query buffer (9 numel bytes, at least the array's).
+fp_status put_zeros(const fp_workspace &ws, const Arr &a, hipStream_t s) {
     const size_t bytes = (size_t)a.numel() * a.itemsize();
-    if (fp_status st = z.alloc(bytes)) return st;
-    if (hipMemsetAsync(z.p, 0, bytes, s)) return FP_EDEVICE;
-    return put_struct(z.p, a, s);
+    if (!ws.qbuf || bytes > 9 * (size_t)ws.numel) return FP_EINTERNAL;
+    fpr::launch_zero_bytes(ws.qbuf, bytes, s);
+    if (hipGetLastError()) return FP_EDEVICE;
+    return put_struct(ws.qbuf, a, s);
 }
 
 fp_status states_impl(const fp_workspace *ws, fp_array *sa, int64_t *nfixed) {
@@ -1413,7 +1428,7 @@ fp_status states_impl(const fp_workspace *ws, fp_array *sa, int64_t *nfixed) {
         if (fp_status s = struct_out(*ws, sa, st, OutKind::Int8)) return s;
     if (sa && ws->numel) {
         DevGuard guard(ws->device);
-        if (fp_status s = put_zeros(st, (hipStream_t)ws->stream)) return s;
+        if (fp_status s = put_zeros(*ws, st, (hipStream_t)ws->stream)) return s;
     }
     if (nfixed) *nfixed = 0;  // every entry of an image operator is FREE
     return FP_OK;
@@ -1432,18 +1447,16 @@ fp_status regions_impl(const fp_op *op, fp_workspace *ws, fp_array *la, fp_array
     DevGuard guard(ws->device);
     hipStream_t s = (hipStream_t)ws->stream;
     if (fp_status st = ensure_deriv(op, ws, s)) return st;
-    QueryBuf lb(s), sb(s);
+    // labels (up to 8 bytes each) then signs in the query buffer
     const size_t n = (size_t)ws->numel;
-    if (la)
-        if (fp_status st = lb.alloc(n * lab.itemsize())) return st;
-    if (ga)
-        if (fp_status st = sb.alloc(n)) return st;
-    if (int r = fpr::deriv_regions(ws->deriv.get(), s, lb.p, la && lab.dtype == FP_I64, (int8_t *)sb.p, nregions))
+    char *lb = la ? (char *)ws->qbuf : nullptr;
+    int8_t *sb = ga ? (int8_t *)ws->qbuf + 8 * n : nullptr;
+    if (int r = fpr::deriv_regions(ws->deriv.get(), s, lb, la && lab.dtype == FP_I64, sb, nregions))
         return (fp_status)r;
     if (la)
-        if (fp_status st = put_struct(lb.p, lab, s)) return st;
+        if (fp_status st = put_struct(lb, lab, s)) return st;
     if (ga)
-        if (fp_status st = put_struct(sb.p, sg, s)) return st;
+        if (fp_status st = put_struct(sb, sg, s)) return st;
     return FP_OK;
 }
 
@@ -1455,7 +1468,7 @@ fp_status constraint_impl(const fp_workspace *ws, fp_array *ca, int8_t *active, 
         if (fp_status s = struct_out(*ws, ca, c, OutKind::Float)) return s;
     if (ca && ws->numel) {
         DevGuard guard(ws->device);
-        if (fp_status s = put_zeros(c, (hipStream_t)ws->stream)) return s;
+        if (fp_status s = put_zeros(*ws, c, (hipStream_t)ws->stream)) return s;
     }
     if (active) std::fill(active, active + count, int8_t{0});  // TV has no linear constraint
     return FP_OK;
@@ -1469,7 +1482,7 @@ fp_status set_branch_impl(fp_workspace *ws, const fp_array *sa, double tol) {
     if (!sa) {
         if (ws->branch_on) {
             ws->branch_on = false;
-            ws->deriv.reset();
+            ws->deriv_built = false;
         }
         return FP_OK;
     }
@@ -1490,27 +1503,23 @@ fp_status set_branch_impl(fp_workspace *ws, const fp_array *sa, double tol) {
     if (a.device == FP_DEVICE_CPU) {
         pack_host(a, h.data());
     } else {
-        QueryBuf tb(s);
         const void *src = a.data;
         if (!a.contiguous()) {
             if (a.ndim > fpr::kMaxStrided) return FP_ESHAPE;
-            if (fp_status st = tb.alloc(n)) return st;
-            fpr::launch_gather<int8_t>((const int8_t *)a.data, (int8_t *)tb.p, (int64_t)n, a.ndim, strided_of(a), s);
-            src = tb.p;
+            fpr::launch_gather<int8_t>((const int8_t *)a.data, (int8_t *)ws->qbuf, (int64_t)n, a.ndim, strided_of(a),
+                                       s);
+            src = ws->qbuf;
         }
         if (hipMemcpyAsync(h.data(), src, n, hipMemcpyDeviceToHost, s) || hipStreamSynchronize(s)) return FP_EDEVICE;
     }
     for (size_t i = 0; i < n; ++i)
         if (h[i] < -1 || h[i] > 1) return FP_EPARAM;
-    if (!ws->branch && hipMalloc((void **)&ws->branch, n)) {
-        ws->branch = nullptr;
-        return FP_ENOMEM;
-    }
+    if (!ws->branch) return FP_EINTERNAL;  // reserved with the structure (aniso)
     if (hipMemcpyAsync(ws->branch, h.data(), n, hipMemcpyHostToDevice, s) || hipStreamSynchronize(s))
         return FP_EDEVICE;
     ws->branch_on = true;
     ws->branch_tol = tol;
-    ws->deriv.reset();  // rebuilt with the override on first use
+    ws->deriv_built = false;  // rebuilt with the override on first use
     return FP_OK;
 }
 
@@ -1525,12 +1534,11 @@ fp_status kinks_impl(const fp_op *op, fp_workspace *ws, double tol, fp_array *ma
         DevGuard guard(ws->device);
         hipStream_t s = (hipStream_t)ws->stream;
         if (fp_status st = ensure_deriv(op, ws, s)) return st;
-        QueryBuf mb(s);
-        if (fp_status st = mb.alloc((size_t)ws->numel)) return st;
-        if (int r = fpr::deriv_kinks(ws->deriv.get(), s, ws->q, ws->N, ws->B * ws->N, tol, (int8_t *)mb.p, &c))
+        int8_t *mb = (int8_t *)ws->qbuf;
+        if (int r = fpr::deriv_kinks(ws->deriv.get(), s, ws->q, ws->N, ws->B * ws->N, tol, mb, &c))
             return (fp_status)r;
         if (ma)
-            if (fp_status st = put_struct(mb.p, m, s)) return st;
+            if (fp_status st = put_struct(mb, m, s)) return st;
     }
     if (count) *count = c;
     return FP_OK;
@@ -1632,6 +1640,8 @@ FP_API const fpr_plugin_table *fastprox_rocm_plugin(void) {
 }
 
 FP_API const char *fpr_last_error(void) { return g_last_error; }
+
+FP_API int64_t fpr_alloc_count(void) { return fpr::alloc_count(); }
 
 FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt, int ry) {
     return guard([&] {
@@ -1906,7 +1916,7 @@ FP_API fp_status fpr_deriv_create(const fp_op *op, fp_dtype dtype, int64_t count
         d->k = op->k;
         d->ns = sp.sw ? (op->iso ? N : (int64_t)op->k * N) * (sp.sw_img ? count : 1) : 0;
         (void)hipGetDevice(&d->device);
-        if (hipMalloc((void **)&d->scr, 8 * (size_t)(count + 4))) return FP_ENOMEM;
+        if (fpr::dev_alloc((void **)&d->scr, 8 * (size_t)(count + 4))) return FP_ENOMEM;
         fpr::Deriv *dv = nullptr;
         if (int r = fpr::deriv_create(sp, (hipStream_t)stream, &dv)) return (fp_status)r;
         d->d.reset(dv);

@@ -49,6 +49,7 @@
 #include <cmath>
 #include <vector>
 
+#include "alloc.hpp"
 #include "../../include/fastprox_rocm.h"
 #include "deriv.hpp"
 
@@ -1054,7 +1055,10 @@ struct Deriv {
     int nrb = 0;   // region blocks
     int nch = 0;   // region-sum blocks
     int npb = 0;   // pixel blocks per image
-    std::vector<DImg> hdi;
+    int max_rb = 0;  // the region blocks reserved (every image's regions fit)
+    std::vector<DImg> hdi, hdi2;      // reserved: B
+    std::vector<int> hrb;             // reserved: B + 1
+    std::vector<RBlock> hblocks;      // reserved: max_rb
     std::vector<void *> bufs;
     DImg *di = nullptr;
     RBlock *rb = nullptr;
@@ -1070,8 +1074,10 @@ struct Deriv {
     CgState *cg[2] = {nullptr, nullptr};
     int *hcount = nullptr;  // pinned
     CgState *hcg = nullptr; // pinned
-    void *cub = nullptr;
-    size_t cub_bytes = 0;
+    void *cub = nullptr, *scan_tmp = nullptr;
+    size_t cub_bytes = 0, scan_bytes = 0;
+    int8_t *km = nullptr;       // aniso: the kink codes of a branch override
+    int64_t *nreg = nullptr;    // the region counts of a query (B)
     int cur = 0;
     bool has_result = false;
 
@@ -1083,7 +1089,7 @@ struct Deriv {
     template <class P>
     hipError_t alloc(P **p, size_t bytes) {
         void *q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(bytes, 16));
+        hipError_t e = fpr::dev_alloc(&q, std::max<size_t>(bytes, 16));
         if (e) return e;
         bufs.push_back(q);
         *p = (P *)q;
@@ -1102,13 +1108,17 @@ int64_t spatial_len(const Deriv &d) { return d.sp.iso ? d.G.N : d.G.k * d.G.N; }
 template <class T>
 int apply_branch(Deriv &d, hipStream_t s, const T *z, const T *u, const T *sw);
 
-template <class T>
-int build(Deriv &d, hipStream_t s) {
+// Every buffer of a structure of d.sp's shape (B images of L, the dtype,
+// iso or aniso, spatial weights shared or per image), allocated once: build()
+// then writes each solution's structure into them. The sort's and scan's
+// scratch are sized for any region count, the region blocks for the most an
+// image's regions can take (ceil(regions / kRB) per image, summed: at most
+// B N / kRB + B).
+int reserve(Deriv &d) {
     const DerivSpec &sp = d.sp;
     const Geo G = d.G;
     const int64_t BN = G.BN;
     const int B = (int)sp.B;
-    const T *x = (const T *)sp.x, *z = (const T *)sp.z, *u = (const T *)sp.u, *sw = (const T *)sp.sw;
     hipError_t e = hipSuccess;
 #define FPR_A(ptr, bytes) \
     if ((e = d.alloc(&(ptr), (bytes)))) return FP_ENOMEM
@@ -1133,6 +1143,7 @@ int build(Deriv &d, hipStream_t s) {
     FPR_A(d.tail, 8 * (size_t)d.nch);
     d.npb = (int)((G.N + kPixBlk - 1) / kPixBlk);
     FPR_A(d.ppart, 8 * (size_t)kMaxQ * (size_t)B * (size_t)d.npb);
+    FPR_A(d.nreg, sizeof(int64_t) * (size_t)B);
     if (sp.iso) {
         FPR_A(d.gz, 8 * (size_t)G.k * (size_t)BN);
         FPR_A(d.hbuf, 8 * (size_t)G.k * (size_t)BN);
@@ -1144,14 +1155,48 @@ int build(Deriv &d, hipStream_t s) {
         FPR_A(d.cg[1], sizeof(CgState) * (size_t)B);
     } else {
         FPR_A(d.nsg, (size_t)G.k * (size_t)BN);
+        FPR_A(d.km, (size_t)BN);
     }
     if (sp.sw) {
         FPR_A(d.gsw_img, 8 * (size_t)B * (size_t)spatial_len(d));
         if (sp.sw_img == 0) FPR_A(d.dsw_full, 8 * (size_t)B * (size_t)spatial_len(d));
     }
+    const int64_t max_rb = BN / kRB + B;
+    if (max_rb >= (int64_t)1 << 31) return FP_ESHAPE;
+    d.max_rb = (int)max_rb;
+    FPR_A(d.rb, sizeof(RBlock) * (size_t)d.max_rb);
+    FPR_A(d.part, 8 * 3 * (size_t)d.max_rb);
+    size_t b1 = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.isroot, d.rank, (int)BN)) return FP_EDEVICE;
+    d.scan_bytes = b1;
+    FPR_A(d.scan_tmp, b1);
+    size_t most = 0;
+    for (int bits = 1; bits <= 31; ++bits) {
+        size_t b2 = 0;
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, b2, d.label, d.skey, d.iota, d.members, (int)BN, 0, bits))
+            return FP_EDEVICE;
+        most = std::max(most, b2);
+    }
+    d.cub_bytes = most;
+    FPR_A(d.cub, d.cub_bytes);
 #undef FPR_A
-    if (hipHostMalloc((void **)&d.hcount, 16) || hipHostMalloc((void **)&d.hcg, sizeof(CgState) * (size_t)B))
+    if (fpr::pinned_alloc((void **)&d.hcount, 16) || fpr::pinned_alloc((void **)&d.hcg, sizeof(CgState) * (size_t)B))
         return FP_ENOMEM;
+    d.hdi.reserve((size_t)B);
+    d.hdi2.reserve((size_t)B);
+    d.hrb.reserve((size_t)B + 1);
+    d.hblocks.reserve((size_t)d.max_rb);
+    return FP_OK;
+}
+
+template <class T>
+int build(Deriv &d, hipStream_t s) {
+    const DerivSpec &sp = d.sp;
+    const Geo G = d.G;
+    const int64_t BN = G.BN;
+    const int B = (int)sp.B;
+    const T *x = (const T *)sp.x, *z = (const T *)sp.z, *u = (const T *)sp.u, *sw = (const T *)sp.sw;
+    d.has_result = false;
     // Per-image constants on the host (tv_iso.cpp:342-402 for the structure's
     // view of the solve).
     d.hdi.assign((size_t)B, DImg{});
@@ -1202,15 +1247,13 @@ int build(Deriv &d, hipStream_t s) {
     hipLaunchKernelGGL(union_kernel, dim3(gpx), dim3(kThr), 0, s, G, (int)sp.iso, d.di, d.flat, d.parent);
     hipLaunchKernelGGL(compress_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.parent, d.isroot);
     hipLaunchKernelGGL(finish_roots_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.parent);
-    // scratch for the scan and the sort
-    size_t b1 = 0, b2 = 0;
-    void *scan_tmp = nullptr;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.isroot, d.rank, (int)BN, s)) return FP_EDEVICE;
-    if ((e = d.alloc(&scan_tmp, b1))) return FP_ENOMEM;
-    if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, b1, d.isroot, d.rank, (int)BN, s)) return FP_EDEVICE;
+    // the scan and the sort in their reserved scratch
+    size_t b1 = d.scan_bytes, b2 = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(d.scan_tmp, b1, d.isroot, d.rank, (int)BN, s)) return FP_EDEVICE;
     hipLaunchKernelGGL(label_kernel, dim3(gpx), dim3(kThr), 0, s, G, d.parent, d.rank, d.isroot, d.label, d.iota,
                        d.rbase);
-    std::vector<int> rb((size_t)B + 1);
+    std::vector<int> &rb = d.hrb;
+    rb.assign((size_t)B + 1, 0);
     if (hipMemcpyAsync(rb.data(), d.rbase, 4 * ((size_t)B + 1), hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
     if (hipStreamSynchronize(s)) return FP_EDEVICE;
     d.nc = rb[(size_t)B];
@@ -1218,14 +1261,15 @@ int build(Deriv &d, hipStream_t s) {
     while (bits < 31 && (1LL << bits) <= (long long)d.nc) ++bits;
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, b2, d.label, d.skey, d.iota, d.members, (int)BN, 0, bits, s))
         return FP_EDEVICE;
-    d.cub_bytes = b2;
-    if ((e = d.alloc(&d.cub, d.cub_bytes))) return FP_ENOMEM;
+    if (b2 > d.cub_bytes) return FP_EINTERNAL;
+    b2 = d.cub_bytes;
     if (hipcub::DeviceRadixSort::SortPairs(d.cub, b2, d.label, d.skey, d.iota, d.members, (int)BN, 0, bits, s))
         return FP_EDEVICE;
     hipLaunchKernelGGL(rstart_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.skey, d.rstart);
     hipLaunchKernelGGL(inv_kernel, dim3(grid_of(d.nc)), dim3(kThr), 0, s, d.nc, d.rstart, d.inv);
     // region blocks per image
-    std::vector<RBlock> blocks;
+    std::vector<RBlock> &blocks = d.hblocks;
+    blocks.clear();
     for (int b = 0; b < B; ++b) {
         DImg &D = d.hdi[(size_t)b];
         D.rbase = rb[(size_t)b];
@@ -1236,12 +1280,12 @@ int build(Deriv &d, hipStream_t s) {
         D.bcount = (int)blocks.size() - D.bbase;
     }
     d.nrb = (int)blocks.size();
-    if ((e = d.alloc(&d.rb, sizeof(RBlock) * blocks.size()))) return FP_ENOMEM;
-    if ((e = d.alloc(&d.part, 8 * 3 * blocks.size()))) return FP_ENOMEM;
+    if (d.nrb > d.max_rb) return FP_EINTERNAL;
     if (hipMemcpyAsync(d.rb, blocks.data(), sizeof(RBlock) * blocks.size(), hipMemcpyHostToDevice, s))
         return FP_EDEVICE;
     // the region fields of the device copy (after prep's fields are final)
-    std::vector<DImg> dev((size_t)B);
+    std::vector<DImg> &dev = d.hdi2;
+    dev.assign((size_t)B, DImg{});
     if (hipMemcpyAsync(dev.data(), d.di, sizeof(DImg) * (size_t)B, hipMemcpyDeviceToHost, s)) return FP_EDEVICE;
     if (hipStreamSynchronize(s)) return FP_EDEVICE;
     for (int b = 0; b < B; ++b) {
@@ -1251,7 +1295,7 @@ int build(Deriv &d, hipStream_t s) {
         D.bbase = d.hdi[(size_t)b].bbase;
         D.bcount = d.hdi[(size_t)b].bcount;
     }
-    d.hdi = dev;
+    d.hdi.swap(dev);
     if (hipMemcpyAsync(d.di, d.hdi.data(), sizeof(DImg) * (size_t)B, hipMemcpyHostToDevice, s)) return FP_EDEVICE;
     if (sp.iso) {
         // the Jacobi diagonal, once per structure
@@ -1390,11 +1434,9 @@ int jvp_t(Deriv &d, hipStream_t s, const void *dx, const double *dgamma, const d
 
 }  // namespace
 
-int deriv_create(const DerivSpec &spec, hipStream_t s, Deriv **out) {
+int deriv_reserve(const DerivSpec &spec, Deriv **out) {
     *out = nullptr;
-    if (spec.B <= 0 || (spec.k != 2 && spec.k != 3) || !spec.x || !spec.z || !spec.gamma) return FP_EINVAL;
-    if (spec.iso && !spec.gap) return FP_EINVAL;
-    if (!spec.u) return FP_EINVAL;
+    if (spec.B <= 0 || (spec.k != 2 && spec.k != 3)) return FP_EINVAL;
     Deriv *d = new (std::nothrow) Deriv();
     if (!d) return FP_ENOMEM;
     d->sp = spec;
@@ -1410,8 +1452,34 @@ int deriv_create(const DerivSpec &spec, hipStream_t s, Deriv **out) {
         delete d;
         return FP_ESHAPE;
     }
-    const int r = spec.tsize == 4 ? build<float>(*d, s) : build<double>(*d, s);
-    if (r) {
+    if (int r = reserve(*d)) {
+        delete d;
+        return r;
+    }
+    *out = d;
+    return FP_OK;
+}
+
+int deriv_build(Deriv *d, const DerivSpec &spec, hipStream_t s) {
+    if (!d) return FP_EINVAL;
+    if (!spec.x || !spec.z || !spec.gamma || !spec.u || (spec.iso && !spec.gap)) return FP_EINVAL;
+    const DerivSpec &r = d->sp;
+    // the solution must have the reserved shape
+    if (spec.B != r.B || spec.k != r.k || spec.iso != r.iso || spec.tsize != r.tsize || !spec.sw != !r.sw ||
+        (spec.sw && (spec.sw_img == 0) != (r.sw_img == 0)))
+        return FP_EINVAL;
+    for (int a = 0; a < 3; ++a)
+        if (spec.L[a] != r.L[a]) return FP_EINVAL;
+    d->sp = spec;
+    return spec.tsize == 4 ? build<float>(*d, s) : build<double>(*d, s);
+}
+
+int deriv_create(const DerivSpec &spec, hipStream_t s, Deriv **out) {
+    *out = nullptr;
+    if (!spec.x || !spec.z || !spec.gamma || !spec.u || (spec.iso && !spec.gap)) return FP_EINVAL;
+    Deriv *d = nullptr;
+    if (int r = deriv_reserve(spec, &d)) return r;
+    if (int r = deriv_build(d, spec, s)) {
         delete d;
         return r;
     }
@@ -1615,8 +1683,7 @@ template <class T>
 int apply_branch(Deriv &d, hipStream_t s, const T *z, const T *u, const T *sw) {
     const Geo G = d.G;
     const DerivSpec &sp = d.sp;
-    int8_t *km = nullptr;
-    if (d.alloc(&km, (size_t)G.BN)) return FP_ENOMEM;
+    int8_t *km = d.km;
     const int gpx = grid_of(G.BN);
     hipLaunchKernelGGL(zero_bytes_kernel, dim3(gpx), dim3(kThr), 0, s, G.BN, km);
     for (int a = 0; a < G.k; ++a) {
@@ -1639,13 +1706,11 @@ int deriv_regions(Deriv *d, hipStream_t s, void *labels, bool lab64, int8_t *sig
                            labels, lab64 ? 1 : 0, sign);
     if (nregions) {
         const int B = (int)d->sp.B;
-        int64_t *dev = nullptr;
-        if (hipMalloc((void **)&dev, sizeof(int64_t) * (size_t)B)) return FP_ENOMEM;
+        int64_t *dev = d->nreg;
         hipLaunchKernelGGL(region_counts_kernel, dim3((B + 255) / 256), dim3(256), 0, s, B, d->di, d->rbase, dev);
-        const bool bad = hipMemcpyAsync(nregions, dev, sizeof(int64_t) * (size_t)B, hipMemcpyDeviceToHost, s) ||
-                         hipStreamSynchronize(s);
-        (void)hipFree(dev);
-        if (bad) return FP_EDEVICE;
+        if (hipMemcpyAsync(nregions, dev, sizeof(int64_t) * (size_t)B, hipMemcpyDeviceToHost, s) ||
+            hipStreamSynchronize(s))
+            return FP_EDEVICE;
     }
     return hipGetLastError() ? FP_EDEVICE : FP_OK;
 }

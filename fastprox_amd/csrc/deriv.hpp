@@ -48,7 +48,14 @@ struct DerivSpec {
     int64_t q_img = 0, q_plane = 0;
 };
 
-// Builds the structure (one host synchronisation); an fp_status.
+// Allocates every buffer of the structure of B images of spec's shape (the
+// shape fields: iso, k, L, B, tsize, whether spatial weights are given and
+// whether they are shared); no later call on it allocates. An fp_status.
+int deriv_reserve(const DerivSpec &spec, Deriv **out);
+// Builds the structure of the solution spec (of the reserved shape) into the
+// reserved buffers (one host synchronisation); an fp_status.
+int deriv_build(Deriv *d, const DerivSpec &spec, hipStream_t s);
+// deriv_reserve, then deriv_build.
 int deriv_create(const DerivSpec &spec, hipStream_t s, Deriv **out);
 void deriv_free(Deriv *d);
 

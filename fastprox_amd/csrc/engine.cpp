@@ -1,6 +1,7 @@
 // The solve engine (engine.hpp).
 //
 // SPDX-License-Identifier: MIT
+#include "alloc.hpp"
 #include "engine.hpp"
 
 #include <algorithm>
@@ -186,21 +187,21 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
     // the streaming 2D kernel's 32-bit plane offsets
     if (kc.stream && !kc.generic && !stream_fits(tsize, maxN_)) return hipErrorInvalidValue;
     hipError_t e;
-    if ((e = hipMalloc(&arena_, (size_t)std::max<int64_t>(arena, 1) * (size_t)tsize))) return e;
-    if ((e = hipMalloc((void **)&desc_, sizeof(ImgDesc) * (size_t)std::max(B, 1)))) return e;
-    if ((e = hipMalloc((void **)&state_, sizeof(ImgState) * (size_t)std::max(B, 1)))) return e;
-    if ((e = hipMalloc((void **)&part_, sizeof(double) * kSlots * (size_t)std::max(tiles, 1)))) return e;
+    if ((e = fpr::dev_alloc(&arena_, (size_t)std::max<int64_t>(arena, 1) * (size_t)tsize))) return e;
+    if ((e = fpr::dev_alloc((void **)&desc_, sizeof(ImgDesc) * (size_t)std::max(B, 1)))) return e;
+    if ((e = fpr::dev_alloc((void **)&state_, sizeof(ImgState) * (size_t)std::max(B, 1)))) return e;
+    if ((e = fpr::dev_alloc((void **)&part_, sizeof(double) * kSlots * (size_t)std::max(tiles, 1)))) return e;
     for (int i = 0; i < 2; ++i)
-        if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)std::max(tiles, 1)))) return e;
-    if ((e = hipMalloc((void **)&counts_, 64))) return e;
-    if ((e = hipMalloc((void **)&gam_, sizeof(double) * (size_t)std::max(B, 1)))) return e;
-    if ((e = hipMalloc((void **)&propart_,
+        if ((e = fpr::dev_alloc((void **)&list_[i], sizeof(Tile) * (size_t)std::max(tiles, 1)))) return e;
+    if ((e = fpr::dev_alloc((void **)&counts_, 64))) return e;
+    if ((e = fpr::dev_alloc((void **)&gam_, sizeof(double) * (size_t)std::max(B, 1)))) return e;
+    if ((e = fpr::dev_alloc((void **)&propart_,
                        sizeof(double) * 4 * (size_t)std::max(B, 1) * (size_t)prologue_blocks(std::max<int64_t>(maxN_, 1)))))
         return e;
-    if ((e = hipHostMalloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(Ball, 1), 0))) return e;
-    if ((e = hipHostMalloc((void **)&hdesc_pinned_, sizeof(ImgDesc) * (size_t)std::max(B, 1), 0))) return e;
-    if ((e = hipHostMalloc((void **)&hcount_, 64, 0))) return e;
-    if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)std::max(B, 1), 0))) return e;
+    if ((e = fpr::pinned_alloc((void **)&hstate_, sizeof(ImgState) * (size_t)std::max(Ball, 1)))) return e;
+    if ((e = fpr::pinned_alloc((void **)&hdesc_pinned_, sizeof(ImgDesc) * (size_t)std::max(B, 1)))) return e;
+    if ((e = fpr::pinned_alloc((void **)&hcount_, 64))) return e;
+    if ((e = fpr::pinned_alloc((void **)&hgam_, sizeof(double) * (size_t)std::max(B, 1)))) return e;
     return hipSuccess;
 }
 
@@ -234,16 +235,16 @@ int Engine::set_tail(const KernelChoice &kc, double frac) {
     hipError_t e;
     if (!desc_tail_) {
         const size_t B = std::max<size_t>(hdesc_.size(), 1);
-        if ((e = hipMalloc((void **)&desc_tail_, sizeof(ImgDesc) * B))) return e;
-        if ((e = hipHostMalloc((void **)&hdesc_tail_pinned_, sizeof(ImgDesc) * B, 0))) return e;
+        if ((e = fpr::dev_alloc((void **)&desc_tail_, sizeof(ImgDesc) * B))) return e;
+        if ((e = fpr::pinned_alloc((void **)&hdesc_tail_pinned_, sizeof(ImgDesc) * B))) return e;
     }
     if (tiles > tiles_cap_) {  // the partials and the lists hold either tiling
         for (void *p : {(void *)part_, (void *)list_[0], (void *)list_[1]}) (void)hipFree(p);
         part_ = nullptr;
         list_[0] = list_[1] = nullptr;
-        if ((e = hipMalloc((void **)&part_, sizeof(double) * kSlots * (size_t)tiles))) return e;
+        if ((e = fpr::dev_alloc((void **)&part_, sizeof(double) * kSlots * (size_t)tiles))) return e;
         for (int i = 0; i < 2; ++i)
-            if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)tiles))) return e;
+            if ((e = fpr::dev_alloc((void **)&list_[i], sizeof(Tile) * (size_t)tiles))) return e;
         tiles_cap_ = tiles;
     }
     tail_kc_ = kc;

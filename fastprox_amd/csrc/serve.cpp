@@ -25,6 +25,7 @@
 // it is the same alone and in any mix.
 //
 // SPDX-License-Identifier: MIT
+#include "alloc.hpp"
 #include "serve.hpp"
 
 #include <algorithm>
@@ -120,26 +121,26 @@ int SlotPool::setup(const Problem &P, const KernelChoice &kc, int tsize, int slo
     tiles_.reset(tile_cap);
     hipError_t e;
     if ((e = hipStreamCreate(&stream))) return e;
-    if ((e = hipMalloc(&pool_, (size_t)pool_elems * (size_t)tsize))) return e;
-    if ((e = hipMalloc((void **)&desc_, sizeof(ImgDesc) * (size_t)slots))) return e;
-    if ((e = hipMalloc((void **)&state_, sizeof(ImgState) * (size_t)slots))) return e;
-    if ((e = hipMalloc((void **)&part_, sizeof(double) * kSlots * (size_t)tile_cap))) return e;
-    if ((e = hipMalloc((void **)&gam_, sizeof(double) * (size_t)slots))) return e;
-    if ((e = hipMalloc((void **)&propart_, sizeof(double) * 4 * (size_t)prologue_blocks(max_pixels)))) return e;
+    if ((e = fpr::dev_alloc(&pool_, (size_t)pool_elems * (size_t)tsize))) return e;
+    if ((e = fpr::dev_alloc((void **)&desc_, sizeof(ImgDesc) * (size_t)slots))) return e;
+    if ((e = fpr::dev_alloc((void **)&state_, sizeof(ImgState) * (size_t)slots))) return e;
+    if ((e = fpr::dev_alloc((void **)&part_, sizeof(double) * kSlots * (size_t)tile_cap))) return e;
+    if ((e = fpr::dev_alloc((void **)&gam_, sizeof(double) * (size_t)slots))) return e;
+    if ((e = fpr::dev_alloc((void **)&propart_, sizeof(double) * 4 * (size_t)prologue_blocks(max_pixels)))) return e;
     for (int i = 0; i < 2; ++i)
-        if ((e = hipMalloc((void **)&list_[i], sizeof(Tile) * (size_t)tile_cap))) return e;
-    if ((e = hipMalloc((void **)&counts_, 64))) return e;
-    if ((e = hipMalloc((void **)&dstage_, sizeof(Stage)))) return e;
-    if ((e = hipMalloc((void **)&sstate_, sizeof(ImgState) * kGroupMax))) return e;
-    if ((e = hipMalloc((void **)&gpart_, sizeof(double) * 4 * (size_t)kGroupBlocks))) return e;
+        if ((e = fpr::dev_alloc((void **)&list_[i], sizeof(Tile) * (size_t)tile_cap))) return e;
+    if ((e = fpr::dev_alloc((void **)&counts_, 64))) return e;
+    if ((e = fpr::dev_alloc((void **)&dstage_, sizeof(Stage)))) return e;
+    if ((e = fpr::dev_alloc((void **)&sstate_, sizeof(ImgState) * kGroupMax))) return e;
+    if ((e = fpr::dev_alloc((void **)&gpart_, sizeof(double) * 4 * (size_t)kGroupBlocks))) return e;
     for (int i = 0; i < 2; ++i) {
-        if ((e = hipHostMalloc((void **)&hstage_[i], sizeof(Stage), 0))) return e;
+        if ((e = fpr::pinned_alloc((void **)&hstage_[i], sizeof(Stage)))) return e;
         if ((e = hipEventCreateWithFlags(&stage_ev_[i], hipEventDisableTiming))) return e;
     }
-    if ((e = hipHostMalloc((void **)&hdesc_, sizeof(ImgDesc) * (size_t)slots, 0))) return e;
-    if ((e = hipHostMalloc((void **)&hgam_, sizeof(double) * (size_t)slots, 0))) return e;
+    if ((e = fpr::pinned_alloc((void **)&hdesc_, sizeof(ImgDesc) * (size_t)slots))) return e;
+    if ((e = fpr::pinned_alloc((void **)&hgam_, sizeof(double) * (size_t)slots))) return e;
     for (int i = 0; i < 2; ++i)
-        if ((e = hipHostMalloc((void **)&hsnap_[i], sizeof(ImgState) * (size_t)slots, 0))) return e;
+        if ((e = fpr::pinned_alloc((void **)&hsnap_[i], sizeof(ImgState) * (size_t)slots))) return e;
     // every slot starts finished (mode kDone = 0); both lists empty
     if ((e = hipMemsetAsync(state_, 0, sizeof(ImgState) * (size_t)slots, stream))) return e;
     if ((e = hipMemsetAsync(desc_, 0, sizeof(ImgDesc) * (size_t)slots, stream))) return e;

@@ -1944,6 +1944,26 @@ __global__ __launch_bounds__(64) void admit_commit_kernel(const ImgDesc *sdesc, 
     }
 }
 
+// src (contiguous, C order) converted into dst with its strides.
+template <class S, class D>
+__global__ void convert_scatter_kernel(const S *src, D *dst, int64_t n, int nd, StridedShape sh) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = i, off = 0;
+        for (int d = nd - 1; d >= 0; --d) {
+            const int64_t c = r % sh.shape[d];
+            r /= sh.shape[d];
+            off += c * sh.stride[d];
+        }
+        dst[off] = (D)src[i];
+    }
+}
+
+// Zeros n bytes.
+__global__ void zero_bytes_kernel(unsigned char *p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 template <class S, class D>
 __global__ void convert_kernel(const S *src, D *dst, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -2129,6 +2149,15 @@ template <class S, class D>
 void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL((convert_kernel<S, D>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n);
 }
+template <class S, class D>
+void launch_convert_scatter(const S *src, D *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s) {
+    hipLaunchKernelGGL((convert_scatter_kernel<S, D>), dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, src, dst, n, nd,
+                       sh);
+}
+void launch_zero_bytes(void *p, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_bytes_kernel, dim3(grid_for((int64_t)n, 256, 8192)), dim3(256), 0, s, (unsigned char *)p,
+                       n);
+}
 
 #define FPR_INST(T)                                                                                              \
     template void launch_prologue<T>(const Problem &, const ImgDesc *, ImgState *, const double *, int, double *, \
@@ -2155,5 +2184,9 @@ template void launch_convert<float, double>(const float *, double *, int64_t, hi
 template void launch_convert<double, float>(const double *, float *, int64_t, hipStream_t);
 template void launch_convert<float, float>(const float *, float *, int64_t, hipStream_t);
 template void launch_convert<double, double>(const double *, double *, int64_t, hipStream_t);
+template void launch_convert_scatter<float, double>(const float *, double *, int64_t, int, const StridedShape &,
+                                                    hipStream_t);
+template void launch_convert_scatter<double, float>(const double *, float *, int64_t, int, const StridedShape &,
+                                                    hipStream_t);
 
 }  // namespace fpr

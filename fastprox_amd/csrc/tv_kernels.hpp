@@ -92,6 +92,9 @@ void launch_copy_u(const T *arena, const ImgDesc *desc, const ImgState *state, i
                    int64_t img_stride, hipStream_t s);
 template <class S, class D>
 void launch_convert(const S *src, D *dst, int64_t n, hipStream_t s);
+template <class S, class D>
+void launch_convert_scatter(const S *src, D *dst, int64_t n, int nd, const StridedShape &sh, hipStream_t s);
+void launch_zero_bytes(void *p, size_t n, hipStream_t s);
 template <class T>
 void launch_gather_group(const GatherEntry *tab, int G, int64_t maxN, hipStream_t s);
 void launch_admit_commit(const ImgDesc *sdesc, const ImgState *sstate, const double *sgam, const int *map,

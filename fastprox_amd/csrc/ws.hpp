@@ -11,6 +11,7 @@
 #include <mutex>
 #include <vector>
 
+#include "alloc.hpp"
 #include "../../include/fastprox_rocm.h"
 #include "deriv.hpp"
 #include "engine.hpp"
@@ -51,7 +52,7 @@ struct fp_op {
             if (p.first == dev) return p.second;
         void *ptr = nullptr;
         const size_t n = spatial.size();
-        if ((err = hipMalloc(&ptr, n * (size_t)tsize))) return nullptr;
+        if ((err = fpr::dev_alloc(&ptr, n * (size_t)tsize))) return nullptr;
         if (tsize == 4) {
             std::vector<float> f(spatial.begin(), spatial.end());
             err = hipMemcpy(ptr, f.data(), n * 4, hipMemcpyHostToDevice);
@@ -105,16 +106,21 @@ struct fp_workspace {
     int deriv_max_iter = 10000;
     double deriv_res = 0.0;
     int deriv_its = 0;
+    // keep_structure: the derivative structure's buffers, reserved at the
+    // workspace's creation and rebuilt into at the first derivative or query
+    // after each kept call (deriv_built)
     std::unique_ptr<fpr::Deriv, fpr::DerivDeleter> deriv;
+    bool deriv_built = false;
     double *dscr = nullptr;  // device doubles for the derivatives' small arrays
     size_t dscr_n = 0;
+    void *qbuf = nullptr;    // keep_structure: the queries' device output (9 numel bytes)
     // fp_workspace_set_branch (aniso): the side array (device int8, numel,
     // C order) and its kink tolerance, until the next prox call
     int8_t *branch = nullptr;
     bool branch_on = false;
     double branch_tol = 0.0;
     ~fp_workspace() {
-        for (void *p : {dstage, zstage, q, xsave, zsave, usave, (void *)dscr, (void *)branch})
+        for (void *p : {dstage, zstage, q, xsave, zsave, usave, (void *)dscr, (void *)branch, qbuf})
             if (p) (void)hipFree(p);
     }
 };

@@ -219,6 +219,18 @@ FP_API const fpr_plugin_table *fastprox_rocm_plugin(void);
 /* A short description of why the last failing fpr_ call on this thread failed. */
 FP_API const char *fpr_last_error(void);
 
+/* The number of device and pinned host allocations (hipMalloc,
+ * hipHostMalloc) the library has made in this process. Every device buffer is
+ * allocated by a create call (fp_workspace_create; with keep_structure also
+ * the derivative structure, its scratch and the queries' buffers;
+ * fpr_batch_create, fpr_server_create, fpr_deriv_create): fp_prox,
+ * fp_prox_steps, fp_jvp, fp_vjp, fp_vjp_device, fp_workspace_refine, the
+ * dual calls, the structure queries and fp_workspace_set_branch leave the
+ * count unchanged (the reference's device rule, docs/spec.md:887, and its
+ * allocation counter, tests/c/test_core.cpp:40-67). No counterpart in the
+ * reference's C ABI. */
+FP_API int64_t fpr_alloc_count(void);
+
 /* The step kernel of a workspace: generic = 1 selects the one-iteration
  * kernel; otherwise the register-tile kernel with tt iterations per launch
  * and ry rows per tile (FP_EINVAL for a combination not built). The default
