@@ -62,8 +62,10 @@ def test_no_allocation_after_workspace_create(cuda, case):
     P = op.nparams
     gp = torch.zeros(P, dtype=torch.float64, device=cuda)
     torch.cuda.synchronize()
+    before = _count()
     ws = Workspace(op, x, keep_structure=True)
     n0 = _count()
+    assert n0 > before  # the counter sees the workspace's buffers
     print(f"{case}: {n0} allocations after create", flush=True)
     for rep in range(2):
         ws.prox(x, 0.15, out, tol=1e-8, max_iter=100000)
