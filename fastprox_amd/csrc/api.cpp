@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <stdexcept>
 #include <vector>
 
 #include "alloc.hpp"
@@ -51,14 +52,28 @@ fp_status fail(fp_status s, const char *why) {
 constexpr double kMaxScaledStep = 0x1p960;  // the step range (op.hpp:48)
 constexpr int64_t kMaxElems = (int64_t(1) << 60) - 1;
 
+// The status for the exception being handled: out of memory (a failed
+// allocation, or a container asked to outgrow its maximum size) or an
+// internal error. No exception leaves an entry point (the reference's rule,
+// src/api/api.cpp:28-38).
+fp_status status_of_exception() noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return FP_ENOMEM;
+    } catch (const std::length_error &) {
+        return FP_ENOMEM;
+    } catch (...) {
+        return FP_EINTERNAL;
+    }
+}
+
 template <class F>
 fp_status guard(F &&f) noexcept {
     try {
         return f();
-    } catch (const std::bad_alloc &) {
-        return FP_ENOMEM;
     } catch (...) {
-        return FP_EINTERNAL;
+        return status_of_exception();
     }
 }
 
@@ -69,18 +84,18 @@ fp_status hip_status(int e) {
     return FP_EDEVICE;
 }
 
+// The int a C caller stored in an enum field or argument, read from its
+// bytes: a C enum may hold any int, while a C++ read of an enum object out of
+// its enumeration's range is undefined.
 template <class E>
 int enum_bits(const E &e) noexcept {
-    int v;
-    std::memcpy(&v, &e, sizeof v);
-    return v;
+    static_assert(sizeof(E) == sizeof(int32_t), "the C enums of the ABI are 32-bit");
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(&e);
+    uint32_t v = 0;
+    for (int i = (int)sizeof(E) - 1; i >= 0; --i) v = (v << 8) | b[i];  // little endian
+    return (int)v;
 }
 
-bool mul_bounded(int64_t a, int64_t b, int64_t &out) {
-    if (a != 0 && b > kMaxElems / a) return false;
-    out = a * b;
-    return out <= kMaxElems;
-}
 
 struct Arr {
     char *data = nullptr;
@@ -117,49 +132,60 @@ struct Arr {
     }
 };
 
-// The reference's to_arr (api.cpp:60-108), with FP_DEVICE_ROCM accepted.
+bool dtype_allowed(int dt, bool floating) {
+    return floating ? (dt == FP_F32 || dt == FP_F64) : (dt == FP_I8 || dt == FP_I32 || dt == FP_I64);
+}
+
+// An fp_array descriptor checked against the boundary's rules and converted
+// (the rules of the reference's descriptor check, src/api/api.cpp:60-108,
+// with FP_DEVICE_ROCM accepted besides the host):
+//   the device is the host or a ROCm device (else FP_EINVAL); the dtype is a
+//   float one (floating) or an integer one (FP_EDTYPE); 0 <= ndim <=
+//   FP_MAX_NDIM (FP_ESHAPE), a 0-d array being one entry of shape [1]; shape
+//   is given when ndim > 0 (FP_EINVAL); no extent is negative, the product of
+//   the extents above 1 and the offset span sum_d (L_d - 1) |s_d| stay within
+//   kMaxElems, as does every stride along an extent above 1 (FP_ESHAPE);
+//   NULL strides mean C order; data is given when there are entries
+//   (FP_EINVAL).
+// The bounds are checked in 128-bit arithmetic, one axis at a time from the
+// last, so that no product can wrap before it is compared.
 fp_status to_arr(const fp_array *a, Arr &out, bool floating = true) noexcept {
     if (!a) return FP_EINVAL;
     const int dev = enum_bits(a->device);
     if (dev != FP_DEVICE_CPU && dev != FP_DEVICE_ROCM) return FP_EINVAL;
     const int dt = enum_bits(a->dtype);
-    if (floating ? !(dt == FP_F32 || dt == FP_F64) : !(dt == FP_I8 || dt == FP_I32 || dt == FP_I64)) return FP_EDTYPE;
+    if (!dtype_allowed(dt, floating)) return FP_EDTYPE;
     if (a->ndim < 0 || a->ndim > FP_MAX_NDIM) return FP_ESHAPE;
-    out.data = static_cast<char *>(a->data);
-    out.dtype = dt;
-    out.device = dev;
+    if (a->ndim > 0 && !a->shape) return FP_EINVAL;
+    Arr r;
+    r.data = static_cast<char *>(a->data);
+    r.dtype = dt;
+    r.device = dev;
     if (a->ndim == 0) {
-        out.ndim = 1;
-        out.shape[0] = 1;
-        out.strides[0] = 1;
+        r.ndim = 1;
+        r.shape[0] = 1;
+        r.strides[0] = 1;
     } else {
-        if (!a->shape) return FP_EINVAL;
-        out.ndim = a->ndim;
-        int64_t count = 1;
-        for (int d = 0; d < a->ndim; ++d) {
-            if (a->shape[d] < 0) return FP_ESHAPE;
-            out.shape[d] = a->shape[d];
-            out.strides[d] = a->strides ? a->strides[d] : 0;
-            if (a->shape[d] > 1 && !mul_bounded(count, a->shape[d], count)) return FP_ESHAPE;
-        }
-        if (!a->strides) {
-            int64_t s = 1;
-            for (int d = a->ndim - 1; d >= 0; --d) {
-                out.strides[d] = s;
-                s *= out.shape[d];
+        r.ndim = a->ndim;
+        const __int128 cap = kMaxElems;
+        __int128 entries = 1, span = 0, c_stride = 1;
+        for (int d = a->ndim - 1; d >= 0; --d) {
+            const int64_t n = a->shape[d];
+            if (n < 0) return FP_ESHAPE;
+            const int64_t st = a->strides ? a->strides[d] : (int64_t)c_stride;
+            r.shape[d] = n;
+            r.strides[d] = st;
+            if (n > 1) {
+                const __int128 mag = st < 0 ? -(__int128)st : (__int128)st;
+                entries *= n;
+                span += (__int128)(n - 1) * mag;
+                if (entries > cap || mag > cap || span > cap) return FP_ESHAPE;
             }
-        }
-        int64_t span = 0;
-        for (int d = 0; d < a->ndim; ++d) {
-            if (out.shape[d] < 2) continue;
-            const int64_t st = out.strides[d];
-            if (st < -kMaxElems || st > kMaxElems) return FP_ESHAPE;
-            int64_t e;
-            if (!mul_bounded(out.shape[d] - 1, st < 0 ? -st : st, e) || e > kMaxElems - span) return FP_ESHAPE;
-            span += e;
+            c_stride *= n;  // at most the entries so far, or 0
         }
     }
-    if (!out.data && out.numel() > 0) return FP_EINVAL;
+    if (!r.data && r.numel() > 0) return FP_EINVAL;
+    out = r;
     return FP_OK;
 }
 
@@ -299,10 +325,14 @@ fp_status check_image(const fp_op &op, const int64_t *ishape) {
     return n < 2147483647.0 ? FP_OK : FP_ESHAPE;
 }
 
+// The step range: gamma times the operator's largest weight (axis weight
+// times spatial weight) at most 2^960, so that every rescaled radius is
+// finite (the reference's range, op.hpp:48). Signs and finiteness are checked
+// before, with the other arguments.
 fp_status step_ok(const fp_op &op, double g) {
-    const double s = op.step_scale();
-    if (s > 0.0 && !(g * s <= kMaxScaledStep)) return FP_EPARAM;
-    return FP_OK;
+    const double largest = op.step_scale();
+    const bool in_range = largest == 0.0 || g * largest <= kMaxScaledStep;
+    return in_range ? FP_OK : FP_EPARAM;
 }
 
 fpr::StridedShape strided_of(const Arr &a) {

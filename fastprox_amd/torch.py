@@ -57,26 +57,37 @@ def inner_accuracy(target, factor=0.1):
 
 
 def _sq_tol(ztol):
-    ztol = float(ztol)
-    if not 0.0 <= ztol < float("inf"):
-        raise ValueError(f"ztol must be finite and >= 0, got {ztol}")
-    if ztol > 0.0 and ztol * ztol < np.finfo(np.float64).tiny:
-        raise ValueError(f"ztol={ztol} is below 1e-154: its square, the gap tolerance, underflows")
-    return ztol * ztol
+    """The gap tolerance that certifies a ztol: the gap bound |z - z*|^2 <=
+    2 gap with gap <= tol |x|^2 / 2 gives |z - z*| <= sqrt(tol) |x|, so ztol
+    asks for tol = ztol^2, which must not underflow."""
+    v = float(ztol)
+    if not (v >= 0.0 and math.isfinite(v)):
+        raise ValueError(f"ztol must be finite and >= 0, got {v}")
+    sq = v * v
+    if v > 0.0 and sq < np.finfo(np.float64).tiny:
+        raise ValueError(f"ztol={v} is below 1e-154: its square, the gap tolerance, underflows")
+    return sq
 
 
 def _call_tol(tol, ztol, default, warn=True):
-    """__init__.py:663-679: the call's tol or ztol, else inner_accuracy's ztol, else the default."""
+    """The gap tolerance of one call (the reference's precedence,
+    bindings/python/fastprox/__init__.py:663-679): its own tol, or its ztol
+    squared; without either, the enclosing inner_accuracy block's ztol
+    squared; outside a block, the default. An own tolerance looser than the
+    block's draws an AccuracyWarning."""
     if tol is not None and ztol is not None:
         raise ValueError("pass tol or ztol, not both")
-    need = _ACCURACY.get()
-    if tol is None and ztol is None:
-        return default if need is None else _sq_tol(need)
-    t = float(tol) if ztol is None else _sq_tol(ztol)
-    if need is not None and warn and t > _sq_tol(need):
-        own = f"tol={float(tol):g}" if ztol is None else f"ztol={float(ztol):g}"
-        warnings.warn(f"{own} certifies less than inner_accuracy's ztol={need:g}", AccuracyWarning, stacklevel=4)
-    return t
+    block = _ACCURACY.get()
+    if tol is not None:
+        own, label = float(tol), f"tol={float(tol):g}"
+    elif ztol is not None:
+        own, label = _sq_tol(ztol), f"ztol={float(ztol):g}"
+    else:
+        return default if block is None else _sq_tol(block)
+    if warn and block is not None and own > _sq_tol(block):
+        warnings.warn(f"{label} certifies less than inner_accuracy's ztol={block:g}", AccuracyWarning,
+                      stacklevel=4)
+    return own
 
 
 def _check_floor(x, tol, ztol, warn):
@@ -117,6 +128,9 @@ def tv_aniso(x, gamma=1.0, *, ndim=2, weights=None, edge_weights=None, tol=None,
 
 
 class _Cfg:
+    """The fixed settings of one call, shared by its forward and its
+    derivatives (which record their CG residual and iterations in deriv)."""
+
     def __init__(self, iso, k, w, tol, max_iter, deriv_tol, deriv_max_iter):
         self.iso, self.k, self.w = iso, k, w
         self.tol, self.max_iter = float(tol), int(max_iter)
@@ -125,32 +139,106 @@ class _Cfg:
         self.deriv = None
 
 
-def _device_tv(iso, x, gamma, ndim, weights, tol, max_iter, warm_start, return_info, deriv_tol, deriv_max_iter,
-               spatial, warn):
+def _broadcasts(src, dst):
+    """Whether shape src broadcasts to shape dst: src is no longer than dst,
+    and each of its trailing extents equals dst's or is 1."""
+    if len(src) > len(dst):
+        return False
+    return all(a == b or a == 1 for a, b in zip(src[::-1], dst[::-1]))
+
+
+def _as_float(v, x):
+    """v (a number, sequence, array or tensor) as a tensor of x's dtype on
+    x's device; a tensor stays in the autograd graph."""
+    t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, np.float64))
+    return t.to(dtype=x.dtype, device=x.device)
+
+
+def _require_finite_nonneg(t, name):
+    d = t.detach()
+    if not bool((torch.isfinite(d) & (d >= 0)).all()):
+        raise ValueError(f"{name} must be finite and >= 0")
+
+
+def _check_input(x, k, tol, max_iter):
+    """The image tensor and the solve's limits (checked in the reference's
+    order, with its messages)."""
     if not isinstance(x, torch.Tensor):
         raise TypeError("x must be a torch.Tensor")
     if x.dtype not in (torch.float32, torch.float64):
         raise TypeError(f"fastprox operates on float32 and float64 tensors, got {x.dtype}")
-    if ndim not in (2, 3):
+    if k not in (2, 3):
         raise ValueError("ndim must be 2 or 3")
-    if x.dim() < ndim:
-        raise ValueError(f"needs a tensor with at least {ndim} dimensions")
-    if not tol >= 0 or max_iter < 0:
+    if x.dim() < k:
+        raise ValueError(f"needs a tensor with at least {k} dimensions")
+    if not (tol >= 0 and max_iter >= 0):
         raise ValueError("tol must be >= 0 and max_iter >= 0")
     if not x.is_cuda:
         raise ValueError("fastprox_amd runs on the GPU: x must be a cuda (ROCm) tensor")
-    wt = None
+
+
+def _axis_weights(weights, k):
+    """(the k axis weights as floats, the weights tensor when it is a float
+    tensor the gradient flows to, else None). One value is every axis's."""
     if weights is None:
-        w = [1.0] * ndim
-    elif isinstance(weights, torch.Tensor) and weights.is_floating_point():
-        wt = weights
-        w = [float(a) for a in weights.detach().cpu().reshape(-1)]
-    else:
-        w = [float(a) for a in np.atleast_1d(np.asarray(weights, np.float64))]
-    if len(w) == 1:
-        w = w * ndim
-    if len(w) != ndim or min(w) < 0 or not all(math.isfinite(a) for a in w):
+        return [1.0] * k, None
+    wt = weights if isinstance(weights, torch.Tensor) and weights.is_floating_point() else None
+    raw = weights.detach().cpu().numpy() if isinstance(weights, torch.Tensor) else weights
+    vals = [float(v) for v in np.ravel(np.asarray(raw, np.float64))]
+    if len(vals) == 1:
+        vals *= k
+    if len(vals) != k or not all(math.isfinite(v) and v >= 0 for v in vals):
         raise ValueError("weights must be one or ndim finite values >= 0")
+    return vals, wt
+
+
+def _per_image(gamma, x, k):
+    """One step per image, shape (B,): gamma is a number or broadcasts to
+    the batch shape x.shape[:-k]; a tensor stays in the graph, so its
+    gradient sums back to its own shape."""
+    batch = tuple(x.shape[:-k])
+    g = _as_float(gamma, x)
+    if g.numel() == 1:
+        g = g.reshape(())
+    if not _broadcasts(tuple(g.shape), batch):
+        raise ValueError(f"gamma must be a number or broadcast to the batch shape {batch}, "
+                         f"got shape {tuple(g.shape)}")
+    _require_finite_nonneg(g, "gamma")
+    return g.expand(batch).reshape(-1)
+
+
+def _spatial(sw, x, k, iso):
+    """Spatial weights in the layout of the batch solve: pixel weights that
+    broadcast to x.shape as (B, *img); edge weights of shape (k, *s), s
+    broadcasting to x.shape, as (k, B, *img). A tensor stays in the graph."""
+    name = "pixel_weights" if iso else "edge_weights"
+    full = tuple(x.shape) if iso else (k,) + tuple(x.shape)
+    t = _as_float(sw, x)
+    shape = tuple(t.shape)
+    fits = True
+    if not iso:
+        # axis 0 holds the k edge directions; unit axes after it bring the
+        # rest to x's rank, aligned on the right
+        fits = len(shape) >= 1 and shape[0] == k and len(shape) - 1 <= x.dim()
+        if fits:
+            shape = (k,) + (1,) * (x.dim() + 1 - len(shape)) + shape[1:]
+            t = t.reshape(shape)
+    if not (fits and _broadcasts(shape, full)):
+        what = "broadcast to" if iso else f"have shape ({k}, ...) broadcasting to"
+        raise ValueError(f"{name} must {what} {full}, got shape {tuple(getattr(sw, 'shape', ()))}")
+    _require_finite_nonneg(t, name)
+    lead = (-1,) if iso else (k, -1)
+    return t.expand(full).reshape(lead + tuple(x.shape[-k:]))
+
+
+def _device_tv(iso, x, gamma, ndim, weights, tol, max_iter, warm_start, return_info, deriv_tol, deriv_max_iter,
+               spatial, warn):
+    """tv_iso and tv_aniso: the arguments checked (the reference's rules and
+    messages, bindings/python/fastprox/torch.py:363-413), then one solve of
+    every image of the batch on the device through _DeviceTV, which also
+    takes the empty batch."""
+    _check_input(x, ndim, tol, max_iter)
+    w, wt = _axis_weights(weights, ndim)
     gb = _per_image(gamma, x, ndim)
     warm = None
     if warm_start is not None:
@@ -163,64 +251,13 @@ def _device_tv(iso, x, gamma, ndim, weights, tol, max_iter, warm_start, return_i
         deriv_tol = float(deriv_tol)
     cfg = _Cfg(iso, ndim, w, tol, max_iter, deriv_tol, deriv_max_iter)
     sw = None if spatial is None else _spatial(spatial, x, ndim, iso)
-    if x.numel() == 0:
-        z = x.clone()
-        for t in (gb, wt, sw):
-            if t is not None:
-                z = z + 0 * t.sum()
-        cfg.info = IterInfo(0.0, 0, 0, True, torch.zeros((ndim, *x.shape), dtype=x.dtype, device=x.device), 0.0)
-        return (z, cfg.info) if return_info else z
-    if bool(torch.isinf(x.detach()).any()):
+    if x.numel() and bool(torch.isinf(x.detach()).any()):
         raise ValueError("x contains ±Inf; image operators reject infinite values")
     z = _DeviceTV.apply(cfg, x, gb, wt, sw, warm)
     if warn and not cfg.info.converged:
         warnings.warn(f"{warn} stopped at max_iter={max_iter} before its tolerance; the certified relative "
                       f"error of z is {cfg.info.zerr:.3g}", ConvergenceWarning, stacklevel=3)
     return (z, cfg.info) if return_info else z
-
-
-def _spatial(sw, x, ndim, iso):
-    """torch.py:416-440: pixel weights as (B, *img), edge weights as (ndim, B, *img)."""
-    name = "pixel_weights" if iso else "edge_weights"
-    full = tuple(x.shape) if iso else (ndim, *x.shape)
-    t = sw if isinstance(sw, torch.Tensor) else torch.as_tensor(np.asarray(sw, np.float64))
-    t = t.to(dtype=x.dtype, device=x.device)
-    try:
-        if not iso:
-            if t.dim() < 1 or t.shape[0] != ndim or t.dim() - 1 > x.dim():
-                raise RuntimeError
-            t = t.reshape((ndim,) + (1,) * (x.dim() + 1 - t.dim()) + tuple(t.shape[1:]))
-        if torch.broadcast_shapes(t.shape, full) != full:
-            raise RuntimeError
-    except RuntimeError:
-        what = "broadcast to" if iso else f"have shape ({ndim}, ...) broadcasting to"
-        raise ValueError(f"{name} must {what} {full}, got shape {tuple(getattr(sw, 'shape', ()))}") from None
-    td = t.detach()
-    if bool(((td < 0) | ~torch.isfinite(td)).any()):
-        raise ValueError(f"{name} must be finite and >= 0")
-    img = tuple(x.shape[-ndim:])
-    return t.expand(full).reshape((-1,) + img if iso else (ndim, -1) + img)
-
-
-def _per_image(gamma, x, ndim):
-    """torch.py:443-463: gamma as one value per image, (B,)."""
-    batch = tuple(x.shape[:-ndim])
-    g = gamma if isinstance(gamma, torch.Tensor) else torch.as_tensor(np.asarray(gamma, np.float64))
-    if not g.is_floating_point():
-        g = g.to(torch.float64)
-    g = g.to(dtype=x.dtype, device=x.device)
-    if g.numel() == 1:
-        g = g.reshape(())
-    try:
-        if torch.broadcast_shapes(g.shape, batch) != batch:
-            raise RuntimeError
-    except RuntimeError:
-        raise ValueError(f"gamma must be a number or broadcast to the batch shape {batch}, "
-                         f"got shape {tuple(g.shape)}") from None
-    gd = g.detach()
-    if bool(((gd < 0) | ~torch.isfinite(gd)).any()):
-        raise ValueError("gamma must be finite and >= 0")
-    return g.expand(batch).reshape(-1)
 
 
 # ---------------------------------------------------------------------------
@@ -428,36 +465,43 @@ LAST_DERIV = None
 
 
 class _DeviceTV(torch.autograd.Function):
-    """torch.py:466-524: forward solves on the device; backward and forward
-    mode are the derivative systems at the solution (deriv.py)."""
+    """The solve as an autograd function (the reference's _DeviceTV,
+    bindings/python/fastprox/torch.py:466-524): the forward solves every
+    image on the device and keeps the solution; reverse and forward mode
+    are the implicit-differentiation systems at it (deriv.py), built once
+    and shared by both. Inputs: (cfg, x, per-image steps, axis-weight
+    tensor or None, spatial weights or None, warm start or None)."""
 
     @staticmethod
     def forward(ctx, cfg, x, gamma, wt, sw, warm):
         k = cfg.k
-        img = tuple(x.shape[-k:])
+        shape = tuple(x.shape)
+        img = shape[-k:]
+        ctx.cfg, ctx.shape, ctx.wt = cfg, shape, wt
+        ctx.sw = None if sw is None else sw.detach()
+        ctx.S = None
+        ctx.empty = x.numel() == 0
+        if ctx.empty:
+            cfg.info = IterInfo(0.0, 0, 0, True, torch.zeros((k,) + shape, dtype=x.dtype, device=x.device), 0.0)
+            return x.detach().clone()
         xb = x.detach().reshape((-1,) + img)
         wb = None if warm is None else warm.reshape((k, -1) + img)
         z, dual, gaps, its, zerrs, conv, exps, uedge = solve(xb, gamma.detach(), cfg.w, cfg.iso, cfg.tol,
-                                                             cfg.max_iter, wb, None if sw is None else sw.detach(),
-                                                             keep=True)
-        cfg.info = IterInfo(float(np.sum(gaps)), int(its.max()) if its.size else 0, 0, bool(conv.all()),
-                            dual.reshape((k,) + tuple(x.shape)), float(np.max(zerrs)) if zerrs.size else 0.0,
-                            gaps, its)
-        ctx.cfg, ctx.xb, ctx.z, ctx.dual, ctx.gam = cfg, xb, z, dual, gamma.detach()
+                                                             cfg.max_iter, wb, ctx.sw, keep=True)
+        n = its.size
+        cfg.info = IterInfo(float(gaps.sum()), int(its.max()) if n else 0, 0, bool(conv.all()),
+                            dual.reshape((k,) + shape), float(zerrs.max()) if n else 0.0, gaps, its)
+        ctx.xb, ctx.z, ctx.dual, ctx.gam = xb, z, dual, gamma.detach()
         ctx.gaps, ctx.exps, ctx.uedge = gaps, exps, uedge
-        ctx.shape = tuple(x.shape)
-        ctx.wt = wt
-        ctx.sw = None if sw is None else sw.detach()
-        ctx.S = None
-        return z.reshape(x.shape)
+        return z.reshape(shape)
 
     @staticmethod
     def _structure(ctx):
         if ctx.S is None:
             from .deriv import Structure
             cfg = ctx.cfg
-            ctx.S = Structure(ctx.xb, ctx.z, ctx.dual if cfg.iso else ctx.uedge, ctx.gaps, ctx.exps, ctx.gam, cfg.w,
-                              cfg.iso, ctx.sw)
+            u = ctx.dual if cfg.iso else ctx.uedge
+            ctx.S = Structure(ctx.xb, ctx.z, u, ctx.gaps, ctx.exps, ctx.gam, cfg.w, cfg.iso, ctx.sw)
         return ctx.S
 
     @staticmethod
@@ -465,30 +509,45 @@ class _DeviceTV(torch.autograd.Function):
         if torch.is_grad_enabled():
             raise RuntimeError("fastprox_amd: second-order differentiation through the prox is not supported; "
                                "use create_graph=False")
+        need = ctx.needs_input_grad  # (cfg, x, gamma, wt, sw, warm)
+        grads = [None] * 6
+        if ctx.empty:  # nothing to differentiate: zero gradients of the inputs' shapes
+            grads[1] = torch.zeros(ctx.shape, dtype=c.dtype, device=c.device) if need[1] else None
+            grads[2] = c.new_zeros((0,)) if need[2] else None
+            if ctx.wt is not None and need[3]:
+                grads[3] = torch.zeros_like(ctx.wt)
+            if ctx.sw is not None and need[4]:
+                grads[4] = torch.zeros_like(ctx.sw)
+            return tuple(grads)
         cfg = ctx.cfg
         S = _DeviceTV._structure(ctx)
         with torch.no_grad():
-            r, ggamma, gw, gsw, cfg.deriv = S.vjp(c.reshape(ctx.z.shape).contiguous(), cfg.deriv_tol,
-                                                 cfg.deriv_max_iter)
+            gx, gg, gw, gsw, cfg.deriv = S.vjp(c.reshape(ctx.z.shape).contiguous(), cfg.deriv_tol,
+                                               cfg.deriv_max_iter)
         global LAST_DERIV
         LAST_DERIV = (cfg.deriv[0], cfg.deriv[1], S.regions())
-        out_g = ggamma if ctx.needs_input_grad[2] else None
-        out_w = None
-        if ctx.wt is not None and ctx.needs_input_grad[3]:
-            v = gw.to(dtype=ctx.wt.dtype, device=ctx.wt.device)
-            out_w = v.sum().reshape(ctx.wt.shape) if ctx.wt.numel() == 1 else v.reshape(ctx.wt.shape)
-        out_x = r.reshape(ctx.shape) if ctx.needs_input_grad[1] else None
-        out_s = gsw if ctx.sw is not None and ctx.needs_input_grad[4] else None
-        return None, out_x, out_g, out_w, out_s, None
+        if need[1]:
+            grads[1] = gx.reshape(ctx.shape)
+        if need[2]:
+            grads[2] = gg
+        if ctx.wt is not None and need[3]:
+            # one weight tensor for every axis takes the sum of the axes' gradients
+            gw = gw.to(dtype=ctx.wt.dtype, device=ctx.wt.device)
+            grads[3] = (gw.sum() if ctx.wt.numel() == 1 else gw).reshape(ctx.wt.shape)
+        if ctx.sw is not None and need[4]:
+            grads[4] = gsw
+        return tuple(grads)
 
     @staticmethod
     def jvp(ctx, _cfg, dx, dgamma, dwt, dsw, _dwarm):
+        if ctx.empty:
+            return torch.zeros(ctx.shape, dtype=ctx.cfg.info.dual.dtype, device=ctx.cfg.info.dual.device)
         cfg = ctx.cfg
         S = _DeviceTV._structure(ctx)
         dw = None
         if dwt is not None:
-            vals = [float(a) for a in dwt.detach().cpu().reshape(-1)]
-            dw = vals * cfg.k if len(vals) == 1 else vals
+            tangent = [float(v) for v in dwt.detach().cpu().reshape(-1)]
+            dw = tangent if len(tangent) == cfg.k else tangent[:1] * cfg.k
         with torch.no_grad():
             dz, cfg.deriv = S.jvp(None if dx is None else dx.reshape(ctx.z.shape), dgamma, dw, cfg.deriv_tol,
                                   cfg.deriv_max_iter, dsw)

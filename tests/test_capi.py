@@ -126,3 +126,86 @@ def test_oracle_is_not_linked_by_the_product():
     for f in (ROOT / "fastprox_amd").rglob("*.py"):
         src = f.read_text()
         assert "import oracle" not in src and "from oracle" not in src, f
+
+
+def _descriptor_cases(buf):
+    """(name, fp_array Array, is_valid) for descriptors of x: the boundary's
+    rules, each broken once, and valid ones beside them."""
+    p = buf.ctypes.data
+    big, huge = 1 << 40, 1 << 62
+    return [
+        ("int32 dtype", A.Array(p, A.FP_I32, (4, 5)), False),
+        ("unknown dtype", A.Array(p, 99, (4, 5)), False),
+        ("cuda device", A.Array(p, A.FP_F64, (4, 5), device=A.FP_DEVICE_CUDA), False),
+        ("unknown device", A.Array(p, A.FP_F64, (4, 5), device=7), False),
+        ("negative extent", A.Array(p, A.FP_F64, (-1, 5)), False),
+        ("too many entries", A.Array(p, A.FP_F64, (big, big)), False),
+        ("stride beyond the range", A.Array(p, A.FP_F64, (4, 5), (huge, 1)), False),
+        ("negative stride beyond the range", A.Array(p, A.FP_F64, (4, 5), (-huge, 1)), False),
+        ("offset span beyond the range", A.Array(p, A.FP_F64, (1 << 20, 1 << 20), (1 << 41, 1)), False),
+        ("no data", A.Array(0, A.FP_F64, (4, 5)), False),
+        ("one axis for a 2D operator", A.Array(p, A.FP_F64, (20,)), False),
+        ("contiguous", A.Array(p, A.FP_F64, (4, 5)), True),
+        ("strided", A.Array(p, A.FP_F64, (2, 5), (10, 1)), True),
+        ("negative stride", A.Array(p + 8 * 15, A.FP_F64, (4, 5), (-5, 1)), True),
+        ("a huge stride on an extent of 1", A.Array(p, A.FP_F64, (1, 4, 5), (huge, 5, 1)), True),
+        ("huge extents beside a zero one", A.Array(0, A.FP_F64, (0, big, big)), False),  # extents above 1 count
+        ("no entries and no data", A.Array(0, A.FP_F64, (0, 5)), True),
+    ]
+
+
+def _ndim_case(buf, ndim, shape_null=False):
+    a = A.Array(buf.ctypes.data, A.FP_F64, (4, 5))
+    a.s.ndim = ndim
+    if shape_null:
+        a.s.shape = None
+    return a
+
+
+def test_descriptor_rules_match_the_reference(lib):
+    """fp_workspace_create checks x's descriptor before any device work: each
+    malformed descriptor gets the status the reference library gives it
+    (oracle/_ref, built from its sources), and the valid ones pass the checks
+    (here, without a GPU, they stop at FP_EDEVICE)."""
+    import refbind
+    buf = np.zeros(64)
+    cases = _descriptor_cases(buf)
+    cases += [("ndim -1", _ndim_case(buf, -1), False), ("ndim 33", _ndim_case(buf, 33), False),
+              ("null shape", _ndim_case(buf, 2, shape_null=True), False)]
+    ref = refbind.lib() if refbind.available() else None
+    ops = []
+    for L in [lib] + ([ref] if ref else []):
+        op = C.c_void_p()
+        assert L.fp_op_tv_iso(2, None, C.byref(op)) == A.FP_OK
+        ops.append(op)
+    expect = {"int32 dtype": A.FP_EDTYPE, "unknown dtype": A.FP_EDTYPE, "cuda device": A.FP_EINVAL,
+              "unknown device": A.FP_EINVAL, "no data": A.FP_EINVAL, "null shape": A.FP_EINVAL}
+    for name, arr, valid in cases:
+        ws = C.c_void_p()
+        ours = lib.fp_workspace_create(ops[0], arr.ref, None, C.byref(ws))
+        if ours == A.FP_OK:
+            lib.fp_workspace_free(ws)
+        if valid:
+            assert ours in (A.FP_OK, A.FP_EDEVICE), (name, A.STATUS.get(ours))
+        else:
+            assert ours == expect.get(name, A.FP_ESHAPE), (name, A.STATUS.get(ours))
+        if ref:
+            theirs = ref.fp_workspace_create(ops[1], arr.ref, None, C.byref(ws))
+            if theirs == A.FP_OK:
+                ref.fp_workspace_free(ws)
+            assert (theirs == A.FP_OK) == valid, (name, A.STATUS.get(theirs))
+            if not valid:
+                assert ours == theirs, (name, A.STATUS.get(ours), A.STATUS.get(theirs))
+    lib.fp_op_free(ops[0])
+    if ref:
+        ref.fp_op_free(ops[1])
+
+
+def test_weight_descriptor_rules(lib):
+    """The axis-weight array of the constructors follows the same rules."""
+    op = C.c_void_p()
+    w = np.ones(2, np.int32)
+    assert lib.fp_op_tv_iso(2, A.numpy_array(w).ref, C.byref(op)) == A.FP_EDTYPE
+    w2 = np.ones((2, 1))
+    assert lib.fp_op_tv_iso(2, A.numpy_array(w2).ref, C.byref(op)) == A.FP_ESHAPE
+    assert lib.fp_op_tv_iso(2, A.Array(0, A.FP_F64, (2,)).ref, C.byref(op)) == A.FP_EINVAL

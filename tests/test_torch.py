@@ -96,6 +96,16 @@ def test_errors_and_empty(cuda):
         tv_iso(xi, 0.1)
     e = torch.empty((0, 4, 5), dtype=torch.float32, device=cuda)
     assert tv_iso(e, 0.1).shape == (0, 4, 5)
+    # an empty batch stays in the graph of every input: zero gradients of their shapes
+    eg = e.clone().requires_grad_(True)
+    g = torch.tensor(0.1, device=cuda, requires_grad=True)
+    w = torch.tensor([1.0, 0.5], device=cuda, requires_grad=True)
+    al = torch.ones((4, 5), device=cuda, requires_grad=True)
+    z, info = tv_iso(eg, g, weights=w, pixel_weights=al, return_info=True)
+    assert info.converged and info.iters == 0 and info.dual.shape == (2, 0, 4, 5)
+    z.sum().backward()
+    assert eg.grad.shape == e.shape and float(g.grad) == 0.0
+    assert torch.equal(w.grad, torch.zeros(2, device=cuda)) and torch.equal(al.grad, torch.zeros((4, 5), device=cuda))
     xn = x.clone()
     xn[2, 2] = float("nan")
     z, info = tv_iso(xn, 0.1, return_info=True)
