@@ -109,6 +109,8 @@ def declare(lib):
         "fp_abi_version": (I, []),
         "fp_version_string": (C.c_char_p, []),
         "fpr_alloc_count": (I64, []),
+        "fpr_last_error": (C.c_char_p, []),
+        "fpr_prox_static": (I, [P, D, A, A, C.POINTER(fp_options), P, C.POINTER(fp_info)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name, None)

@@ -27,6 +27,7 @@
 #include "engine.hpp"
 #include "ws.hpp"
 #include "serve.hpp"
+#include "static_solve.hpp"
 
 #ifndef FASTPROX_ROCM_BUILD
 #define FASTPROX_ROCM_BUILD "1.0.6"
@@ -791,6 +792,86 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     return status;
 }
 
+// fpr_prox_static: fp_prox without a workspace (the arguments checked as
+// prox_impl checks them), on contiguous device arrays, with its device
+// scratch in the fixed array of static_solve.cpp. The kernel is fixed (2D:
+// the streaming kernel, 6 iterations per launch, 16-row segments; 3D: the
+// streaming kernel with 16-slice slabs; the generic kernel for planes too
+// large for the streaming kernel's offsets), so that nothing is read from
+// the tuning table. Nothing here allocates.
+fp_status static_prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *oa, const fp_options *opt,
+                           void *stream, fp_info *info) {
+    if (!op || !xa || !oa) return FP_EINVAL;
+    if (!std::isfinite(step) || step < 0) return FP_EPARAM;
+    const fp_options o = opt ? *opt : defaults();
+    if (o.tie != 0 && o.tie != 1) return FP_EINVAL;
+    Arr x, out;
+    if (fp_status s = to_arr(xa, x)) return s;
+    if (fp_status s = to_arr(oa, out)) return s;
+    if (x.dtype != out.dtype) return FP_EDTYPE;
+    if (!same_shape(x, out)) return FP_ESHAPE;
+    if (fp_status s = check_output(&x, out)) return s;
+    const int k = op->k;
+    if (x.ndim < k) return FP_ESHAPE;
+    if (info) *info = fp_info{0.0, 0, 0};
+    if (!(o.tol >= 0.0) || !std::isfinite(o.tol) || o.max_iter < 1) return FP_EPARAM;
+    if (fp_status s = step_ok(*op, step)) return s;
+    if (x.numel() != 0)
+        if (fp_status s = check_image(*op, x.shape + (x.ndim - k))) return s;
+    if (o.keep_structure) return fail(FP_EINVAL, "fpr_prox_static keeps no structure (it has no workspace)");
+    if (x.device != FP_DEVICE_ROCM || out.device != FP_DEVICE_ROCM || !x.contiguous() || !out.contiguous())
+        return fail(FP_EINVAL, "fpr_prox_static takes contiguous device arrays");
+    if (x.numel() == 0) return FP_OK;
+    fpr::StaticSolve c;
+    c.tsize = (int)x.itemsize();
+    c.P.k = k;
+    c.P.iso = op->iso ? 1 : 0;
+    c.P.sw = op->spatial.empty() ? 0 : 1;
+    c.P.sw_per_image = 0;
+    for (int a = 0; a < 3; ++a) c.P.w[a] = op->w[a];
+    c.P.sched = op->iso ? fpr::kSchedC : fpr::kSchedDevice;
+    c.P.wmax_norm = op->iso ? 1 : 0;
+    c.P.alpha_max = op->spatial.empty() ? 1.0 : op->spatial_max;
+    int64_t N = 1;
+    for (int a = 0; a < k; ++a) {
+        c.L[3 - k + a] = x.shape[x.ndim - k + a];
+        N *= c.L[3 - k + a];
+    }
+    c.N = N;
+    const int64_t B = x.numel() / N;
+    if (B > fpr::kStaticMaxImages)
+        return fail(FP_ENOMEM, "fpr_prox_static: more images than its scratch holds (256 at most)");
+    c.B = (int)B;
+    if (k == 3) {
+        c.kc.generic = false;
+        c.kc.stream = false;
+        c.kc.s3 = true;
+        c.kc.TT = 1;
+        c.kc.RY = 16;
+    } else {
+        c.kc.stream = true;
+        c.kc.TT = 6;
+        c.kc.RY = 16;
+    }
+    if ((c.kc.stream && !fpr::stream_fits(c.tsize, N)) || (k == 3 && !fpr::kernel_available_s3(16)))
+        c.kc = generic_kernel();
+    c.x = x.data;
+    c.z = out.data;
+    c.stage_x = x.data == out.data;
+    c.sw_host = op->spatial.empty() ? nullptr : op->spatial.data();
+    c.gamma = step;
+    c.tol = o.tol;
+    c.max_iter = o.max_iter;
+    c.stream = (hipStream_t)stream;
+    if (fpr::static_bytes_needed(c) > fpr::kStaticScratchBytes)
+        return fail(FP_ENOMEM, "fpr_prox_static: the images need more than its 64 MiB of scratch");
+    fpr::StaticResult R;
+    if (int r = fpr::static_solve(c, &R)) return hip_status(r);
+    if (R.bad) return FP_EINVAL;
+    if (info) *info = fp_info{R.gap, R.iters, 0};
+    return R.notconv ? FP_ENOTCONV : FP_OK;
+}
+
 // fp_workspace_refine (the reference's refine_impl, src/api/api.cpp:1185-1217):
 // the kept call solved again from its saved input, warm-started from its
 // dual, to tol within max_iter; the derivatives then read the refined point.
@@ -1319,6 +1400,11 @@ FP_API fp_status fp_workspace_deriv_info(const fp_workspace *ws, double *residua
 FP_API fp_status fp_prox(const fp_op *op, double gamma, const fp_array *x, fp_array *out, const fp_options *opt,
                          fp_workspace *ws, fp_info *info) {
     return guard([&] { return prox_impl(op, gamma, x, out, opt, ws, info, nullptr); });
+}
+
+FP_API fp_status fpr_prox_static(const fp_op *op, double gamma, const fp_array *x, fp_array *out,
+                                 const fp_options *opt, void *stream, fp_info *info) {
+    return guard([&] { return static_prox_impl(op, gamma, x, out, opt, stream, info); });
 }
 
 FP_API fp_status fp_prox_steps(const fp_op *op, const fp_array *gamma, const fp_array *x, fp_array *out,

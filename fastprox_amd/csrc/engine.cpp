@@ -416,6 +416,20 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     return err == (hipError_t)-1 ? -1 : err;
 }
 
+template <class T>
+int run_solve_in(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, void *arena, ImgDesc *desc,
+                 ImgState *state, double *part, Tile **list, int *counts, double *gam, double *propart, int B,
+                 int total_tiles, int64_t maxN, bool tv1d, int *hcount, int steps_per_sync, int max_iter) {
+    return run_solve<T>(E, P, kc, s, arena, desc, state, part, list, counts, gam, propart, B, total_tiles, maxN, tv1d,
+                        hcount, steps_per_sync, max_iter, kc, nullptr, 0, 0.0);
+}
+template int run_solve_in<float>(Engine &, const Problem &, const KernelChoice &, hipStream_t, void *, ImgDesc *,
+                                 ImgState *, double *, Tile **, int *, double *, double *, int, int, int64_t, bool,
+                                 int *, int, int);
+template int run_solve_in<double>(Engine &, const Problem &, const KernelChoice &, hipStream_t, void *, ImgDesc *,
+                                  ImgState *, double *, Tile **, int *, double *, double *, int, int, int64_t, bool,
+                                  int *, int, int);
+
 namespace {
 
 // Length of the union of the intervals.

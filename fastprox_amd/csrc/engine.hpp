@@ -165,4 +165,16 @@ private:
     double *hgam_ = nullptr;      // pinned
 };
 
+// The solve loop of an engine over buffers the caller provides (laid out as
+// Engine::setup lays out its own: the states, descriptors, partials, the two
+// tile lists, the counts, the steps and the prologue's partials; hcount two
+// host ints), with no tail kernel: the prologue, then chunks of step and
+// epilogue launches until no tile is live. E lends its events; set its
+// graphs and timing off to launch every kernel directly. Returns a
+// hipError_t, or -1 when the schedule does not finish.
+template <class T>
+int run_solve_in(Engine &E, const Problem &P, const KernelChoice &kc, hipStream_t s, void *arena, ImgDesc *desc,
+                 ImgState *state, double *part, Tile **list, int *counts, double *gam, double *propart, int B,
+                 int total_tiles, int64_t maxN, bool tv1d, int *hcount, int steps_per_sync, int max_iter);
+
 }  // namespace fpr

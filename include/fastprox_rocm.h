@@ -219,6 +219,23 @@ FP_API const fpr_plugin_table *fastprox_rocm_plugin(void);
 /* A short description of why the last failing fpr_ call on this thread failed. */
 FP_API const char *fpr_last_error(void);
 
+/* fp_prox without a workspace for callers that cannot allocate device memory
+ * (code under graph capture, serving loops, harnesses that forbid it): no
+ * counterpart in the reference. It allocates nothing: its device scratch (the
+ * dual iterates, descriptors, states, tile lists, partial sums, the dual
+ * output, a copy of x when out aliases x, the weights) is a fixed 64 MiB array
+ * that every GPU the library runs on holds until the process ends. x and out:
+ * contiguous device arrays on the current device, at most 256 images whose
+ * buffers fit the scratch (FP_ENOMEM otherwise, with fpr_last_error saying
+ * which limit); stream: a hipStream_t (NULL: the null stream); the call
+ * blocks until the solve is done. Results as fp_prox's (info: the gaps
+ * summed and the largest iteration count; no dual or structure is kept). The
+ * kernel is fixed: 2D, the streaming kernel with 6 iterations per launch and
+ * 16-row segments; 3D, the streaming kernel with 16-slice slabs. Calls that
+ * overlap on one GPU share its scratch and race: serialise them. */
+FP_API fp_status fpr_prox_static(const fp_op *op, double gamma, const fp_array *x, fp_array *out,
+                                 const fp_options *opt, void *stream, fp_info *info);
+
 /* The number of device and pinned host allocations (hipMalloc,
  * hipHostMalloc) the library has made in this process. Every device buffer is
  * allocated by a create call (fp_workspace_create; with keep_structure also
