@@ -24,40 +24,143 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-BYTES_PER_PIXEL_ITER = {(2, 4): 36, (2, 8): 72, (3, 4): 52, (3, 8): 104}  # 1 + 4k words
+# Bytes of one pixel's state pass (read d, u, v; write u, v: 1 + 4k words),
+# per (k, dtype bytes). A step launch makes one pass over every image it
+# iterates, whatever its iterations; a one-iteration-per-pass kernel makes
+# one pass per pixel-iteration.
+BYTES_PER_PIXEL_PASS = {(2, 4): 36, (2, 8): 72, (3, 4): 52, (3, 8): 104}
+BYTES_PER_PIXEL_ITER = BYTES_PER_PIXEL_PASS
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(x, gamma, tol, max_iter, budget_s=20.0):
+def cpu_baseline(x, gamma, tol, max_iter, reps=5, batch=16):
     """The reference's CPU solver on a bounded sample of the same images:
-    the reference library built from its sources (oracle/_ref) with its own
-    thread pool when present, else the oracle's single-threaded restatement."""
+    the reference library built from its sources (oracle/_ref), one fp_prox
+    call on the first `batch` images with fp_options.threads = 16 (its pool
+    runs the images in parallel), the workspace created outside the timed
+    call; the median of `reps` timed calls after one untimed call. Without
+    the library, the oracle's single-threaded restatement on one image."""
     ref = ROOT / "oracle" / "_ref" / "libfastprox_ref.so"
-    n_img = x.shape[0]
+    host = host_cpu()
     if ref.exists():
+        import ctypes as C
         sys.path.insert(0, str(ROOT / "tests"))
         import refbind
-        threads = min(16, os.cpu_count() or 1)
+        from fastprox_amd import _capi as A
+        L = refbind.lib()
+        threads = min(16, host["nproc"] or 1)
+        n = min(batch, x.shape[0])
+        xs = np.ascontiguousarray(x[:n])
+        z = np.empty_like(xs)
+        opt = A.fp_options()
+        L.fp_options_init(C.byref(opt))
+        opt.tol, opt.max_iter, opt.threads = tol, max_iter, threads
         op = refbind.Op(True, 2)
-        done, t0 = 0, time.perf_counter()
-        while done < n_img and time.perf_counter() - t0 < budget_s:
-            op.solve(x[done:done + 1].astype(np.float32), gamma, tol=tol, max_iter=max_iter, threads=threads)
-            done += 1
-        dt = time.perf_counter() - t0
-        return {"value": done / dt, "unit": "images/s", "cores": threads, "kind": "reference",
-                "sample": f"{done} of the {n_img} images, one fp_prox call each, fp_options.threads={threads}, "
-                          f"{dt:.1f} s"}
+        xa, za = A.numpy_array(xs), A.numpy_array(z)
+        ws = C.c_void_p()
+        assert L.fp_workspace_create(op.h, xa.ref, C.byref(opt), C.byref(ws)) == A.FP_OK
+        info = A.fp_info()
+        med, ts, st = _median_time(lambda: L.fp_prox(op.h, float(gamma), xa.ref, za.ref, C.byref(opt), ws,
+                                                     C.byref(info)), reps)
+        L.fp_workspace_free(ws)
+        return {"value": n / med, "unit": "images/s", "cores": threads, "kind": "reference",
+                "host": host, "reps": reps, "median_s": med, "times_s": ts,
+                "sample": f"the first {n} of the {x.shape[0]} images ({x.dtype}, promoted by the library), one "
+                          f"fp_prox call per repetition with fp_options.threads={threads}; median of {reps} after "
+                          f"one untimed call"}
     import oracle
-    done, t0 = 0, time.perf_counter()
-    while done < n_img and time.perf_counter() - t0 < budget_s:
-        oracle.tv_iso(x[done].astype(np.float64), gamma, tol=tol, max_iter=max_iter)
-        done += 1
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "images/s", "cores": 1, "kind": "port",
-            "sample": f"{done} of the {n_img} images, the oracle's C restatement, 1 thread, {dt:.1f} s"}
+    med, ts, _ = _median_time(lambda: oracle.tv_iso(x[0].astype(np.float64), gamma, tol=tol, max_iter=max_iter),
+                              reps)
+    return {"value": 1.0 / med, "unit": "images/s", "cores": 1, "kind": "port", "host": host, "reps": reps,
+            "median_s": med, "times_s": ts,
+            "sample": f"image 0 of the {x.shape[0]}, the oracle's C restatement, 1 thread, median of {reps}"}
+
+
+def host_cpu():
+    """The host's CPU model and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        for ln in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "nproc": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else os.cpu_count()}
+
+
+def _median_time(fn, reps):
+    """fn() once untimed, then the median wall time of reps timed calls."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts, r
+
+
+def cpu_table(threads_list, reps=5):
+    """The reference library (oracle/_ref, built from its sources) on the
+    reference's own CPU configuration and the anisotropic one, through its C
+    ABI with a workspace created outside the timed call: C0 (8 x 256^2, 12
+    rectangles + N(0, 0.1^2), seed 0, iso, gamma 0.1, tol 1e-6) and C1 (64 x
+    512^2 disks, aniso w (1, 0.5), gamma 0.05, tol 1e-6, float32 and float64
+    inputs, promoted by the reference). Median of reps timed calls after one
+    untimed call, per thread count."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import ctypes as C
+
+    import refbind
+    from fastprox_amd import _capi as A
+    from fastprox_amd.synth import digest, disks, rectangles
+    if not refbind.available():
+        raise SystemExit("oracle/_ref is not built")
+    L = refbind.lib()
+    out = {"host": host_cpu(), "reps": reps, "rows": []}
+
+    def timed_solve(op, x, gamma, tol, threads):
+        xs = np.ascontiguousarray(x)
+        z = np.empty_like(xs)
+        opt = A.fp_options()
+        L.fp_options_init(C.byref(opt))
+        opt.tol, opt.max_iter, opt.threads = tol, 100000, threads
+        xa, za = A.numpy_array(xs), A.numpy_array(z)
+        ws = C.c_void_p()
+        assert L.fp_workspace_create(op.h, xa.ref, C.byref(opt), C.byref(ws)) == A.FP_OK
+        info = A.fp_info()
+
+        def call():
+            return L.fp_prox(op.h, float(gamma), xa.ref, za.ref, C.byref(opt), ws, C.byref(info))
+        med, ts, st = _median_time(call, reps)
+        B = xs.shape[0]
+        gaps, its, zerrs = np.empty(B), np.empty(B, np.int32), np.empty(B)
+        L.fp_workspace_image_info(ws, gaps.ctypes.data_as(C.POINTER(C.c_double)),
+                                  its.ctypes.data_as(C.POINTER(C.c_int)), zerrs.ctypes.data_as(C.POINTER(C.c_double)),
+                                  B)
+        L.fp_workspace_free(ws)
+        return med, ts, st, gaps, its
+
+    cases = [("C0 8x256^2 f32 iso gamma 0.1 tol 1e-6", rectangles(8, (256, 256), n=12, sigma=0.1, seed=0),
+              refbind.Op(True, 2), 0.1)]
+    d = disks(64, (512, 512), seed=0, dtype=np.float64)
+    cases.append(("C1 64x512^2 f32 aniso w (1,0.5) gamma 0.05 tol 1e-6", d.astype(np.float32),
+                  refbind.Op(False, 2, [1.0, 0.5]), 0.05))
+    cases.append(("C1 64x512^2 f64 aniso w (1,0.5) gamma 0.05 tol 1e-6", d, refbind.Op(False, 2, [1.0, 0.5]), 0.05))
+    for name, x, op, gamma in cases:
+        for th in threads_list:
+            med, ts, st, gaps, its = timed_solve(op, x, gamma, 1e-6, th)
+            row = {"config": name, "threads": th, "median_s": med, "times_s": ts,
+                   "images_per_s": x.shape[0] / med, "status": A.STATUS.get(st, st),
+                   "iters": [int(v) for v in its], "gaps": [float(v) for v in gaps], "input_digest": digest(x)}
+            log(f"{name}, {th} threads: {med:.3f} s median ({x.shape[0] / med:.2f} images/s), iters "
+                f"{int(its.min())}..{int(its.max())}")
+            out["rows"].append(row)
+    return out
 
 
 def main():
@@ -75,16 +178,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="copy 1 GiB five times (the HBM counters' scale for the traffic figure)")
-    ap.add_argument("--legs", default="b256,uneven,aniso,3d,backward,serve512,serve",
+    ap.add_argument("--legs", default="b256,f64,uneven,aniso,3d,backward,serve512,serve",
                     help="the other configurations to measure after the batched one: any of b256 (the same "
-                         "images, 256 per batch), uneven (256 uneven "
+                         "images, 256 per batch), f64 (the headline in float64), uneven (256 uneven "
                          "images with pixel weights), aniso (64 x 512^2 f32 and f64), 3d (16 x 128^3), "
                          "backward ((16,3,256,256) "
                          "autograd), serve (the request trace); '' for none")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool-gib", type=float, default=24.0)
+    ap.add_argument("--serve-cpu-requests", type=int, default=None,
+                    help="requests in the served CPU baseline's sample (default: as many as 25 s allow)")
+    ap.add_argument("--cpu-table", default=None,
+                    help="only time the reference library on the CPU configurations at these thread counts "
+                         "(comma-separated) and print that table as JSON; no GPU is used")
     args = ap.parse_args()
+    if args.cpu_table:
+        print(json.dumps(cpu_table([int(v) for v in args.cpu_table.split(",")])), flush=True)
+        return
     if args.copy_calibration:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -133,7 +244,7 @@ def main():
         log(f"warmup {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    step_ms_sum, epi_ms_sum, launches_sum, busy_ms_sum = 0.0, 0.0, 0, 0.0
+    step_ms_sum, epi_ms_sum, launches_sum, busy_ms_sum, passes_sum = 0.0, 0.0, 0, 0.0, 0.0
     for i in range(args.steps):
         st = step()
         a, b_, c = bt.timing()
@@ -141,31 +252,43 @@ def main():
         epi_ms_sum += b_
         launches_sum += c
         busy_ms_sum += bt.busy_ms()
+        passes_sum += bt.pixel_passes()
         log(f"step {i + 1}/{args.steps} status {st}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     gaps, its, zerrs, conv = bt.image_info()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = B * args.steps / (t1 - t0)
-    # Roofline of the step kernel: algorithmic bytes (36 B per pixel-iteration
-    # in fp32 2D: read x, u, v; write u, v) over the time the step kernels
-    # held the device. The batch runs as two halves on two streams whose
-    # launches overlap, so that time is the union of the event-timed launch
-    # intervals of both streams (busy), not their sum.
-    alg_bytes = float(np.sum(its.astype(np.float64)) * S * S * BYTES_PER_PIXEL_ITER[(2, 4)]) * args.steps
-    step_s = busy_ms_sum / 1e3
-    achieved = alg_bytes / step_s / 1e9 if step_s > 0 else 0.0
-    # HBM bytes per step dispatch from the committed rocprofv3 counter runs
-    # of this same workload and kernel choice (FETCH_SIZE and WRITE_SIZE,
-    # copy-calibrated), when they match.
-    traffic, traffic_src, hbm_gbs = None, None, None
-    prof = ROOT / "profiles" / "stream2d_iso_f32_512x512_b64_traffic.json"
+    # Roofline of the step kernel (HBM-bound: no dense contraction). The
+    # algorithmic bytes of one launch are one state pass over every pixel it
+    # iterates: 36 B per pixel (read x, u, v; write u, v in float32), whatever
+    # its iterations (the fused TT iterations of a launch cost one round trip
+    # of the state). achieved: those bytes per launch over the launch's
+    # average duration (HIP events around each step launch on the stream it
+    # runs on; the launches of the batch's two halves overlap, so each runs
+    # longer than alone); achieved_device: the same bytes over the time at
+    # least one step kernel ran (the union of the launch intervals of both
+    # streams). traffic: the HBM bytes per dispatch the rocprofv3 counters
+    # read for this kernel and workload (FETCH_SIZE, WRITE_SIZE,
+    # copy-calibrated), committed under profiles/; its rate over the same
+    # launch durations is the measured HBM rate.
+    bpp = BYTES_PER_PIXEL_PASS[(2, 4)]
+    alg_bytes = passes_sum * bpp
+    launch_ms = step_ms_sum / launches_sum if launches_sum else 0.0
+    alg_per_launch = alg_bytes / launches_sum if launches_sum else 0.0
+    achieved = alg_per_launch / (launch_ms * 1e6) if launch_ms > 0 else 0.0
+    achieved_device = alg_bytes / (busy_ms_sum * 1e6) if busy_ms_sum > 0 else 0.0
+    one_pass_equiv = float(np.sum(its.astype(np.float64)) * S * S * bpp) * args.steps / (busy_ms_sum * 1e6) \
+        if busy_ms_sum > 0 else 0.0
+    traffic, traffic_src, hbm_gbs, valu_busy = None, None, None, None
+    prof = ROOT / "profiles" / "stream2d_iso_f32_512x512_b64_counters.json"
     if prof.exists():
         rec = json.loads(prof.read_text())
         if rec.get("kernel_choice") == kernel_name(bt) and rec.get("batch") == B and rec.get("size") == S:
             traffic, traffic_src = rec["hbm_bytes_per_dispatch"], str(prof.relative_to(ROOT))
-            if launches_sum and busy_ms_sum:
-                hbm_gbs = traffic * launches_sum / (busy_ms_sum * 1e6)
+            valu_busy = rec.get("valu_busy_frac")
+            if launch_ms > 0:
+                hbm_gbs = traffic / (launch_ms * 1e6)
     log(f"batched: {value:.1f} images/s, {ms_per_step:.2f} ms per batch; iters mean {its.mean():.1f} "
         f"max {its.max()}; step kernels {step_ms_sum / args.steps:.2f} ms ({busy_ms_sum / args.steps:.2f} ms "
         f"busy), epilogues {epi_ms_sum / args.steps:.2f}"
@@ -196,6 +319,8 @@ def main():
     extra = {}
     if "b256" in legs and S == 512:
         extra["batched_b256"] = leg_b256(args, plain)
+    if "f64" in legs and S == 512:
+        extra["batched_f64"] = leg_f64(args)
     if "uneven" in legs:
         extra["uneven_batch"] = leg_uneven(args)
     if "aniso" in legs:
@@ -239,13 +364,19 @@ def main():
         "speedup_vs_torch_per_call": (value / torch_pc["images_per_s"]) if torch_pc else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "alg_bytes_per_launch": alg_per_launch, "launch_ms_mean": launch_ms,
+                     "achieved_device": achieved_device, "frac_device": achieved_device / HBM_PEAK_GBS,
                      "hbm_gbs_measured": hbm_gbs,
-                     "note": "achieved counts the algorithmic 36 B of every pixel-iteration over the time the "
-                             "step kernels held the device (step_kernel_busy_ms: the union of the event-timed "
-                             "launch intervals of the batch's two streams, whose launches overlap); the kernel "
-                             "runs TT iterations per HBM pass, so achieved can exceed the one-pass peak, while "
-                             "hbm_gbs_measured (traffic per dispatch x dispatches over the same time) is the "
-                             "HBM rate",
+                     "hbm_frac_measured": hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None,
+                     "valu_busy_frac": valu_busy,
+                     "alg_equiv_one_pass_gbs": one_pass_equiv,
+                     "note": "achieved: the algorithmic bytes of a step launch (36 B per pixel it iterates: one "
+                             "state pass, read x, u, v and write u, v, whatever its TT iterations) over the "
+                             "launch's average duration; achieved_device: the same bytes over the union of the "
+                             "launch intervals of the batch's two streams; hbm_gbs_measured: the counters' HBM "
+                             "bytes per dispatch (traffic) over the average launch duration; "
+                             "alg_equiv_one_pass_gbs: 36 B per pixel-iteration over the union time, the rate a "
+                             "one-iteration-per-pass kernel would need (not a fraction of any roof)",
                      "kernel": {0: "stream2d_kernel", 1: "step1_kernel", 2: "step2d_kernel",
                                 3: "stream3d_kernel"}[bt_kernel(bt)[0]],
                      "step_kernel_busy_ms_per_batch": busy_ms_sum / args.steps,
@@ -401,6 +532,43 @@ def leg_uneven(args, n=256, seed=0):
     return out
 
 
+def leg_f64(args, B=64, S=512):
+    """The headline workload in float64 (the reference computes in FP64,
+    tv_iso.cpp:342-347): 64 x 512^2, the headline generator, gamma 0.1, tol
+    1e-6; one state pass is 72 B per pixel."""
+    import torch
+
+    from fastprox_amd import Op
+    from fastprox_amd.synth import rectangles
+    from fastprox_amd.torch import Batch
+    x = rectangles(B, (S, S), n=12, sigma=0.1, seed=0, dtype=np.float32).astype(np.float64)
+    xs = torch.tensor(x, device="cuda")
+    zs = torch.empty_like(xs)
+    ds = torch.empty((B, 2, S, S), dtype=torch.float64, device="cuda")
+    bt = Batch(Op(True, 2), torch.float64, [(S, S)] * B)
+    bt.set_timing(True)
+    xl, zl, dl = list(xs), list(zs), list(ds)
+
+    def run():
+        return bt.solve(xl, zl, dl, [args.gamma] * B, tol=args.tol, max_iter=args.max_iter)
+    run()
+    sec, st = _sync_time(run, 2)
+    gaps, its, zerrs, conv = bt.image_info()
+    a, b_, c = bt.timing()
+    busy = bt.busy_ms()
+    alg = bt.pixel_passes() * BYTES_PER_PIXEL_PASS[(2, 8)]
+    out = {"workload": f"isotropic TV, batch {B} x {S}x{S} float64 (the headline images), gamma {args.gamma}, "
+                       f"tol {args.tol}", "images_per_s": B / sec, "ms_per_batch": sec * 1e3,
+           "iters_mean": float(its.mean()), "iters_max": int(its.max()), "converged": bool(conv.all()),
+           "kernel": kernel_name(bt), "step_kernel_ms": a, "step_kernel_busy_ms": busy, "launches": c,
+           "alg_gbs_per_launch": alg / (a * 1e6) if a else None,
+           "alg_gbs_device": alg / (busy * 1e6) if busy else None,
+           "frac_device": alg / (busy * 1e6) / HBM_PEAK_GBS if busy else None}
+    log(f"batched f64: {out}")
+    del xs, zs, ds, bt
+    return out
+
+
 def leg_aniso(args, B=64, S=512):
     """64 x 512^2 disks (20 disks, radius U[8,96], + N(0, 0.15^2)), weights
     (1, 0.5), gamma 0.05, gap tol 1e-6, in float32 and float64."""
@@ -453,11 +621,16 @@ def leg_3d(args, B=16, S=128):
     sec, st = _sync_time(run, 1)
     gaps, its, zerrs, conv = bt.image_info()
     a, b_, c = bt.timing()
-    alg = float(its.sum()) * S ** 3 * BYTES_PER_PIXEL_ITER[(3, 4)]
+    busy = bt.busy_ms()
+    # one state pass (52 B per voxel in float32) per launch and voxel iterated
+    alg = bt.pixel_passes() * BYTES_PER_PIXEL_PASS[(3, 4)]
     out = {"workload": f"isotropic TV, {B} x {S}^3 float32 boxes, gamma 0.05, tol 1e-5", "volumes_per_s": B / sec,
            "ms_per_batch": sec * 1e3, "iters_mean": float(its.mean()), "iters_max": int(its.max()),
-           "kernel": kernel_name(bt), "step_kernel_ms": a, "epilogue_ms": b_, "launches": c,
-           "alg_gbs": alg / (a / 1e3) / 1e9 if a else None}
+           "kernel": kernel_name(bt), "step_kernel_ms": a, "step_kernel_busy_ms": busy, "epilogue_ms": b_,
+           "launches": c, "alg_bytes": alg,
+           "alg_gbs_per_launch": alg / (a * 1e6) if a else None,
+           "alg_gbs_device": alg / (busy * 1e6) if busy else None,
+           "frac_device": alg / (busy * 1e6) / HBM_PEAK_GBS if busy else None}
     log(f"3d: {out}")
     return out
 
@@ -524,7 +697,7 @@ def leg_serve(args):
     st["launches"] = srv.launches()
     log(f"served: {st}")
     if not args.no_cpu:
-        st["cpu"] = serve_cpu_baseline(trace, a, b)
+        st["cpu"] = serve_cpu_baseline(trace, a, b, requests=args.serve_cpu_requests)
         log(f"served cpu: {st['cpu']}")
     return st
 
@@ -571,29 +744,51 @@ def leg_serve512(args, xs, plain, n=2048):
     return out
 
 
-def serve_cpu_baseline(trace, src2d, src3d, budget_s=25.0):
-    """The reference CPU solver's latency per request (no queueing) on the
-    first requests of the same trace, at 16 threads, while the budget lasts."""
+def serve_cpu_baseline(trace, src2d, src3d, budget_s=25.0, requests=None, seed=1):
+    """The reference CPU solver's latency per request (no queueing), at 16
+    threads, on a random sample of the same trace stratified by class (the
+    2D and 3D requests in the trace's proportion, each class in a random
+    order), until `requests` are done or, without a count, the budget ends."""
     ref = ROOT / "oracle" / "_ref" / "libfastprox_ref.so"
     if not ref.exists():
         return None
     from fastprox_amd.synth import crop_of
     sys.path.insert(0, str(ROOT / "tests"))
     import refbind
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(16, host_cpu()["nproc"] or 1)
     ops = {2: refbind.Op(True, 2), 3: refbind.Op(True, 3)}
-    lat, t_all = [], time.perf_counter()
-    for r in trace:
-        if time.perf_counter() - t_all > budget_s:
+    rng = np.random.default_rng(seed)
+    by = {k: list(rng.permutation([i for i, r in enumerate(trace) if r.k == k])) for k in (2, 3)}
+    n_all = len(trace)
+    share3 = len(by[3]) / n_all if n_all else 0.0
+    lat = {2: [], 3: []}
+    t_all = time.perf_counter()
+    done = 0
+    while by[2] or by[3]:
+        if requests is not None and done >= requests:
             break
+        if requests is None and time.perf_counter() - t_all > budget_s:
+            break
+        # the class whose share of the sample is furthest below its share of the trace
+        k = 3 if by[3] and (not by[2] or len(lat[3]) < share3 * (done + 1)) else 2
+        r = trace[by[k].pop()]
         x = np.ascontiguousarray(crop_of(r, src2d, src3d))
         t0 = time.perf_counter()
         ops[r.k].solve(x, r.gamma, tol=r.tol, max_iter=100000, threads=threads)
-        lat.append(time.perf_counter() - t0)
-    lat = np.array(lat) * 1e3
-    return {"requests": int(lat.size), "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()), "cores": threads,
-            "kind": "reference", "sample": f"the first {lat.size} requests of the trace, one fp_prox call each"}
+        lat[k].append(time.perf_counter() - t0)
+        done += 1
+    both = np.array(lat[2] + lat[3]) * 1e3
+    out = {"requests": int(both.size), "p50_ms": float(np.percentile(both, 50)),
+           "p99_ms": float(np.percentile(both, 99)), "mean_ms": float(both.mean()), "cores": threads,
+           "host": host_cpu(), "kind": "reference",
+           "sample": f"{both.size} requests of the trace drawn at random, stratified by class ({len(lat[2])} 2D, "
+                     f"{len(lat[3])} 3D), one fp_prox call each, fp_options.threads={threads}"}
+    for k in (2, 3):
+        if lat[k]:
+            a = np.array(lat[k]) * 1e3
+            out[f"{k}d"] = {"requests": int(a.size), "p50_ms": float(np.percentile(a, 50)),
+                            "p99_ms": float(np.percentile(a, 99))}
+    return out
 
 
 def bt_kernel(bt):

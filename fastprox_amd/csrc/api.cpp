@@ -2211,6 +2211,20 @@ FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_sy
     return FP_OK;
 }
 
+FP_API fp_status fpr_batch_pixel_passes(const fpr_batch *b, double *pixel_passes) {
+    if (!b || !pixel_passes) return FP_EINVAL;
+    double n = 0.0;
+    if (b->eng && b->valid) {
+        const fpr::ImgState *S = b->eng->state();
+        for (size_t i = 0; i < b->specs.size(); ++i) {
+            const fpr::ImageSpec &sp = b->specs[i];
+            n += (double)(sp.L[0] * sp.L[1] * sp.L[2]) * (double)S[i].passes;
+        }
+    }
+    *pixel_passes = n;
+    return FP_OK;
+}
+
 FP_API fp_status fpr_batch_set_timing(fpr_batch *b, int on) {
     if (!b || !b->eng) return FP_EINVAL;
     b->eng->timing = on != 0;

@@ -86,6 +86,7 @@ struct ImgState {
     int32_t converged;
     int32_t status;    // 0 ok; 1 an infinite entry (FP_EINVAL)
     int32_t active_axis;  // kTV1D: the active axis
+    int32_t passes;    // step launches that iterated the image (each one HBM pass over its state)
     double tv1d_lam;   // kTV1D: gamma_s w[a]
     double zerr;       // certified relative error (filled when done)
 };

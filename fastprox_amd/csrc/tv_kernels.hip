@@ -301,6 +301,7 @@ __global__ __launch_bounds__(64) void epilogue_kernel(const ImgDesc *desc, ImgSt
     __syncthreads();
     __shared__ int base;
     if (lane == 0) {
+        if ((mode == kIterate || mode == kOutput) && cnt > 0) ++S.passes;
         walk(S, gaps, dots, TT, N, u_round, eta);
         const int m = S.mode;
         base = (m == kIterate || m == kOutput || m == kOutputNaN) ? atomicAdd(count_out, D.ntiles) : -1;
@@ -570,6 +571,7 @@ __global__ void prologue_combine2(Problem P, const ImgDesc *desc, ImgState *stat
     const double tol = tols ? tols[img] : P.tol;
     S.target = tol * 0.5 * s2;
     S.active_axis = -1;
+    S.passes = 0;
     S.tv1d_lam = 0.0;
     for (int a = 0; a < 3; ++a) S.wa[a] = 0.0;
     if (S.status) {  // an infinite entry: the call fails (FP_EINVAL)

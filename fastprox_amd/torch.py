@@ -286,6 +286,8 @@ def _declare_batch(L):
     L.fpr_batch_set_kernel.argtypes = [P, C.c_int, C.c_int, C.c_int]
     L.fpr_batch_stats.restype = C.c_int
     L.fpr_batch_stats.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fpr_batch_pixel_passes.restype = C.c_int
+    L.fpr_batch_pixel_passes.argtypes = [P, C.POINTER(C.c_double)]
     L.fpr_batch_set_timing.restype = C.c_int
     L.fpr_batch_set_timing.argtypes = [P, C.c_int]
     L.fpr_batch_set_graph.restype = C.c_int
@@ -354,6 +356,13 @@ class Batch:
         a, b = C.c_int(), C.c_int()
         _check(lib().fpr_batch_stats(self.h, C.byref(a), C.byref(b)), "fpr_batch_stats")
         return a.value, b.value
+
+    def pixel_passes(self):
+        """The last solve's pixels times the step launches that iterated them
+        (fpr_batch_pixel_passes)."""
+        v = C.c_double()
+        _check(lib().fpr_batch_pixel_passes(self.h, C.byref(v)), "fpr_batch_pixel_passes")
+        return v.value
 
     def set_tail(self, tt, ry, frac):
         """Once the live tiles drop below frac, continue on the streaming

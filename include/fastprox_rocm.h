@@ -289,6 +289,12 @@ FP_API fp_status fpr_batch_image_info(const fpr_batch *b, double *gaps, int *ite
 FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry);
 FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry);
 FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs);
+/* The last solve's pixel passes: sum over the images of the pixels times the
+ * step launches that iterated the image (each launch reads and writes an
+ * image's state once, whatever its iterations): the solve's algorithmic HBM
+ * traffic is this times (1 + 4k) words, plus one word per pixel with pixel
+ * weights. No counterpart in the reference. */
+FP_API fp_status fpr_batch_pixel_passes(const fpr_batch *b, double *pixel_passes);
 /* Kernel timing of the solves on b (HIP events on the solve's stream around
  * every step and epilogue launch): on = 1 enables it for later solves; the
  * last solve's summed step-kernel and epilogue milliseconds and the number of
