@@ -33,6 +33,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -113,9 +115,25 @@ __device__ __forceinline__ void st32(gptr<T> p, uint32_t o, T v) {
     *(gptr<T>)((gptr<char>)p + o * (uint32_t)sizeof(T)) = v;
 }
 
+// A pointer every lane holds the same value of, moved to scalar registers:
+// accesses at it plus a 32-bit per-lane byte offset then take the
+// scalar-base form (one 32-bit offset register per lane, no 64-bit address
+// arithmetic), which the compiler does not find by itself when the base is
+// built from fields loaded into vector registers.
+template <class P>
+__device__ __forceinline__ P uniform(P p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (P)(((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
+// x 2^-e: exact unless the result leaves the type's range, so float64 and
+// float32 give the correctly rounded product directly (one ldexp).
 template <class T>
 __device__ __forceinline__ T scale_in(T v, int e) {
-    return e ? (T)ldexp((double)v, -e) : v;
+    if constexpr (sizeof(T) == 4) return e ? ldexpf(v, -e) : v;
+    else return e ? ldexp(v, -e) : v;
 }
 
 // The radius of pixel p with pixel weights: tv_iso.cpp:180-185.
@@ -1124,9 +1142,24 @@ struct StreamAcc {
     double v[2][TT + 1][64];
 };
 
-template <class T, bool ISO, bool SW, int TT>
+// The float32 gap term's square root: the hardware's (about 1 ulp), the
+// term itself being a float32 quantity summed in FP64.
+__device__ __forceinline__ float sqrt32(float v) { return __builtin_amdgcn_sqrtf(v); }
+
+// One tile of the streaming 2D kernel. INTERIOR: the segment's rows and the
+// pipeline's TT rows (and one more) on either side lie inside the image, so no
+// level tests a row bound. The column bounds and inactive axes are folded
+// into per-lane weights (0 where the difference leaves the image), which
+// float64 multiplies exactly as the reference's select (a 0 weight gives a
+// zero difference term, added to the axis's zero dual). float32 computes in
+// its own order (tau folded into the weights, the gap and restart terms per
+// pixel in float32) and sums the partials in FP64 as float64 does: its
+// iterate is float32 either way.
+template <class T, bool ISO, bool SW, int TT, bool INTERIOR>
 __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl, const int lane, const int SEG,
                                             StreamAcc<TT> &acc) {
+    constexpr bool F32 = sizeof(T) == 4;
+    using G = std::conditional_t<F32, float, double>;  // gap and restart terms per pixel
     constexpr int TXI = 64 - 2 * TT;
     const int img = __builtin_amdgcn_readfirstlane(tl.img);
     const int tx = __builtin_amdgcn_readfirstlane(tl.tx);
@@ -1167,15 +1200,22 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     const T w0 = (T)w0d, w1 = (T)w1d;
     const bool act0 = w0d != 0.0, act1 = w1d != 0.0;
     const bool out_mode = mode == kOutput;
-    // Plane bases (uniform); every access below is base + a 32-bit offset.
-    const auto dp = gin<T>(D.d);
-    const gptr<const T> U0 = (gptr<const T>)(A.arena + D.st_off + (int64_t)S.cur * 4 * N), U1 = U0 + N;
-    const gptr<const T> V0 = S.vsrc_u ? U0 : U0 + 2 * N, V1 = V0 + N;
-    const gptr<T> Uo = (gptr<T>)(A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N);
-    const gptr<T> Uo1 = Uo + N, Uo2 = Uo + 2 * N, Uo3 = Uo + 3 * N;
-    const auto sw = SW ? gin<T>(D.sw) : gptr<const T>(nullptr);
-    const auto swb = SW && !ISO ? sw + D.sw_cstride : sw;
-    const auto zo = gout<T>(D.z), qo0 = gout<T>(D.q), qo1 = qo0 + D.q_cstride;
+    // Per-lane weights of the two differences, 0 where the difference leaves
+    // the image or the axis is inactive: of g (gw*) and, float32, of tau g (tw*).
+    const T gw0 = (act0 && xin) ? w0 : (T)0, gw1 = (act1 && xnext) ? w1 : (T)0;
+    const T tw0 = F32 ? tau * gw0 : gw0, tw1 = F32 ? tau * gw1 : gw1;
+    const G gg0 = (G)gw0, gg1 = (G)gw1;
+    // Plane bases (uniform, in scalar registers); every access below is base
+    // + a 32-bit offset.
+    const auto dp = uniform(gin<T>(D.d));
+    const gptr<const T> U0 = uniform((gptr<const T>)(A.arena + D.st_off + (int64_t)S.cur * 4 * N));
+    const gptr<const T> U1 = uniform(U0 + N);
+    const gptr<const T> V0 = uniform(S.vsrc_u ? U0 : U0 + 2 * N), V1 = uniform(V0 + N);
+    const gptr<T> Uo = uniform((gptr<T>)(A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 4 * N));
+    const gptr<T> Uo1 = uniform(Uo + N), Uo2 = uniform(Uo + 2 * N), Uo3 = uniform(Uo + 3 * N);
+    const auto sw = SW ? uniform(gin<T>(D.sw)) : gptr<const T>(nullptr);
+    const auto swb = SW && !ISO ? uniform(sw + D.sw_cstride) : sw;
+    const auto zo = uniform(gout<T>(D.z)), qo0 = uniform(gout<T>(D.q)), qo1 = uniform(qo0 + D.q_cstride);
     T beta[TT + 1];
 #pragma unroll
     for (int m = 1; m <= TT; ++m) beta[m] = (T)S.beta[m];
@@ -1185,15 +1225,15 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
         if ((tm >> m) & 1u) acc.v[1][m][lane] = 0.0;
     }
 
-    // Per level m < TT: its latest row's v, u, z(v) and z(u) (FP64).
+    // Per level m < TT: its latest row's v, u, z(v) and z(u).
     T ov0[TT], ov1[TT], ou0[TT], ou1[TT], oz[TT];
-    double ozu[TT];
+    G ozu[TT];
     T dw[TT + 2];                                          // dw[m] = x_s at row ys - m
     T ra[SW ? TT + 2 : 1], rb[(SW && !ISO) ? TT + 2 : 1];  // radii at row ys - m
 #pragma unroll
     for (int m = 0; m < TT; ++m) {
         ov0[m] = ov1[m] = ou0[m] = ou1[m] = oz[m] = (T)0;
-        ozu[m] = 0.0;
+        ozu[m] = (G)0;
     }
 #pragma unroll
     for (int m = 0; m < TT + 2; ++m) {
@@ -1204,16 +1244,29 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     T uf0 = (T)0;  // the final level's previous row, u0
 
     // The gap term of one pixel (tv_iso.cpp:290-306; _tv_torch.py:179-186).
-    auto gap_term = [&](double g0, double g1, double u0, double u1, double r0, double r1) -> double {
-        if constexpr (ISO) return r0 * gap_sqrt<T>(g0 * g0 + g1 * g1) - (g0 * u0 + g1 * u1);
-        else return (r0 * fabs(g0) - g0 * u0) + (r1 * fabs(g1) - g1 * u1);
+    auto gap_term = [&](G g0, G g1, G u0, G u1, G r0, G r1) -> G {
+        if constexpr (ISO) {
+            if constexpr (F32) return r0 * sqrt32(g0 * g0 + g1 * g1) - (g0 * u0 + g1 * u1);
+            else return r0 * gap_sqrt<T>(g0 * g0 + g1 * g1) - (g0 * u0 + g1 * u1);
+        } else {
+            return (r0 * fabs(g0) - g0 * u0) + (r1 * fabs(g1) - g1 * u1);
+        }
     };
+    // z of row r from its v (or u) and the row above's: float32 and float64
+    // alike, d + w0 (a0 - a0_above) + w1 (a1 - a1_left).
+    auto zrow = [&](G d, G a0, G a0up, G a1) -> G {
+        if constexpr (F32) return madd(w1, a1 - from_left(a1), madd(w0, a0 - a0up, d));
+        else return (d + w0d * (a0 - a0up)) + w1d * (a1 - from_left(a1));
+    };
+    // Row-bound tests of the general tiles (constant true for interior ones).
+    auto row_g0 = [&](int r) { return INTERIOR || (r >= 0 && r + 1 < H); };
+    auto row_g1 = [&](int r) { return INTERIOR || (r >= 0 && r < H); };
 
     const int ystart = y0 - TT, ylast = yend - 1 + TT;
     // Row ys + 1's input, loaded one step ahead.
     T qd = (T)0, qu0 = (T)0, qu1 = (T)0, qv0 = (T)0, qv1 = (T)0, qa = (T)0, qb = (T)0;
     auto load_row = [&](int ys) {
-        const bool in = xin && ys >= 0 && ys < H;
+        const bool in = xin && (INTERIOR || (ys >= 0 && ys < H));
         const uint32_t o = in ? (uint32_t)(ys * W + x) : 0u;
         qd = in ? ld32<T>(dp, o) : (T)0;
         qu0 = qu1 = qv0 = qv1 = (T)0;
@@ -1249,22 +1302,22 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
         }
         if (ys < ylast) load_row(ys + 1);
         T zn = madd(w1, nv1 - from_left(nv1), madd(w0, nv0 - ov0[0], dw[0]));
-        double zun = 0.0;
+        G zun = (G)0;
         // The gap of level 0 (the loaded state) on row ys - 1.
         if (gm & 1u) {
-            zun = ((double)dw[0] + w0d * ((double)nu0 - (double)ou0[0])) + w1d * ((double)nu1 - from_left((double)nu1));
+            zun = zrow((G)dw[0], (G)nu0, (G)ou0[0], (G)nu1);
             const int r = ys - 1;
-            const double zc = ozu[0], zr = from_right(zc);
-            const double g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0d * (zun - zc) : 0.0;
-            const double g1 = (act1 && xnext && r >= 0 && r < H) ? w1d * (zr - zc) : 0.0;
-            const double gp = gap_term(g0, g1, (double)ou0[0], (double)ou1[0], SW ? (double)ra[1] : gamd,
-                                       SW && !ISO ? (double)rb[1] : gamd);
-            if (lane_in && r >= y0 && r < yend) acc.v[0][0][lane] += gp;
+            const G zc = ozu[0], zr = from_right(zc);
+            const G g0 = row_g0(r) ? gg0 * (zun - zc) : (G)0;
+            const G g1 = row_g1(r) ? gg1 * (zr - zc) : (G)0;
+            const G gp = gap_term(g0, g1, (G)ou0[0], (G)ou1[0], SW ? (G)ra[1] : (G)gamd,
+                                  SW && !ISO ? (G)rb[1] : (G)gamd);
+            if (lane_in && r >= y0 && r < yend) acc.v[0][0][lane] += (double)gp;
         }
         // The new row's values rise through the levels.
         T pv0 = nv0, pv1 = nv1, pu0 = nu0, pu1 = nu1;  // level m - 1's new row
         T pz = zn;
-        double pzu = zun;
+        G pzu = zun;
 #pragma unroll
         for (int m = 1; m <= TT; ++m) {
             const int r = ys - m;  // the row level m produces
@@ -1277,12 +1330,19 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             const T zr = from_right(zc);
             T v0n, v1n, u0n, u1n;
             if (m <= cnt && need) {
-                // An inactive axis has w = 0 and v = 0 (cold start, or zeroed by the
-                // warm start), so its c is 0 as the reference's select makes it.
-                const T g0 = (act0 && xin && r >= 0 && r + 1 < H) ? w0 * (pz - zc) : (T)0;
-                const T g1 = (act1 && xnext && r >= 0 && r < H) ? w1 * (zr - zc) : (T)0;
-                const T c0 = madd(tau, g0, ov0[m - 1]);
-                const T c1 = madd(tau, g1, ov1[m - 1]);
+                // An inactive axis has a 0 weight and v = 0 (cold start, or
+                // zeroed by the warm start), so its c is 0 as the reference's
+                // select makes it.
+                T c0, c1;
+                if constexpr (F32) {
+                    c0 = madd(row_g0(r) ? tw0 : (T)0, pz - zc, ov0[m - 1]);
+                    c1 = madd(row_g1(r) ? tw1 : (T)0, zr - zc, ov1[m - 1]);
+                } else {
+                    const T g0 = row_g0(r) ? gw0 * (pz - zc) : (T)0;
+                    const T g1 = row_g1(r) ? gw1 * (zr - zc) : (T)0;
+                    c0 = madd(tau, g0, ov0[m - 1]);
+                    c1 = madd(tau, g1, ov1[m - 1]);
+                }
                 T n0, n1;
                 if constexpr (ISO) {
                     const T sc = ball_scale(madd(c1, c1, c0 * c0), SW ? ra[m] : gam);
@@ -1294,9 +1354,9 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                     n1 = c1 > r1 ? r1 : (c1 < -r1 ? -r1 : c1);
                 }
                 if ((tm >> m) & 1u) {
-                    const double dd = ((double)ov0[m - 1] - (double)n0) * ((double)n0 - (double)ou0[m - 1]) +
-                                      ((double)ov1[m - 1] - (double)n1) * ((double)n1 - (double)ou1[m - 1]);
-                    if (lane_in && r >= y0 && r < yend) acc.v[1][m][lane] += dd;
+                    const G dd = ((G)ov0[m - 1] - (G)n0) * ((G)n0 - (G)ou0[m - 1]) +
+                                 ((G)ov1[m - 1] - (G)n1) * ((G)n1 - (G)ou1[m - 1]);
+                    if (lane_in && r >= y0 && r < yend) acc.v[1][m][lane] += (double)dd;
                 }
                 v0n = madd(beta[m], n0 - ou0[m - 1], n0);
                 v1n = madd(beta[m], n1 - ou1[m - 1], n1);
@@ -1319,15 +1379,14 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                 if (m < cnt && need) pz = madd(w1, v1n - from_left(v1n), madd(w0, v0n - ov0[m], dw[m]));
                 if (((gm >> m) & 1u) && need) {
                     // The gap of level m on row r - 1.
-                    pzu = ((double)dw[m] + w0d * ((double)u0n - (double)ou0[m])) +
-                          w1d * ((double)u1n - from_left((double)u1n));
+                    pzu = zrow((G)dw[m], (G)u0n, (G)ou0[m], (G)u1n);
                     const int rg = r - 1;
-                    const double zc2 = ozu[m], zr2 = from_right(zc2);
-                    const double g0 = (act0 && xin && rg >= 0 && rg + 1 < H) ? w0d * (pzu - zc2) : 0.0;
-                    const double g1 = (act1 && xnext && rg >= 0 && rg < H) ? w1d * (zr2 - zc2) : 0.0;
-                    const double gp = gap_term(g0, g1, (double)ou0[m], (double)ou1[m],
-                                               SW ? (double)ra[m + 1] : gamd, SW && !ISO ? (double)rb[m + 1] : gamd);
-                    if (lane_in && rg >= y0 && rg < yend) acc.v[0][m][lane] += gp;
+                    const G zc2 = ozu[m], zr2 = from_right(zc2);
+                    const G g0 = row_g0(rg) ? gg0 * (pzu - zc2) : (G)0;
+                    const G g1 = row_g1(rg) ? gg1 * (zr2 - zc2) : (G)0;
+                    const G gp = gap_term(g0, g1, (G)ou0[m], (G)ou1[m], SW ? (G)ra[m + 1] : (G)gamd,
+                                          SW && !ISO ? (G)rb[m + 1] : (G)gamd);
+                    if (lane_in && rg >= y0 && rg < yend) acc.v[0][m][lane] += (double)gp;
                 }
                 pv0 = v0n;
                 pv1 = v1n;
@@ -1399,7 +1458,13 @@ void stream2d_kernel(StepArgs<T> A, int SEG) {
     const int cend = min((xcd + 1) * cpx, nchunks);
     for (int c = xcd * cpx + j8; c < cend; c += per) {
         const int ti = c * 4 + wv;
-        if (ti < ntiles) stream_tile<T, ISO, SW, TT>(A, A.list_in[ti], lane, SEG, accs[wv]);
+        if (ti >= ntiles) continue;
+        const Tile tl = A.list_in[ti];
+        // rows y0 - TT - 1 .. yend + TT inside the image: the interior form
+        const int H = A.desc[__builtin_amdgcn_readfirstlane(tl.img)].L[1];
+        const int y0 = __builtin_amdgcn_readfirstlane(tl.ty) * SEG;
+        if (y0 - TT - 1 >= 0 && y0 + SEG + TT < H) stream_tile<T, ISO, SW, TT, true>(A, tl, lane, SEG, accs[wv]);
+        else stream_tile<T, ISO, SW, TT, false>(A, tl, lane, SEG, accs[wv]);
     }
 }
 
