@@ -1149,9 +1149,9 @@ __device__ __forceinline__ float sqrt32(float v) { return __builtin_amdgcn_sqrtf
 // One tile of the streaming 2D kernel. INTERIOR: the segment's rows and the
 // pipeline's TT rows (and one more) on either side lie inside the image, so no
 // level tests a row bound. The column bounds and inactive axes are folded
-// into per-lane weights (0 where the difference leaves the image), which
-// float64 multiplies exactly as the reference's select (a 0 weight gives a
-// zero difference term, added to the axis's zero dual). float32 computes in
+// into per-lane weights for float32 (0 where the difference leaves the
+// image); float64 keeps the reference's select on the lane's masks (doubles of
+// such weights would cost the registers of a resident wave). float32 computes in
 // its own order (tau folded into the weights, the gap and restart terms per
 // pixel in float32) and sums the partials in FP64 as float64 does: its
 // iterate is float32 either way.
@@ -1203,8 +1203,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     // Per-lane weights of the two differences, 0 where the difference leaves
     // the image or the axis is inactive: of g (gw*) and, float32, of tau g (tw*).
     const T gw0 = (act0 && xin) ? w0 : (T)0, gw1 = (act1 && xnext) ? w1 : (T)0;
-    const T tw0 = F32 ? tau * gw0 : gw0, tw1 = F32 ? tau * gw1 : gw1;
-    const G gg0 = (G)gw0, gg1 = (G)gw1;
+    const T tw0 = tau * gw0, tw1 = tau * gw1;  // float32 only
+    const G gg0 = (G)gw0, gg1 = (G)gw1;        // float32 only
     // Plane bases (uniform, in scalar registers); every access below is base
     // + a 32-bit offset.
     const auto dp = uniform(gin<T>(D.d));
@@ -1308,8 +1308,10 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             zun = zrow((G)dw[0], (G)nu0, (G)ou0[0], (G)nu1);
             const int r = ys - 1;
             const G zc = ozu[0], zr = from_right(zc);
-            const G g0 = row_g0(r) ? gg0 * (zun - zc) : (G)0;
-            const G g1 = row_g1(r) ? gg1 * (zr - zc) : (G)0;
+            const G g0 = F32 ? (row_g0(r) ? gg0 * (zun - zc) : (G)0)
+                             : ((act0 && xin && row_g0(r)) ? (G)w0d * (zun - zc) : (G)0);
+            const G g1 = F32 ? (row_g1(r) ? gg1 * (zr - zc) : (G)0)
+                             : ((act1 && xnext && row_g1(r)) ? (G)w1d * (zr - zc) : (G)0);
             const G gp = gap_term(g0, g1, (G)ou0[0], (G)ou1[0], SW ? (G)ra[1] : (G)gamd,
                                   SW && !ISO ? (G)rb[1] : (G)gamd);
             if (lane_in && r >= y0 && r < yend) acc.v[0][0][lane] += (double)gp;
@@ -1338,8 +1340,10 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                     c0 = madd(row_g0(r) ? tw0 : (T)0, pz - zc, ov0[m - 1]);
                     c1 = madd(row_g1(r) ? tw1 : (T)0, zr - zc, ov1[m - 1]);
                 } else {
-                    const T g0 = row_g0(r) ? gw0 * (pz - zc) : (T)0;
-                    const T g1 = row_g1(r) ? gw1 * (zr - zc) : (T)0;
+                    // float64: the lane masks (doubles of folded weights would
+                    // cost the registers of a resident wave)
+                    const T g0 = (act0 && xin && row_g0(r)) ? w0 * (pz - zc) : (T)0;
+                    const T g1 = (act1 && xnext && row_g1(r)) ? w1 * (zr - zc) : (T)0;
                     c0 = madd(tau, g0, ov0[m - 1]);
                     c1 = madd(tau, g1, ov1[m - 1]);
                 }
@@ -1382,8 +1386,10 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                     pzu = zrow((G)dw[m], (G)u0n, (G)ou0[m], (G)u1n);
                     const int rg = r - 1;
                     const G zc2 = ozu[m], zr2 = from_right(zc2);
-                    const G g0 = row_g0(rg) ? gg0 * (pzu - zc2) : (G)0;
-                    const G g1 = row_g1(rg) ? gg1 * (zr2 - zc2) : (G)0;
+                    const G g0 = F32 ? (row_g0(rg) ? gg0 * (pzu - zc2) : (G)0)
+                                     : ((act0 && xin && row_g0(rg)) ? (G)w0d * (pzu - zc2) : (G)0);
+                    const G g1 = F32 ? (row_g1(rg) ? gg1 * (zr2 - zc2) : (G)0)
+                                     : ((act1 && xnext && row_g1(rg)) ? (G)w1d * (zr2 - zc2) : (G)0);
                     const G gp = gap_term(g0, g1, (G)ou0[m], (G)ou1[m], SW ? (G)ra[m + 1] : (G)gamd,
                                           SW && !ISO ? (G)rb[m + 1] : (G)gamd);
                     if (lane_in && rg >= y0 && rg < yend) acc.v[0][m][lane] += (double)gp;
@@ -1440,7 +1446,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
 // the compiler's choice). More resident waves hide the per-row load latency.
 template <class T, bool ISO, bool SW, int TT>
 constexpr int stream2d_waves() {
-    if (sizeof(T) != 4) return 1;
+    if (sizeof(T) != 4) return (!SW && TT <= 6) ? 3 : 1;  // float64: three where 168 registers suffice
     if (!SW) return TT <= 6 ? 4 : 3;
     return TT <= (ISO ? 2 : 4) ? 4 : 3;
 }
@@ -1449,7 +1455,9 @@ template <class T, bool ISO, bool SW, int TT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stream2d_waves<T, ISO, SW, TT>())))
 void stream2d_kernel(StepArgs<T> A, int SEG) {
     __shared__ StreamAcc<TT> accs[4];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // the wave's index is the same on all its lanes: in a scalar register, so
+    // that the tile loop compiles as uniform control flow
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;
     const int ntiles = *A.count_in;
     const int nchunks = (ntiles + 3) >> 2;
