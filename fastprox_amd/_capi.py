@@ -65,6 +65,10 @@ def numpy_array(a):
     return Array(a.ctypes.data, dt, a.shape, strides, FP_DEVICE_CPU)
 
 
+FPR_METHOD_ADBPG, FPR_METHOD_FISTA = 0, 1
+METHODS = {"adbpg": FPR_METHOD_ADBPG, "fista": FPR_METHOD_FISTA}
+
+
 def declare(lib):
     """Argument and result types of the fp_* entry points present in lib."""
     P, I, D, I64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
@@ -111,6 +115,11 @@ def declare(lib):
         "fpr_alloc_count": (I64, []),
         "fpr_last_error": (C.c_char_p, []),
         "fpr_prox_static": (I, [P, D, A, A, C.POINTER(fp_options), P, C.POINTER(fp_info)]),
+        "fpr_workspace_set_method": (I, [P, I]),
+        "fpr_workspace_get_method": (I, [P, C.POINTER(I)]),
+        "fpr_workspace_polished": (I, [P, C.POINTER(I), I64, C.POINTER(I)]),
+        "fpr_batch_set_method": (I, [P, I]),
+        "fpr_batch_get_method": (I, [P, C.POINTER(I)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name, None)

@@ -114,7 +114,7 @@ class Workspace:
     every solve for arrays shaped like `like`; calls on one workspace must
     not overlap. stream: a HIP stream handle (int) or None."""
 
-    def __init__(self, op, like, keep_structure=False, stream=None, kernel=None):
+    def __init__(self, op, like, keep_structure=False, stream=None, kernel=None, method=None):
         self.op = op
         self.h = C.c_void_p()
         L = lib()
@@ -129,6 +129,27 @@ class Workspace:
             self.set_stream(stream)
         if kernel is not None:
             self.set_kernel(*kernel)
+        if method is not None:
+            self.set_method(method)
+
+    def set_method(self, method):
+        """fpr_workspace_set_method: "adbpg" (the reference CPU library's
+        method, the default for anisotropic TV without edge weights) or
+        "fista" (dual FISTA on the edge duals)."""
+        _check(lib().fpr_workspace_set_method(self.h, A.METHODS[method]), "set_method")
+
+    def method(self):
+        m = C.c_int()
+        _check(lib().fpr_workspace_get_method(self.h, C.byref(m)), "get_method")
+        return {v: k for k, v in A.METHODS.items()}[m.value]
+
+    def polished(self, n):
+        """Per image of the last ADBPG solve, whether its polish certified
+        it; and the fibres that took the taut string."""
+        p = (C.c_int * max(n, 1))()
+        f = C.c_int()
+        _check(lib().fpr_workspace_polished(self.h, p, n, C.byref(f)), "polished")
+        return [bool(p[i]) for i in range(n)], f.value
 
     def set_stream(self, stream):
         _check(lib().fp_workspace_set_stream(self.h, C.c_void_p(int(stream) if stream else None)), "set_stream")

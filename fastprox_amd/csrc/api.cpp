@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -445,6 +446,31 @@ fp_status build_engine(fp_workspace &w) {
     return FP_OK;
 }
 
+// The ADBPG solver of an anisotropic workspace (its buffers for ws's shape).
+fp_status build_ad(fp_workspace &w) {
+    const fp_op &op = *w.op;
+    fpr::AdSpec sp;
+    sp.k = op.k;
+    for (int a = 0; a < 3; ++a) sp.L[a] = w.img[a];
+    sp.B = w.B;
+    sp.tsize = (int)w.tsize;
+    for (int a = 0; a < 3; ++a) sp.w[a] = op.w[a];
+    fpr::AdSolver *a = nullptr;
+    if (int r = fpr::ad_reserve(sp, &a)) return (fp_status)r;
+    w.ad.reset(a);
+    w.ptrs.assign((size_t)w.B, fpr::ImagePtrs{});
+    return FP_OK;
+}
+
+// The default method of an anisotropic workspace: the reference's (ADBPG)
+// unless the operator has edge weights or FASTPROX_ROCM_ANISO=fista.
+int default_method(const fp_op &op) {
+    if (op.iso || !op.spatial.empty()) return FPR_METHOD_FISTA;
+    const char *e = std::getenv("FASTPROX_ROCM_ANISO");
+    if (e && std::strcmp(e, "fista") == 0) return FPR_METHOD_FISTA;
+    return FPR_METHOD_ADBPG;
+}
+
 // kind 0: the streaming kernel (tt iterations, segments of ry rows); 1: the
 // generic one-iteration kernel; 2: the block kernel (tt, ry).
 fpr::KernelChoice generic_kernel() {
@@ -565,7 +591,12 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     if (fp_status s = check_image(*op, x.shape + (x.ndim - op->k))) return s;
     w->kc = fpr::default_kernel(op->k, (int)w->tsize, w->img, op->iso, !op->spatial.empty(), B);
     if (w->kc.stream && !fpr::stream_fits((int)w->tsize, w->N)) w->kc = generic_kernel();
-    if (fp_status s = build_engine(*w)) return s;
+    w->method = default_method(*op);
+    if (w->method == FPR_METHOD_ADBPG) {
+        if (fp_status s = build_ad(*w)) return s;
+    } else if (fp_status s = build_engine(*w)) {
+        return s;
+    }
     const size_t bytes = (size_t)w->numel * w->tsize;
     if (fp_status s = alloc(&w->dstage, bytes)) return s;
     if (fp_status s = alloc(&w->zstage, bytes)) return s;
@@ -596,11 +627,24 @@ struct DevGuard {  // the workspace's device current for the call
 // 1216-1236, 1331-1341): gaps at the input's scale, iterations, certified
 // errors, fp_info; FP_EINVAL for an infinite entry.
 fp_status record_results(const fp_op *op, fp_workspace *ws, fp_info *info, fp_status *status) {
-    const fpr::ImgState *S = ws->eng->state();
     bool bad = false, notconv = false;
     double gsum = 0.0;
     int itmax = 0;
-    for (int64_t b = 0; b < ws->B; ++b) {
+    if (ws->ad) {
+        const fpr::AdResult *R = fpr::ad_results(ws->ad.get());
+        for (int64_t b = 0; b < ws->B; ++b) {
+            const fpr::AdResult &t = R[b];
+            if (t.status) bad = true;
+            ws->gaps[(size_t)b] = t.gap;
+            ws->iters[(size_t)b] = t.iters;
+            ws->zerrs[(size_t)b] = t.zerr;
+            if (!t.converged) notconv = true;
+            gsum += t.gap;
+            itmax = std::max(itmax, t.iters);
+        }
+    }
+    const fpr::ImgState *S = ws->ad ? nullptr : ws->eng->state();
+    for (int64_t b = 0; b < ws->B && S; ++b) {
         const fpr::ImgState &t = S[b];
         if (t.status) bad = true;
         double g = t.gp;
@@ -633,12 +677,14 @@ fp_status record_results(const fp_op *op, fp_workspace *ws, fp_info *info, fp_st
 // input's scale). Waits for the stream.
 fp_status record_kept(const fp_op *op, fp_workspace *ws, hipStream_t s) {
     const int k = op->k;
-    if (!op->iso && ws->eng->copy_dual_iterate(s, ws->usave, (int64_t)k * ws->N)) return FP_EDEVICE;
+    // ADBPG wrote the per-edge dual during the solve
+    if (!op->iso && !ws->ad && ws->eng->copy_dual_iterate(s, ws->usave, (int64_t)k * ws->N)) return FP_EDEVICE;
     if (hipStreamSynchronize(s)) return FP_EDEVICE;
-    const fpr::ImgState *S = ws->eng->state();
+    const fpr::ImgState *S = ws->ad ? nullptr : ws->eng->state();
+    const fpr::AdResult *R = ws->ad ? fpr::ad_results(ws->ad.get()) : nullptr;
     for (int64_t b = 0; b < ws->B; ++b) {
-        const fpr::ImgState &t = S[b];
-        ws->exps[(size_t)b] = t.e;
+        const int te = R ? R[b].e : S[b].e;
+        ws->exps[(size_t)b] = te;
         double mu = 0.0;
         if (op->iso) {
             double wmax = 0.0, wL = 0.0;
@@ -651,14 +697,44 @@ fp_status record_kept(const fp_op *op, fp_workspace *ws, hipStream_t s) {
                     wmax = std::max(wmax, op->w[a]);
                     ++na;
                 }
-            const double gs = std::ldexp(ws->gaps[(size_t)b], -2 * t.e);
+            const double gs = std::ldexp(ws->gaps[(size_t)b], -2 * te);
             if (na >= 2 && gs == gs) mu = 0.1 * wmax * std::sqrt(2.0 * std::max(gs, 0.0) / (double)ws->N);
         }
-        ws->smooth[(size_t)b] = std::ldexp(mu, t.e);
+        ws->smooth[(size_t)b] = std::ldexp(mu, te);
         ws->exact[(size_t)b] = !(mu > 0.0);
     }
     ws->valid = true;
     return FP_OK;
+}
+
+// One solve of ws's images from input d into z (device, contiguous), by the
+// workspace's method; ws->ptrs hold the engine's per-image pointers. keep:
+// the ADBPG solve also writes the per-edge dual the derivatives read.
+fp_status run_solve(fp_workspace *ws, hipStream_t s, const void *d, void *z, bool warm, double tol, int max_iter,
+                    bool keep) {
+    if (ws->ad) {
+        std::vector<const void *> xs((size_t)ws->B);
+        std::vector<void *> zs((size_t)ws->B), qs((size_t)ws->B);
+        for (int64_t b = 0; b < ws->B; ++b) {
+            const size_t off = (size_t)(b * ws->N) * ws->tsize;
+            xs[(size_t)b] = (const char *)d + off;
+            zs[(size_t)b] = (char *)z + off;
+            qs[(size_t)b] = (char *)ws->q + off;
+        }
+        fpr::AdCall c;
+        c.xs = xs.data();
+        c.zs = zs.data();
+        c.qs = qs.data();
+        c.q_plane = ws->B * ws->N;
+        c.usave = keep ? ws->usave : nullptr;
+        c.u_img = (int64_t)ws->k * ws->N;
+        c.gammas = ws->gammas.data();
+        c.warm = warm;
+        c.tol = tol;
+        c.max_iter = max_iter;
+        return (fp_status)fpr::ad_solve(ws->ad.get(), s, c);
+    }
+    return hip_status(ws->eng->solve(s, ws->ptrs.data(), ws->gammas.data(), warm, tol, max_iter));
 }
 
 fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *oa, const fp_options *opt,
@@ -769,11 +845,10 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
         p.q = (char *)ws->q + (size_t)(b * ws->N) * ws->tsize;
         p.sw = sw;
     }
-    int r = ws->eng->solve(s, ws->ptrs.data(), ws->gammas.data(), warm, o.tol, o.max_iter);
     ws->seeded = false;
-    if (r) {
+    if (fp_status r = run_solve(ws, s, d, z, warm, o.tol, o.max_iter, keep)) {
         ws->dual_valid = false;
-        return hip_status(r);
+        return r;
     }
     if (!z_direct)
         if (fp_status st2 = stage_out(*ws, z, out, s)) return st2;
@@ -909,11 +984,10 @@ fp_status refine_impl(const fp_op *op, fp_workspace *ws, double tol, int max_ite
         p.q = (char *)ws->q + (size_t)(b * ws->N) * ws->tsize;
         p.sw = sw;
     }
-    int r = ws->eng->solve(s, ws->ptrs.data(), ws->gammas.data(), true, tol, max_iter);
-    if (r) {
+    if (fp_status r = run_solve(ws, s, ws->xsave, ws->zsave, true, tol, max_iter, true)) {
         ws->dual_valid = false;
         ws->valid = false;
-        return hip_status(r);
+        return r;
     }
     fp_status status = FP_OK;
     if (fp_status st = record_results(op, ws, info, &status)) {
@@ -1256,11 +1330,19 @@ struct fpr_batch {
     int device = 0;
     std::vector<fpr::ImageSpec> specs;
     fpr::KernelChoice kc;
-    std::unique_ptr<fpr::Engine> eng;
+    std::unique_ptr<fpr::Engine> eng;                   // dual FISTA (and iso)
+    std::unique_ptr<fpr::AdSolver, fpr::AdDeleter> ad;  // aniso by the reference's method (uniform batches)
+    int method = FPR_METHOD_FISTA;
     std::vector<fpr::ImagePtrs> ptrs;
     std::vector<double> gammas, gaps, zerrs;
-    std::vector<int> iters, conv;
+    std::vector<int> iters, conv, exps;
     bool valid = false;
+    bool uniform() const {
+        for (size_t i = 1; i < specs.size(); ++i)
+            for (int a = 0; a < 3; ++a)
+                if (specs[i].L[a] != specs[0].L[a]) return false;
+        return true;
+    }
 };
 
 struct fpr_server {
@@ -1282,6 +1364,21 @@ struct fpr_deriv {
 };
 
 namespace {
+
+// The ADBPG solver of a uniform, unweighted anisotropic batch.
+fp_status batch_build_ad(fpr_batch &b) {
+    const fp_op &op = *b.op;
+    fpr::AdSpec sp;
+    sp.k = op.k;
+    for (int a = 0; a < 3; ++a) sp.L[a] = b.specs[0].L[a];
+    sp.B = (int64_t)b.specs.size();
+    sp.tsize = (int)b.tsize;
+    for (int a = 0; a < 3; ++a) sp.w[a] = op.w[a];
+    fpr::AdSolver *a = nullptr;
+    if (int r = fpr::ad_reserve(sp, &a)) return (fp_status)r;
+    b.ad.reset(a);
+    return FP_OK;
+}
 
 fp_status batch_build(fpr_batch &b) {
     const fp_op &op = *b.op;
@@ -1770,8 +1867,48 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
         ws->kc = kc;
         if (ws->numel == 0) return FP_OK;
         ws->dual_valid = false;
+        // a step kernel is dual FISTA's: an anisotropic workspace takes that method
+        ws->ad.reset();
+        ws->method = FPR_METHOD_FISTA;
         return build_engine(*ws);
     });
+}
+
+FP_API fp_status fpr_workspace_set_method(fp_workspace *ws, int method) {
+    return guard([&] {
+        g_last_error[0] = 0;
+        if (!ws || !ws->op) return fail(FP_EINVAL, "null workspace");
+        if (ws->op->iso) return fail(FP_EINVAL, "isotropic TV has one method");
+        if (method != FPR_METHOD_ADBPG && method != FPR_METHOD_FISTA) return fail(FP_EINVAL, "unknown method");
+        if (method == FPR_METHOD_ADBPG && !ws->op->spatial.empty())
+            return fail(FP_EINVAL, "edge-weighted anisotropic TV runs dual FISTA");
+        if (method == ws->method) return FP_OK;
+        ws->method = method;
+        ws->valid = false;
+        ws->deriv_built = false;
+        if (ws->numel == 0) return FP_OK;
+        DevGuard dg(ws->device);
+        if (method == FPR_METHOD_ADBPG) {
+            ws->eng.reset();
+            return build_ad(*ws);
+        }
+        ws->ad.reset();
+        return build_engine(*ws);
+    });
+}
+
+FP_API fp_status fpr_workspace_get_method(const fp_workspace *ws, int *method) {
+    if (!ws || !method) return FP_EINVAL;
+    *method = ws->method;
+    return FP_OK;
+}
+
+FP_API fp_status fpr_workspace_polished(const fp_workspace *ws, int *polished, int64_t count, int *fallbacks) {
+    if (!ws || !ws->ad || count < 0 || count > ws->B) return FP_EINVAL;
+    const fpr::AdResult *R = fpr::ad_results(ws->ad.get());
+    for (int64_t b = 0; b < count && polished; ++b) polished[b] = R[b].polished;
+    if (fallbacks) fpr::ad_stats(ws->ad.get(), nullptr, nullptr, fallbacks);
+    return FP_OK;
 }
 
 FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, int *tt, int *ry) {
@@ -1836,7 +1973,13 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
         b->zerrs.assign((size_t)count, 0.0);
         b->iters.assign((size_t)count, 0);
         b->conv.assign((size_t)count, 1);
-        if (fp_status s = batch_build(*b)) return s;
+        b->exps.assign((size_t)count, 0);
+        b->method = (count > 0 && !b->weighted && b->uniform()) ? default_method(*op) : FPR_METHOD_FISTA;
+        if (b->method == FPR_METHOD_ADBPG) {
+            if (fp_status s = batch_build_ad(*b)) return s;
+        } else if (fp_status s = batch_build(*b)) {
+            return s;
+        }
         *out = b.release();
         return FP_OK;
     });
@@ -1872,6 +2015,41 @@ FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp
         int cur = 0;
         (void)hipGetDevice(&cur);
         if (cur != b->device && hipSetDevice(b->device)) return FP_EDEVICE;
+        if (b->ad) {
+            std::vector<const void *> xs((size_t)B);
+            std::vector<void *> zs((size_t)B), qs((size_t)B);
+            for (int64_t i = 0; i < B; ++i) {
+                xs[(size_t)i] = images[i].x;
+                zs[(size_t)i] = images[i].z;
+                qs[(size_t)i] = images[i].dual;
+            }
+            fpr::AdCall c;
+            c.xs = xs.data();
+            c.zs = zs.data();
+            c.qs = qs.data();
+            c.q_plane = B ? b->specs[0].L[0] * b->specs[0].L[1] * b->specs[0].L[2] : 0;
+            c.gammas = b->gammas.data();
+            c.warm = warm != 0;
+            c.tol = o.tol;
+            c.max_iter = o.max_iter;
+            const int r = B ? fpr::ad_solve(b->ad.get(), (hipStream_t)stream, c) : 0;
+            if (cur != b->device) (void)hipSetDevice(cur);
+            if (r) return (fp_status)r;
+            const fpr::AdResult *R = fpr::ad_results(b->ad.get());
+            bool bad = false, notconv = false;
+            for (int64_t i = 0; i < B; ++i) {
+                if (R[i].status) bad = true;
+                b->gaps[(size_t)i] = R[i].gap;
+                b->iters[(size_t)i] = R[i].iters;
+                b->zerrs[(size_t)i] = R[i].zerr;
+                b->conv[(size_t)i] = R[i].converged;
+                b->exps[(size_t)i] = R[i].e;
+                if (!R[i].converged) notconv = true;
+            }
+            if (bad) return FP_EINVAL;
+            b->valid = true;
+            return notconv ? FP_ENOTCONV : FP_OK;
+        }
         const int r = b->eng->solve((hipStream_t)stream, b->ptrs.data(), b->gammas.data(), warm != 0, o.tol,
                                     o.max_iter);
         if (cur != b->device) (void)hipSetDevice(cur);
@@ -1888,6 +2066,7 @@ FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp
             b->iters[(size_t)i] = t.it;
             b->zerrs[(size_t)i] = t.zerr;
             b->conv[(size_t)i] = t.converged;
+            b->exps[(size_t)i] = t.e;
             if (!t.converged) notconv = true;
         }
         if (bad) return FP_EINVAL;
@@ -1919,10 +2098,40 @@ FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry)
         b->kc = kc;
         b->valid = false;
         const bool timing = b->eng && b->eng->timing;
+        // a step kernel is dual FISTA's: an anisotropic batch takes that method
+        b->ad.reset();
+        b->method = FPR_METHOD_FISTA;
         fp_status st = batch_build(*b);
         if (!st) b->eng->timing = timing;
         return st;
     });
+}
+
+FP_API fp_status fpr_batch_set_method(fpr_batch *b, int method) {
+    return guard([&] {
+        g_last_error[0] = 0;
+        if (!b || !b->op) return fail(FP_EINVAL, "null batch");
+        if (b->op->iso) return fail(FP_EINVAL, "isotropic TV has one method");
+        if (method != FPR_METHOD_ADBPG && method != FPR_METHOD_FISTA) return fail(FP_EINVAL, "unknown method");
+        if (method == b->method) return FP_OK;
+        if (method == FPR_METHOD_ADBPG && (b->weighted || !b->op->spatial.empty() || !b->uniform() || b->specs.empty()))
+            return fail(FP_EINVAL, "ADBPG takes uniform batches without edge weights");
+        b->method = method;
+        b->valid = false;
+        DevGuard dg(b->device);
+        if (method == FPR_METHOD_ADBPG) {
+            b->eng.reset();
+            return batch_build_ad(*b);
+        }
+        b->ad.reset();
+        return batch_build(*b);
+    });
+}
+
+FP_API fp_status fpr_batch_get_method(const fpr_batch *b, int *method) {
+    if (!b || !method) return FP_EINVAL;
+    *method = b->method;
+    return FP_OK;
 }
 
 FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry) {
@@ -1967,19 +2176,17 @@ FP_API fp_status fpr_workspace_set_tail(fp_workspace *ws, int tt, int ry, double
 FP_API fp_status fpr_batch_scale_exps(const fpr_batch *b, int *exps, int64_t count) {
     if (!b || !exps || !b->valid) return FP_EINVAL;
     if (count != (int64_t)b->specs.size()) return FP_ESHAPE;
-    const fpr::ImgState *S = b->eng->state();
-    for (int64_t i = 0; i < count; ++i) exps[i] = S[i].e;
+    for (int64_t i = 0; i < count; ++i) exps[i] = b->exps[(size_t)i];
     return FP_OK;
 }
 
 FP_API fp_status fpr_batch_copy_dual_iterate(const fpr_batch *b, void *dst, void *stream) {
     return guard([&] {
-        if (!b || !dst || !b->valid || !b->eng) return FP_EINVAL;
-        for (const fpr::ImageSpec &sp : b->specs)
-            for (int a = 0; a < 3; ++a)
-                if (sp.L[a] != b->specs[0].L[a]) return FP_ESHAPE;
+        if (!b || !dst || !b->valid || (!b->eng && !b->ad)) return FP_EINVAL;
+        if (!b->uniform()) return FP_ESHAPE;
         const int64_t N = b->specs.empty() ? 0 : b->specs[0].L[0] * b->specs[0].L[1] * b->specs[0].L[2];
         DevGuard dg(b->device);
+        if (b->ad) return (fp_status)fpr::ad_copy_edge_dual(b->ad.get(), (hipStream_t)stream, dst, (int64_t)b->op->k * N);
         return hip_status(b->eng->copy_dual_iterate((hipStream_t)stream, dst, (int64_t)b->op->k * N));
     });
 }
@@ -2206,6 +2413,10 @@ FP_API fp_status fpr_server_get_kernel(const fpr_server *s, int k, int *kind, in
 
 FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs) {
     if (!b) return FP_EINVAL;
+    if (b->ad) {
+        fpr::ad_stats(b->ad.get(), launches, host_syncs, nullptr);
+        return FP_OK;
+    }
     if (launches) *launches = b->eng ? b->eng->stats.launches : 0;
     if (host_syncs) *host_syncs = b->eng ? b->eng->stats.host_syncs : 0;
     return FP_OK;
@@ -2253,6 +2464,10 @@ FP_API fp_status fpr_batch_busy(const fpr_batch *b, double *busy_ms) {
 
 FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs) {
     if (!ws) return FP_EINVAL;
+    if (ws->ad) {
+        fpr::ad_stats(ws->ad.get(), launches, host_syncs, nullptr);
+        return FP_OK;
+    }
     if (launches) *launches = ws->eng ? ws->eng->stats.launches : 0;
     if (host_syncs) *host_syncs = ws->eng ? ws->eng->stats.host_syncs : 0;
     return FP_OK;

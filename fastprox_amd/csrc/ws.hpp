@@ -14,6 +14,7 @@
 #include "alloc.hpp"
 #include "../../include/fastprox_rocm.h"
 #include "deriv.hpp"
+#include "adbpg.hpp"
 #include "engine.hpp"
 
 namespace fpr {
@@ -82,7 +83,9 @@ struct fp_workspace {
     bool keep = false;
     size_t tsize = 8;
     fpr::KernelChoice kc;
-    std::unique_ptr<fpr::Engine> eng;
+    std::unique_ptr<fpr::Engine> eng;                       // iso, and aniso by dual FISTA
+    std::unique_ptr<fpr::AdSolver, fpr::AdDeleter> ad;      // aniso by the reference's method
+    int method = 0;                                         // aniso: FPR_METHOD_ADBPG or FPR_METHOD_FISTA
     void *dstage = nullptr, *zstage = nullptr, *q = nullptr;
     void *xsave = nullptr, *zsave = nullptr;  // keep_structure: the point (x, z) of the last kept call
     void *usave = nullptr;                    // keep_structure, aniso: the per-edge dual of that call

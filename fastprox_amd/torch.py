@@ -116,23 +116,31 @@ def tv_iso(x, gamma=1.0, *, ndim=2, weights=None, pixel_weights=None, tol=None, 
 
 
 def tv_aniso(x, gamma=1.0, *, ndim=2, weights=None, edge_weights=None, tol=None, ztol=None, max_iter=1000,
-             warm_start=None, return_info=False, deriv_tol=None, deriv_max_iter=10000, warn=True):
+             warm_start=None, return_info=False, deriv_tol=None, deriv_max_iter=10000, warn=True, method=None):
     r"""Anisotropic TV of the last ``ndim`` axes (torch.py:335-351 in the
-    reference), by the device method (dual FISTA on the edge duals); the
-    dual is in the C form q_a with x - z = sum_a q_a. ``edge_weights``:
-    (ndim, ...) broadcasting to (ndim, *x.shape)."""
+    reference); the dual is in the C form q_a with x - z = sum_a q_a.
+    ``edge_weights``: (ndim, ...) broadcasting to (ndim, *x.shape).
+    ``method``: "adbpg", the reference CPU library's method (exact 1D steps
+    on every fibre, block-dual FISTA, polish; the default without edge
+    weights), or "fista", dual FISTA on the edge duals (the reference's
+    device path, and the method with edge weights)."""
+    if method not in (None, "adbpg", "fista"):
+        raise ValueError('method must be "adbpg", "fista" or None')
+    if method == "adbpg" and edge_weights is not None:
+        raise ValueError('method "adbpg" takes no edge_weights')
     t = _call_tol(tol, ztol, 1e-6, warn)
     _check_floor(x, tol, ztol, warn)
     return _device_tv(False, x, gamma, ndim, weights, t, max_iter, warm_start, return_info, deriv_tol,
-                      deriv_max_iter, edge_weights, warn and "fastprox_amd.torch.tv_aniso")
+                      deriv_max_iter, edge_weights, warn and "fastprox_amd.torch.tv_aniso", method)
 
 
 class _Cfg:
     """The fixed settings of one call, shared by its forward and its
     derivatives (which record their CG residual and iterations in deriv)."""
 
-    def __init__(self, iso, k, w, tol, max_iter, deriv_tol, deriv_max_iter):
+    def __init__(self, iso, k, w, tol, max_iter, deriv_tol, deriv_max_iter, method=None):
         self.iso, self.k, self.w = iso, k, w
+        self.method = method
         self.tol, self.max_iter = float(tol), int(max_iter)
         self.deriv_tol, self.deriv_max_iter = deriv_tol, int(deriv_max_iter)
         self.info = None
@@ -232,7 +240,7 @@ def _spatial(sw, x, k, iso):
 
 
 def _device_tv(iso, x, gamma, ndim, weights, tol, max_iter, warm_start, return_info, deriv_tol, deriv_max_iter,
-               spatial, warn):
+               spatial, warn, method=None):
     """tv_iso and tv_aniso: the arguments checked (the reference's rules and
     messages, bindings/python/fastprox/torch.py:363-413), then one solve of
     every image of the batch on the device through _DeviceTV, which also
@@ -249,7 +257,7 @@ def _device_tv(iso, x, gamma, ndim, weights, tol, max_iter, warm_start, return_i
         if not deriv_tol > 0:
             raise ValueError("deriv_tol must be > 0 or None")
         deriv_tol = float(deriv_tol)
-    cfg = _Cfg(iso, ndim, w, tol, max_iter, deriv_tol, deriv_max_iter)
+    cfg = _Cfg(iso, ndim, w, tol, max_iter, deriv_tol, deriv_max_iter, method)
     sw = None if spatial is None else _spatial(spatial, x, ndim, iso)
     if x.numel() and bool(torch.isinf(x.detach()).any()):
         raise ValueError("x contains ±Inf; image operators reject infinite values")
@@ -300,6 +308,10 @@ def _declare_batch(L):
     L.fpr_batch_scale_exps.argtypes = [P, C.POINTER(C.c_int), C.c_int64]
     L.fpr_batch_copy_dual_iterate.restype = C.c_int
     L.fpr_batch_copy_dual_iterate.argtypes = [P, P, P]
+    L.fpr_batch_set_method.restype = C.c_int
+    L.fpr_batch_set_method.argtypes = [P, C.c_int]
+    L.fpr_batch_get_method.restype = C.c_int
+    L.fpr_batch_get_method.argtypes = [P, C.POINTER(C.c_int)]
     L._batch_declared = True
 
 
@@ -307,7 +319,7 @@ class Batch:
     """fpr_batch_create: one persistent solve over images of (possibly)
     different shapes, each with its own step and spatial weights."""
 
-    def __init__(self, op, dtype, shapes, weighted=False, kernel=None):
+    def __init__(self, op, dtype, shapes, weighted=False, kernel=None, method=None):
         L = lib()
         _declare_batch(L)
         self.op = op
@@ -321,9 +333,20 @@ class Batch:
         self.weighted = bool(weighted)
         if kernel is not None:
             self.set_kernel(*kernel)
+        if method is not None:
+            self.set_method(method)
 
     def set_kernel(self, generic, tt=1, ry=4):
         _check(lib().fpr_batch_set_kernel(self.h, int(generic), int(tt), int(ry)), "fpr_batch_set_kernel")
+
+    def set_method(self, method):
+        """fpr_batch_set_method: "adbpg" or "fista" (anisotropic batches)."""
+        _check(lib().fpr_batch_set_method(self.h, A.METHODS[method]), "fpr_batch_set_method")
+
+    def method(self):
+        m = C.c_int()
+        _check(lib().fpr_batch_get_method(self.h, C.byref(m)), "fpr_batch_get_method")
+        return {v: k for k, v in A.METHODS.items()}[m.value]
 
     def solve(self, xs, zs, duals, gammas, weights=None, tol=1e-6, max_iter=1000, warm=False, stream=None):
         """Lists of device tensors (contiguous); returns the status."""
@@ -418,21 +441,21 @@ _CACHE_LOCK = threading.Lock()
 _CACHE_MAX = 16
 
 
-def _batch_for(iso, k, w, dtype, device, B, img, weighted):
-    key = (iso, k, tuple(w), dtype, device, B, img, weighted)
+def _batch_for(iso, k, w, dtype, device, B, img, weighted, method=None):
+    key = (iso, k, tuple(w), dtype, device, B, img, weighted, method)
     with _CACHE_LOCK:
         hit = _CACHE.pop(key, None)
         if hit is None:
             with torch.cuda.device(device):
                 op = Op(iso, k, w)
-                hit = Batch(op, dtype, [img] * B, weighted=weighted)
+                hit = Batch(op, dtype, [img] * B, weighted=weighted, method=method)
         _CACHE[key] = hit
         while len(_CACHE) > _CACHE_MAX:
             _CACHE.pop(next(iter(_CACHE)))
         return hit
 
 
-def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None, keep=False):
+def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None, keep=False, method=None):
     """(z, dual (k, B, *img) in the C form, gaps, iters, zerrs, converged)
     for x (B, *img) on the GPU; gamma (B,); warm: a dual (k, B, *img) or None;
     sw: pixel weights (B, *img) or edge weights (k, B, *img). keep: also the
@@ -451,7 +474,7 @@ def solve(x, gamma, w, iso, tol, max_iter, warm=None, sw=None, keep=False):
         swd = sw.detach().to(dtype=x.dtype)
         wlist = [swd[b].contiguous() for b in range(B)] if iso else \
             [swd[:, b].contiguous() for b in range(B)]
-    bt = _batch_for(iso, k, w, x.dtype, x.device.index or 0, B, img, sw is not None)
+    bt = _batch_for(iso, k, w, x.dtype, x.device.index or 0, B, img, sw is not None, method)
     g = gamma.detach().double().cpu().numpy().reshape(-1)
     with torch.cuda.device(x.device):
         bt.solve(list(xb), list(z), list(dual), g, weights=wlist, tol=tol, max_iter=max_iter,
@@ -496,7 +519,8 @@ class _DeviceTV(torch.autograd.Function):
         xb = x.detach().reshape((-1,) + img)
         wb = None if warm is None else warm.reshape((k, -1) + img)
         z, dual, gaps, its, zerrs, conv, exps, uedge = solve(xb, gamma.detach(), cfg.w, cfg.iso, cfg.tol,
-                                                             cfg.max_iter, wb, ctx.sw, keep=True)
+                                                             cfg.max_iter, wb, ctx.sw, keep=True,
+                                                             method=cfg.method)
         n = its.size
         cfg.info = IterInfo(float(gaps.sum()), int(its.max()) if n else 0, 0, bool(conv.all()),
                             dual.reshape((k,) + shape), float(zerrs.max()) if n else 0.0, gaps, its)

@@ -257,6 +257,30 @@ FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, 
 /* Launch statistics of the last solve on ws: step launches, host syncs. */
 FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs);
 
+/* The method of an anisotropic workspace's solves. FPR_METHOD_ADBPG (the
+ * default for operators without edge weights) is the reference CPU
+ * library's (src/ops/tv_aniso.cpp:259-616, 886-953): exact 1D TV steps on
+ * every fibre, the accelerated dual block proximal gradient method for two
+ * active axes and the three-block fast dual proximal gradient method for
+ * three, its gap schedule, restarts and polish, so that iterates, iteration
+ * counts and results follow the reference's fp_prox. FPR_METHOD_FISTA is dual
+ * FISTA on the per-edge dual, the reference's device path
+ * (bindings/python/fastprox/_tv_torch.py:209-282), and the method of edge-
+ * weighted operators. The environment variable FASTPROX_ROCM_ANISO=fista
+ * makes FISTA the default of new workspaces. Setting a method builds its
+ * solver's buffers (it allocates); the dual carries over. FP_EINVAL for an
+ * isotropic workspace, or ADBPG on edge weights. No counterpart in the
+ * reference's C ABI. */
+#define FPR_METHOD_ADBPG 0
+#define FPR_METHOD_FISTA 1
+FP_API fp_status fpr_workspace_set_method(fp_workspace *ws, int method);
+FP_API fp_status fpr_workspace_get_method(const fp_workspace *ws, int *method);
+/* ADBPG workspaces: per image of the last solve, whether its polish
+ * certified it (the regions read off the dual, z their closed-form means),
+ * and the number of fibres whose 1D step took the taut string after
+ * Condat's rescan budget (tv1d.cpp:264). */
+FP_API fp_status fpr_workspace_polished(const fp_workspace *ws, int *polished, int64_t count, int *fallbacks);
+
 /* A batch of images of uneven sizes, each with its own step and, for a
  * weighted batch, its own spatial weights, solved in one persistent solve
  * (the same dual FISTA per image as fp_prox). Every pointer is a device
@@ -286,7 +310,14 @@ FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp
 /* Per image, the results of the last solve (any pointer may be NULL). */
 FP_API fp_status fpr_batch_image_info(const fpr_batch *b, double *gaps, int *iters, double *zerrs,
                                       int *converged, int64_t count);
+/* Selecting a step kernel on an anisotropic batch (or workspace) selects
+ * FPR_METHOD_FISTA, whose kernels they are. */
 FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry);
+/* The method of an anisotropic batch (as fpr_workspace_set_method):
+ * FPR_METHOD_ADBPG, the default, for batches of one shape without edge
+ * weights; FPR_METHOD_FISTA otherwise. */
+FP_API fp_status fpr_batch_set_method(fpr_batch *b, int method);
+FP_API fp_status fpr_batch_get_method(const fpr_batch *b, int *method);
 FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry);
 FP_API fp_status fpr_batch_stats(const fpr_batch *b, int *launches, int *host_syncs);
 /* The last solve's pixel passes: sum over the images of the pixels times the

@@ -153,7 +153,7 @@ def test_aniso_float64_bitwise_against_the_oracle(cuda, case):
     x = image(shape, 40 + case)
     xt = torch.tensor(x, device=cuda)
     out = torch.empty_like(xt)
-    ws = Workspace(Op(False, k, w, spatial=beta), xt)
+    ws = Workspace(Op(False, k, w, spatial=beta), xt, method="fista")
     ws.prox(xt, gamma, out, tol=1e-8, max_iter=100000)
     B = int(np.prod(shape[:len(shape) - k]))
     gaps, its, _ = ws.image_info(B)
@@ -343,7 +343,7 @@ def test_3d_float64_bitwise(cuda, iso):
     w = [1.0, 0.6, 1.3]
     xt = torch.tensor(x, device=cuda)
     out = torch.empty_like(xt)
-    ws = Workspace(Op(iso, 3, w), xt)
+    ws = Workspace(Op(iso, 3, w), xt, method=None if iso else "fista")
     ws.prox(xt, 0.2, out, tol=1e-8, max_iter=100000)
     _, its, _ = ws.image_info(2)
     for b in range(2):
