@@ -75,6 +75,10 @@ constexpr int kThr = 256;
 constexpr int kFib = 64;          // fibres per block of a pass
 constexpr int kElem = kThr * 8;   // pixels per block of the per-image sums
 constexpr int kMaxSlots = 1024;   // taut-string scratch slots
+constexpr int kRing = 128;        // recent samples of each fibre in LDS (Condat's rescans read them)
+constexpr int kChunkS = 16;       // samples per load batch
+constexpr int kSweep = 8;         // entries per load batch of the updates
+constexpr int kQueue = 16;        // finished segments of a fibre waiting to be written
 
 // Per-image phases.
 enum : int {
@@ -185,11 +189,18 @@ __device__ Prep tv1d_prep(const IO &io, int n, double gamma) {
     P.lam = 0.0;
     double lo = io.in(0), hi = lo;
     bool bad = false;
-    for (int i = 0; i < n; ++i) {
-        const double v = io.in(i);
-        bad |= !isfinite(v);
-        lo = min_nn(lo, v);
-        hi = max_nn(hi, v);
+    // in chunks of independent loads (the fibre's entries are far apart in
+    // memory; one at a time, each would wait for its own)
+    for (int i0 = 0; i0 < n; i0 += kChunkS) {
+        double v[kChunkS];
+#pragma unroll
+        for (int j = 0; j < kChunkS; ++j) v[j] = io.in(i0 + j < n ? i0 + j : n - 1);  // unconditional: one wait
+#pragma unroll
+        for (int j = 0; j < kChunkS; ++j) {
+            bad |= !isfinite(v[j]);
+            lo = min_nn(lo, v[j]);
+            hi = max_nn(hi, v[j]);
+        }
     }
     if (bad) {
         P.kind = kPrepNaN;
@@ -263,112 +274,172 @@ __device__ void tv1d_unscale(const IO &io, int n, const Prep &P) {
     }
 }
 
-// Condat's algorithm with a constant radius lam; false when more than budget
-// steps were rescanned (the output is then incomplete).
+// Condat's algorithm with a constant radius lam (tv1d.cpp:72-217, the
+// generalised form: the paper's bounds for the first three samples of a
+// segment, then the running sum), step for step, with its input staged
+// through LDS and its output written in lockstep. Every lane of the wave
+// solves its own fibre. The wave loads the next kChunkS samples of all its
+// fibres at once (issued before the lanes scan the samples already staged,
+// so the loads overlap the scan) into a ring of the last kRing samples of
+// each fibre; a rescan further back than the ring reads global memory. A
+// finished segment goes into a short per-lane queue, and each lockstep
+// iteration writes one entry of every lane's queue, so the segments' stores
+// are issued together rather than one lane's at a time. act: this lane's
+// fibre is solved here (every lane of the wave runs the loop). Returns 2
+// when solved, 4 past the budget (the output is then incomplete), 3 for an
+// inactive lane.
+struct RingLds {  // 76 KiB: two blocks per CU
+    double ring[kRing][kFib];
+    double qv[kQueue][kFib];
+    int qe[kQueue][kFib];
+};
+
 template <class IO>
-__device__ bool condat(const IO &io, int n, double lam, int sk, int budget) {
-    int k = 0, k0 = 0, km = 0, kp = 0;
-    const double y0 = scaled_in(io, 0, sk);
-    double vmin = y0 - lam, vmax = y0 + lam;
-    double umin = lam, umax = -lam;
-    double ref = 0.0, s = 0.0;
-    int rescanned = 0;
-    // u_{km} = +lam: z jumps down after km; a new segment at km + 1.
+__device__ int condat_ring(const IO &io, int n, double lam, int sk, int budget, bool act, RingLds &L) {
+    const int lane = threadIdx.x;
+    int k = 0, k0 = 0, km = 0, kp = 0, rescanned = 0;
+    double vmin = 0.0, vmax = 0.0, umin = lam, umax = -lam, ref = 0.0, s = 0.0;
+    int state = act ? 0 : 3;  // 0 not started, 1 scanning, 2 solved, 3 inactive, 4 past the budget
+    int F = 0;                // samples staged so far (uniform over the wave)
+    int head = 0, tail = 0, w = 0;  // the segment queue and the next entry to write
+    double pf[kChunkS];
+#pragma unroll
+    for (int j = 0; j < kChunkS; ++j) pf[j] = scaled_in(io, j < n ? j : n - 1, sk);  // every lane: in bounds
+    auto rcp = [&](int m) -> double { return 1.0 / (double)m; };  // the reference's table of 1/m (tv1d.cpp:49-57)
+    auto Y = [&](int i) -> double {
+        return i >= F - kRing ? L.ring[i & (kRing - 1)][lane] : scaled_in(io, i, sk);
+    };
+    auto push = [&](int end, double v) {
+        L.qe[tail & (kQueue - 1)][lane] = end;
+        L.qv[tail & (kQueue - 1)][lane] = v;
+        ++tail;
+    };
+    auto drain = [&]() {
+        if (head < tail) {
+            const int h = head & (kQueue - 1);
+            io.put(w, L.qv[h][lane]);
+            if (w == L.qe[h][lane]) ++head;
+            ++w;
+        }
+    };
     auto jump_down = [&]() {
-        const double v = vmin;
-        for (int i = k0; i <= km; ++i) io.put(i, v);
+        push(km, vmin);
         rescanned += k - km;
         k = k0 = kp = km = km + 1;
-        const double yk = scaled_in(io, k < n ? k : n - 1, sk);  // k < n: no jump starts past the last sample
+        const double yk = Y(k < n ? k : n - 1);  // k < n: no jump starts past the last sample
         vmin = yk + (lam - lam);
         vmax = yk + (lam + lam);
         umin = lam;
         umax = -lam;
     };
     auto jump_up = [&]() {
-        const double v = vmax;
-        for (int i = k0; i <= kp; ++i) io.put(i, v);
+        push(kp, vmax);
         rescanned += k - kp;
         k = k0 = km = kp = kp + 1;
-        const double yk = scaled_in(io, k < n ? k : n - 1, sk);
+        const double yk = Y(k < n ? k : n - 1);
         vmin = yk - (lam + lam);
         vmax = yk - (lam - lam);
         umin = lam;
         umax = -lam;
     };
-    for (;;) {
-        while (k < n - 1) {
-            const double yn = scaled_in(io, k + 1, sk);
+    // One step of the scan (the body of the reference's loops). Both forms'
+    // tests and updates are formed and the lane's one selected, so the wave
+    // runs two paths per step (advance or jump) rather than one per form and
+    // outcome; every value selected is the reference's expression.
+    auto step = [&]() {
+        if (k < n - 1) {
+            const double yn = Y(k + 1);
             const double ln = lam;
-            const int m = k - k0 + 2;  // length after this step
-            if (m <= 3) {
-                if (yn + umin < vmin - ln) {
-                    jump_down();
-                    if (rescanned > budget) return false;
-                } else if (yn + umax > vmax + ln) {
-                    jump_up();
-                    if (rescanned > budget) return false;
-                } else {
-                    ++k;
-                    umin += yn - vmin;
-                    umax += yn - vmax;
-                    const double r = 1.0 / (double)m;
-                    const bool cm = umin >= ln, cp = umax <= -ln;
-                    const double dm = max_nn(umin - ln, 0.0), dp = min_nn(umax + ln, 0.0);
-                    vmin += dm * r;
-                    umin = min_nn(umin, ln);
-                    km = cm ? k : km;
-                    vmax += dp * r;
-                    umax = max_nn(umax, -ln);
-                    kp = cp ? k : kp;
-                    if (m == 3) {  // to the running sum
-                        ref = scaled_in(io, k0, sk);
-                        s = umin + 3.0 * (vmin - ref);
-                    }
-                }
-            } else {
-                const double s1 = s + (yn - ref);
-                const double r = 1.0 / (double)m;
-                const double a = ref + (s1 - ln) * r, b = ref + (s1 + ln) * r;
-                if (b < vmin) {
-                    jump_down();
-                    if (rescanned > budget) return false;
-                } else if (a > vmax) {
-                    jump_up();
-                    if (rescanned > budget) return false;
-                } else {
-                    ++k;
-                    s = s1;
-                    km = a >= vmin ? k : km;
-                    vmin = max_nn(vmin, a);
-                    kp = b <= vmax ? k : kp;
-                    vmax = min_nn(vmax, b);
-                }
+            const int m = k - k0 + 2;
+            const bool shrt = m <= 3;
+            const double r = rcp(m);
+            // the running-sum form's candidates
+            const double s1 = s + (yn - ref);
+            const double ca = ref + (s1 - ln) * r, cb = ref + (s1 + ln) * r;
+            const bool down = shrt ? (yn + umin < vmin - ln) : (cb < vmin);
+            const bool up = !down && (shrt ? (yn + umax > vmax + ln) : (ca > vmax));
+            if (down || up) {
+                const int e = down ? km : kp;
+                push(e, down ? vmin : vmax);
+                rescanned += k - e;
+                k = k0 = km = kp = e + 1;
+                const double yk = Y(k < n ? k : n - 1);  // k < n: no jump starts past the last sample
+                vmin = down ? yk + (lam - lam) : yk - (lam + lam);
+                vmax = down ? yk + (lam + lam) : yk - (lam - lam);
+                umin = lam;
+                umax = -lam;
+                if (rescanned > budget) state = 4;
+                return;
             }
-        }
-        // The last segment closes with u_{n-1} = 0.
-        const int m = k - k0 + 1;
-        bool down, up;
-        double v;
-        if (m <= 3) {
-            down = umin < 0.0;
-            up = umax > 0.0;
-            v = vmin + umin / (double)m;
+            ++k;
+            // the first samples' form
+            const double un = umin + (yn - vmin), ux = umax + (yn - vmax);
+            const bool cm = un >= ln, cp = ux <= -ln;
+            const double dm = max_nn(un - ln, 0.0), dp = min_nn(ux + ln, 0.0);
+            const double vn_s = vmin + dm * r, vx_s = vmax + dp * r;
+            const double un_s = min_nn(un, ln), ux_s = max_nn(ux, -ln);
+            // the running-sum form's
+            const double vn_l = max_nn(vmin, ca), vx_l = min_nn(vmax, cb);
+            km = shrt ? (cm ? k : km) : (ca >= vmin ? k : km);
+            kp = shrt ? (cp ? k : kp) : (cb <= vmax ? k : kp);
+            vmin = shrt ? vn_s : vn_l;
+            vmax = shrt ? vx_s : vx_l;
+            umin = shrt ? un_s : umin;
+            umax = shrt ? ux_s : umax;
+            s = shrt ? s : s1;
+            if (m == 3) {  // to the running sum
+                ref = Y(k0);
+                s = umin + 3.0 * (vmin - ref);
+            }
         } else {
-            v = ref + s * (1.0 / (double)m);
-            down = v < vmin;
-            up = v > vmax;
+            const int m = k - k0 + 1;
+            bool down, up;
+            double v;
+            if (m <= 3) {
+                down = umin < 0.0;
+                up = umax > 0.0;
+                v = vmin + umin / (double)m;
+            } else {
+                v = ref + s * rcp(m);
+                down = v < vmin;
+                up = v > vmax;
+            }
+            if (down) {
+                jump_down();
+            } else if (up) {
+                jump_up();
+            } else {
+                push(n - 1, v);
+                state = 2;
+                return;
+            }
+            if (rescanned > budget) state = 4;
         }
-        if (down) {
-            jump_down();
-        } else if (up) {
-            jump_up();
-        } else {
-            for (int i = k0; i < n; ++i) io.put(i, v);
-            return true;
+    };
+    for (;;) {
+#pragma unroll
+        for (int j = 0; j < kChunkS; ++j)
+            if (F + j < n) L.ring[(F + j) & (kRing - 1)][lane] = pf[j];
+        F += kChunkS;
+#pragma unroll
+        for (int j = 0; j < kChunkS; ++j) pf[j] = scaled_in(io, F + j < n ? F + j : n - 1, sk);
+        if (state == 0) {
+            const double y0 = Y(0);
+            vmin = y0 - lam;
+            vmax = y0 + lam;
+            state = 1;
         }
-        if (rescanned > budget) return false;
+        for (;;) {
+            // a step needs its next sample staged and room for one segment
+            const bool can = state == 1 && (k >= n - 1 || k + 1 < F) && tail - head < kQueue;
+            if (state == 4) head = tail;  // the taut string rewrites the fibre
+            if (!__any(can || head < tail)) break;
+            if (can) step();
+            drain();
+        }
+        if (!__any(state <= 1)) break;
     }
+    return state;
 }
 
 // The taut string with a constant radius lam; up and lo hold n + 1
@@ -534,32 +605,54 @@ __device__ double sweep(const PassArgs &A, const AdImg &S, const FibreIO<MODE> &
         double *wv = B.wv + (int64_t)img * P * N;
         const double *wr = S.wv_is_y ? y : wv;
         const double beta = S.beta;
-        for (int i = 0; i < n; ++i) {
-            const int o = io.o0 + i * io.s;
-            const double zo = io.buf[o];
-            const double wo = wr[o], yo = y[o];
-            const double v = io.u[o] - wo;
-            const double yn = zo - v;
-            const double r = (wo - yn) * (yn - yo);
-            wv[o] = yn + beta * (yn - yo);
-            y[o] = yn;
-            acc += r;
+        for (int i0 = 0; i0 < n; i0 += kSweep) {
+            double zo[kSweep], wo[kSweep], yo[kSweep], uo[kSweep];
+#pragma unroll
+            for (int j = 0; j < kSweep; ++j) {
+                const int o = io.o0 + (i0 + j < n ? i0 + j : n - 1) * io.s;
+                zo[j] = io.buf[o];
+                wo[j] = wr[o];
+                yo[j] = y[o];
+                uo[j] = io.u[o];
+            }
+#pragma unroll
+            for (int j = 0; j < kSweep; ++j) {
+                if (i0 + j >= n) break;
+                const int o = io.o0 + (i0 + j) * io.s;
+                const double v = uo[j] - wo[j];
+                const double yn = zo[j] - v;
+                const double r = (wo[j] - yn) * (yn - yo[j]);
+                wv[o] = yn + beta * (yn - yo[j]);
+                y[o] = yn;
+                acc += r;
+            }
         }
     } else if (MODE == kPass3) {
         double *y = B.y + ((int64_t)img * P + a) * N;
         double *wv = B.wv + ((int64_t)img * P + a) * N;
         const double *wr = S.wv_is_y ? y : wv;
         const double beta = S.beta;
-        for (int i = 0; i < n; ++i) {
-            const int o = io.o0 + i * io.s;
-            const double zo = io.buf[o];
-            const double wo = wr[o], yo = y[o];
-            const double v = io.u[o] - 3.0 * wo;
-            const double yn = (zo - v) / 3.0;
-            const double r = (wo - yn) * (yn - yo);
-            wv[o] = yn + beta * (yn - yo);
-            y[o] = yn;
-            acc += r;
+        for (int i0 = 0; i0 < n; i0 += kSweep) {
+            double zo[kSweep], wo[kSweep], yo[kSweep], uo[kSweep];
+#pragma unroll
+            for (int j = 0; j < kSweep; ++j) {
+                const int o = io.o0 + (i0 + j < n ? i0 + j : n - 1) * io.s;
+                zo[j] = io.buf[o];
+                wo[j] = wr[o];
+                yo[j] = y[o];
+                uo[j] = io.u[o];
+            }
+#pragma unroll
+            for (int j = 0; j < kSweep; ++j) {
+                if (i0 + j >= n) break;
+                const int o = io.o0 + (i0 + j) * io.s;
+                const double v = uo[j] - 3.0 * wo[j];
+                const double yn = (zo[j] - v) / 3.0;
+                const double r = (wo[j] - yn) * (yn - yo[j]);
+                wv[o] = yn + beta * (yn - yo[j]);
+                y[o] = yn;
+                acc += r;
+            }
         }
     } else if (MODE == kPassExact) {
         double *qa = B.q + ((int64_t)img * G.k + a) * N;
@@ -580,6 +673,7 @@ __device__ __forceinline__ int part_slot(const Geo &G, int a, int f) {
 
 template <int MODE>
 __global__ __launch_bounds__(kFib) void pass_kernel(PassArgs A) {
+    __shared__ RingLds ring;
     const int img = blockIdx.y;
     const AdImg &S = A.b.st[img];
     if (S.phase != A.phase) return;
@@ -591,21 +685,28 @@ __global__ __launch_bounds__(kFib) void pass_kernel(PassArgs A) {
     const int ax = 3 - G.k + a;
     const int n = G.L[ax], s = G.S[ax];
     const int f = blockIdx.x * kFib + threadIdx.x;
-    if (f >= fibres_of(G, ax)) return;
-    const FibreIO<MODE> io = make_io<MODE>(A, S, img, a, fibre_base(G, ax, f), s);
+    const int nf = fibres_of(G, ax);
+    if (blockIdx.x * kFib >= nf) return;  // (uniform over the block)
+    // every lane runs the staged scan's loop; lanes past the last fibre idle
+    const bool valid = f < nf;
+    const FibreIO<MODE> io = make_io<MODE>(A, S, img, a, valid ? fibre_base(G, ax, f) : 0, s);
     const double gamma = MODE == kPass3 ? 3.0 * S.lam[a] : S.lam[a];
-    const Prep P = tv1d_prep(io, n, gamma);
-    if (P.kind == kPrepSolve) {
-        if (!condat(io, n, P.lam, P.sk, 3 * n + 1024)) {
-            const int at = atomicAdd(&A.b.ctr[A.qslot], 1);
-            atomicAdd(&A.b.ctr[2], 1);
-            A.b.queue[at * 3 + 0] = img;
-            A.b.queue[at * 3 + 1] = a;
-            A.b.queue[at * 3 + 2] = f;
-            return;
-        }
-    } else {
-        tv1d_simple(io, n, P);
+    Prep P;
+    P.kind = kPrepId;
+    P.sk = 0;
+    P.lam = 0.0;
+    if (valid) P = tv1d_prep(io, n, gamma);
+    const bool act = valid && P.kind == kPrepSolve;
+    if (valid && !act) tv1d_simple(io, n, P);
+    const int st = condat_ring(io, n, P.lam, P.sk, 3 * n + 1024, act, ring);
+    if (!valid) return;
+    if (st == 4) {
+        const int at = atomicAdd(&A.b.ctr[A.qslot], 1);
+        atomicAdd(&A.b.ctr[2], 1);
+        A.b.queue[at * 3 + 0] = img;
+        A.b.queue[at * 3 + 1] = a;
+        A.b.queue[at * 3 + 2] = f;
+        return;
     }
     tv1d_unscale(io, n, P);
     const double acc = sweep<MODE>(A, S, io, img, a, n);

@@ -462,13 +462,17 @@ fp_status build_ad(fp_workspace &w) {
     return FP_OK;
 }
 
-// The default method of an anisotropic workspace: the reference's (ADBPG)
-// unless the operator has edge weights or FASTPROX_ROCM_ANISO=fista.
-int default_method(const fp_op &op) {
+// The default anisotropic method: workspaces (fp_prox, the reference CPU
+// library's call) take the reference's method (ADBPG), batches (the device
+// path's batched solve) dual FISTA; FASTPROX_ROCM_ANISO=adbpg or fista sets
+// both. Edge-weighted operators take FISTA.
+int default_method(const fp_op &op, bool batch) {
     if (op.iso || !op.spatial.empty()) return FPR_METHOD_FISTA;
-    const char *e = std::getenv("FASTPROX_ROCM_ANISO");
-    if (e && std::strcmp(e, "fista") == 0) return FPR_METHOD_FISTA;
-    return FPR_METHOD_ADBPG;
+    if (const char *e = std::getenv("FASTPROX_ROCM_ANISO")) {
+        if (std::strcmp(e, "fista") == 0) return FPR_METHOD_FISTA;
+        if (std::strcmp(e, "adbpg") == 0) return FPR_METHOD_ADBPG;
+    }
+    return batch ? FPR_METHOD_FISTA : FPR_METHOD_ADBPG;
 }
 
 // kind 0: the streaming kernel (tt iterations, segments of ry rows); 1: the
@@ -591,7 +595,7 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     if (fp_status s = check_image(*op, x.shape + (x.ndim - op->k))) return s;
     w->kc = fpr::default_kernel(op->k, (int)w->tsize, w->img, op->iso, !op->spatial.empty(), B);
     if (w->kc.stream && !fpr::stream_fits((int)w->tsize, w->N)) w->kc = generic_kernel();
-    w->method = default_method(*op);
+    w->method = default_method(*op, false);
     if (w->method == FPR_METHOD_ADBPG) {
         if (fp_status s = build_ad(*w)) return s;
     } else if (fp_status s = build_engine(*w)) {
@@ -1974,7 +1978,7 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
         b->iters.assign((size_t)count, 0);
         b->conv.assign((size_t)count, 1);
         b->exps.assign((size_t)count, 0);
-        b->method = (count > 0 && !b->weighted && b->uniform()) ? default_method(*op) : FPR_METHOD_FISTA;
+        b->method = (count > 0 && !b->weighted && b->uniform()) ? default_method(*op, true) : FPR_METHOD_FISTA;
         if (b->method == FPR_METHOD_ADBPG) {
             if (fp_status s = batch_build_ad(*b)) return s;
         } else if (fp_status s = batch_build(*b)) {

@@ -120,10 +120,10 @@ def tv_aniso(x, gamma=1.0, *, ndim=2, weights=None, edge_weights=None, tol=None,
     r"""Anisotropic TV of the last ``ndim`` axes (torch.py:335-351 in the
     reference); the dual is in the C form q_a with x - z = sum_a q_a.
     ``edge_weights``: (ndim, ...) broadcasting to (ndim, *x.shape).
-    ``method``: "adbpg", the reference CPU library's method (exact 1D steps
-    on every fibre, block-dual FISTA, polish; the default without edge
-    weights), or "fista", dual FISTA on the edge duals (the reference's
-    device path, and the method with edge weights)."""
+    ``method``: "fista" (the default), dual FISTA on the edge duals, the
+    reference's device path (_tv_torch.py:209-282); or "adbpg", the
+    reference CPU library's method (exact 1D steps on every fibre, block-dual
+    FISTA, polish: its iteration counts and results; no edge weights)."""
     if method not in (None, "adbpg", "fista"):
         raise ValueError('method must be "adbpg", "fista" or None')
     if method == "adbpg" and edge_weights is not None:

@@ -258,16 +258,17 @@ FP_API fp_status fpr_workspace_get_kernel(const fp_workspace *ws, int *generic, 
 FP_API fp_status fpr_workspace_stats(const fp_workspace *ws, int *launches, int *host_syncs);
 
 /* The method of an anisotropic workspace's solves. FPR_METHOD_ADBPG (the
- * default for operators without edge weights) is the reference CPU
- * library's (src/ops/tv_aniso.cpp:259-616, 886-953): exact 1D TV steps on
- * every fibre, the accelerated dual block proximal gradient method for two
- * active axes and the three-block fast dual proximal gradient method for
- * three, its gap schedule, restarts and polish, so that iterates, iteration
- * counts and results follow the reference's fp_prox. FPR_METHOD_FISTA is dual
- * FISTA on the per-edge dual, the reference's device path
- * (bindings/python/fastprox/_tv_torch.py:209-282), and the method of edge-
- * weighted operators. The environment variable FASTPROX_ROCM_ANISO=fista
- * makes FISTA the default of new workspaces. Setting a method builds its
+ * default of workspaces for operators without edge weights) is the
+ * reference CPU library's (src/ops/tv_aniso.cpp:259-616, 886-953): exact 1D
+ * TV steps on every fibre, the accelerated dual block proximal gradient
+ * method for two active axes and the three-block fast dual proximal gradient
+ * method for three, its gap schedule, restarts and polish, so that
+ * iterates, iteration counts and results follow the reference's fp_prox.
+ * FPR_METHOD_FISTA is dual FISTA on the per-edge dual, the reference's
+ * device path (bindings/python/fastprox/_tv_torch.py:209-282): the default of
+ * batches (fpr_batch_*, and so of fastprox_amd.torch), and the method of
+ * edge-weighted operators. FASTPROX_ROCM_ANISO=adbpg or fista sets the
+ * default of new workspaces and batches. Setting a method builds its
  * solver's buffers (it allocates); the dual carries over. FP_EINVAL for an
  * isotropic workspace, or ADBPG on edge weights. No counterpart in the
  * reference's C ABI. */
@@ -314,8 +315,8 @@ FP_API fp_status fpr_batch_image_info(const fpr_batch *b, double *gaps, int *ite
  * FPR_METHOD_FISTA, whose kernels they are. */
 FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry);
 /* The method of an anisotropic batch (as fpr_workspace_set_method):
- * FPR_METHOD_ADBPG, the default, for batches of one shape without edge
- * weights; FPR_METHOD_FISTA otherwise. */
+ * FPR_METHOD_FISTA by default; FPR_METHOD_ADBPG for batches of one shape
+ * without edge weights. */
 FP_API fp_status fpr_batch_set_method(fpr_batch *b, int method);
 FP_API fp_status fpr_batch_get_method(const fpr_batch *b, int *method);
 FP_API fp_status fpr_batch_get_kernel(const fpr_batch *b, int *generic, int *tt, int *ry);

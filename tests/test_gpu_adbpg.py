@@ -221,22 +221,25 @@ def test_scales_nan_inf_and_trivial_images(cuda):
     np.testing.assert_array_equal(z0, x)
 
 
-def test_torch_entry_point_defaults_to_the_reference_method(cuda):
-    """fastprox_amd.torch.tv_aniso: the default is the reference's method
-    (the same bits as a workspace's ADBPG solve); method="fista" is dual
-    FISTA (the same bits as a workspace's FISTA solve)."""
+def test_torch_entry_point_methods(cuda):
+    """fastprox_amd.torch.tv_aniso: the default is dual FISTA, the
+    reference's device path (the same bits as a workspace's FISTA solve);
+    method="adbpg" is the reference's CPU method (the same bits as a
+    workspace's ADBPG solve, the workspaces' default)."""
     from fastprox_amd.torch import tv_aniso
     x = rectangles(3, (64, 80), n=6, sigma=0.1, seed=4, dtype=np.float64)
     xt = torch.tensor(x, device="cuda")
-    z_a, info_a = tv_aniso(xt, 0.1, weights=[1.0, 0.5], tol=1e-7, max_iter=100000, return_info=True)
-    z_f, info_f = tv_aniso(xt, 0.1, weights=[1.0, 0.5], tol=1e-7, max_iter=100000, return_info=True,
-                           method="fista")
+    z_f, info_f = tv_aniso(xt, 0.1, weights=[1.0, 0.5], tol=1e-7, max_iter=100000, return_info=True)
+    z_a, info_a = tv_aniso(xt, 0.1, weights=[1.0, 0.5], tol=1e-7, max_iter=100000, return_info=True,
+                           method="adbpg")
     _, zw_a, _, _, its_a, _, _ = _ours(x, 0.1, [1.0, 0.5], 1e-7)
     _, zw_f, _, _, its_f, _, _ = _ours(x, 0.1, [1.0, 0.5], 1e-7, method="fista")
     np.testing.assert_array_equal(z_a.cpu().numpy(), zw_a)
     np.testing.assert_array_equal(z_f.cpu().numpy(), zw_f)
     assert info_a.iters == int(its_a.max()) and info_f.iters == int(its_f.max())
     assert info_a.iters < info_f.iters
+    ws = Workspace(Op(False, 2, [1.0, 0.5]), xt)
+    assert ws.method() == "adbpg"
 
 
 def test_derivatives_at_the_polished_solution(cuda):
@@ -245,7 +248,7 @@ def test_derivatives_at_the_polished_solution(cuda):
     from fastprox_amd.torch import tv_aniso
     x = rectangles(2, (40, 48), n=5, sigma=0.05, seed=8, dtype=np.float64)
     xt = torch.tensor(x, device="cuda", requires_grad=True)
-    z = tv_aniso(xt, 0.08, weights=[1.0, 0.7], tol=1e-10, max_iter=100000)
+    z = tv_aniso(xt, 0.08, weights=[1.0, 0.7], tol=1e-10, max_iter=100000, method="adbpg")
     g = torch.tensor(np.random.default_rng(1).standard_normal(x.shape), device="cuda")
     (z * g).sum().backward()
     gx = xt.grad.cpu().numpy()

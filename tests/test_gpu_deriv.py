@@ -236,14 +236,15 @@ def test_kept_iterate_is_the_output_dual(cuda, kernel):
         assert np.abs(_q_to_u(ds[b].cpu().numpy(), w) - ue[b].cpu().numpy()).max() <= 1e-12
 
 
-def test_aniso_workspace_vjp_matches_structure(cuda):
+@pytest.mark.parametrize("method", ["fista", "adbpg"])
+def test_aniso_workspace_vjp_matches_structure(cuda, method):
     """fp_vjp on an anisotropic keep_structure workspace reads the same
-    per-edge dual as the autograd path."""
+    per-edge dual as the autograd path, for either method."""
     x = image((2, 30, 34), 41, scale=0.2)
     w = [1.0, 0.6]
     xt = torch.tensor(x, device="cuda")
     op = Op(False, 2, w)
-    ws = Workspace(op, xt, keep_structure=True)
+    ws = Workspace(op, xt, keep_structure=True, method=method)
     z = torch.empty_like(xt)
     ws.prox(xt, 0.3, z, tol=1e-10, max_iter=10 ** 6)
     g = torch.tensor(np.random.default_rng(1).normal(size=x.shape), device="cuda")
@@ -251,7 +252,7 @@ def test_aniso_workspace_vjp_matches_structure(cuda):
     gp = np.zeros(2)
     gg = ws.vjp(g, gx, gparams=gp)
     gam = torch.tensor([0.3, 0.3], dtype=torch.float64, device="cuda")
-    zz, dual, gaps, its, zerrs, conv, exps, ue = tsolve(xt, gam, w, False, 1e-10, 10 ** 6, keep=True)
+    zz, dual, gaps, its, zerrs, conv, exps, ue = tsolve(xt, gam, w, False, 1e-10, 10 ** 6, keep=True, method=method)
     np.testing.assert_array_equal(zz.cpu().numpy(), z.cpu().numpy())
     S = Structure(xt, zz, ue, gaps, exps, gam, w, False)
     gx2, gg2, gw2, _, _ = S.vjp(g, 0.0, 100)

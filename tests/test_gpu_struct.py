@@ -1,11 +1,12 @@
 """The structure queries and kinks on the device (fp_op_struct_kind,
 fp_workspace_units/states/regions/constraint/kinks) against the reference
 library built from its sources (oracle/_ref) on the same problems: float64
-solves have the reference's bits, so the regions, labels and kinks must be
-the reference's exactly. Anisotropic structures (the device method's) are
-checked against the oracle's region count and the regions' defining
-property, and their kinks against the reference on one-axis images, where
-both solve the exact 1D problem."""
+isotropic solves have the reference's bits, and anisotropic solves by the
+reference's method (ADBPG) its z to rounding, so the regions, labels and
+kinks must be the reference's exactly. Anisotropic structures of dual FISTA
+solves are checked against the oracle's region count and the regions'
+defining property, and their kinks against the reference on one-axis
+images, where both solve the exact 1D problem."""
 
 from __future__ import annotations
 
@@ -41,9 +42,9 @@ def _ref_struct(ws, shape, k, tol):
     return lab, sg, nr, m, cnt.value
 
 
-def _ours(iso, k, w, x, gamma, tol, spatial=None, dtype=torch.float64):
+def _ours(iso, k, w, x, gamma, tol, spatial=None, dtype=torch.float64, method=None):
     xt = torch.tensor(x, dtype=dtype, device="cuda")
-    ws = Workspace(Op(iso, k, w, spatial=spatial), xt, keep_structure=True)
+    ws = Workspace(Op(iso, k, w, spatial=spatial), xt, keep_structure=True, method=method)
     out = torch.empty_like(xt)
     ws.prox(xt, gamma, out, tol=tol, max_iter=10 ** 6)
     return ws, xt, out
@@ -100,20 +101,21 @@ def test_iso_regions_and_kinks_match_the_reference(cuda, case):
         refbind.Op.free_ws(rws)
 
 
-def test_aniso_regions_are_the_oracle_structure(cuda):
-    """The device method's anisotropic regions: as many as the oracle's
-    structure at the same point, numbered in the order of their first
-    entries, z constant on each (to the solve's accuracy)."""
+@pytest.mark.parametrize("method", ["fista", "adbpg"])
+def test_aniso_regions_are_the_oracle_structure(cuda, method):
+    """The anisotropic regions at a solution of either method: as many as
+    the oracle's structure at the same point, numbered in the order of their
+    first entries, z constant on each (to the solve's accuracy)."""
     x = rectangles(2, (36, 40), n=8, sigma=0.05, seed=12, dtype=np.float64)
     w = [1.0, 0.6]
-    ws, xt, z = _ours(False, 2, w, x, 0.1, 1e-12)
+    ws, xt, z = _ours(False, 2, w, x, 0.1, 1e-12, method=method)
     assert Op(False, 2, w).struct_kind() == A.FP_STRUCT_PROJECTOR
     lab = torch.empty(x.shape, dtype=torch.int32, device="cuda")
     sg = torch.empty(x.shape, dtype=torch.int8, device="cuda")
     nr = ws.regions(lab, sg)
     from fastprox_amd.torch import solve as tsolve
     gam = torch.tensor([0.1, 0.1], dtype=torch.float64, device="cuda")
-    zz, dual, gaps, its, zerrs, conv, exps, ue = tsolve(xt, gam, w, False, 1e-12, 10 ** 6, keep=True)
+    zz, dual, gaps, its, zerrs, conv, exps, ue = tsolve(xt, gam, w, False, 1e-12, 10 ** 6, keep=True, method=method)
     np.testing.assert_array_equal(zz.cpu().numpy(), z.cpu().numpy())
     zc, uc, lb = z.cpu().numpy(), ue.cpu().numpy(), lab.cpu().numpy()
     assert (sg.cpu().numpy() == 1).all()
@@ -127,6 +129,47 @@ def test_aniso_regions_are_the_oracle_structure(cuda):
         spread = np.zeros(nr[b])
         np.maximum.at(spread, l, zb - zb[first[l]])
         assert np.abs(spread).max() <= 1e-8 * (1 + np.abs(zb).max())
+
+
+@pytest.mark.parametrize("case", ["2d", "2d_b", "3d"])
+def test_aniso_adbpg_structure_matches_the_reference(cuda, case):
+    """At a solution of the reference's method (FPR_METHOD_ADBPG, the
+    workspaces' default: z within rounding of the reference's), the
+    anisotropic regions, labels, signs and kinks are the reference library's
+    exactly, and the VJP in x, gamma and the axis weights agrees to
+    rounding."""
+    if not refbind.available():
+        pytest.skip("the reference library is not built")
+    shape, w, gamma, seed = {"2d": ((3, 40, 46), [1.0, 0.7], 0.08, 8), "2d_b": ((2, 36, 40), [1.0, 0.6], 0.1, 9),
+                             "3d": ((2, 10, 12, 14), [1.0, 0.8, 1.2], 0.08, 10)}[case]
+    k = len(w)
+    x = rectangles(shape[0], shape[1:], n=8, sigma=0.05, seed=seed, dtype=np.float64)
+    ws, xt, z = _ours(False, k, w, x, gamma, 1e-9, method="adbpg")
+    rop = refbind.Op(False, k, w)
+    zr, dual, gaps, its, zerrs, st, rws = rop.solve(x, gamma, tol=1e-9, max_iter=10 ** 6, keep=True)
+    try:
+        assert np.abs(z.cpu().numpy() - zr).max() <= 1e-12
+        for ktol in (1e-7, 1e-3):
+            lab_r, sg_r, nr_r, m_r, c_r = _ref_struct(rws, shape, k, ktol)
+            lab = torch.empty(shape, dtype=torch.int64, device="cuda")
+            sg = torch.empty(shape, dtype=torch.int8, device="cuda")
+            nr = ws.regions(lab, sg)
+            np.testing.assert_array_equal(nr, nr_r)
+            np.testing.assert_array_equal(lab.cpu().numpy(), lab_r)
+            np.testing.assert_array_equal(sg.cpu().numpy(), sg_r)
+            m = torch.empty(shape, dtype=torch.int8, device="cuda")
+            assert ws.kinks(ktol, m) == c_r
+            np.testing.assert_array_equal(m.cpu().numpy(), m_r)
+        g = np.random.default_rng(1).standard_normal(shape)
+        gx_r, gg_r, gp_r = rop.vjp(rws, g)
+        gx = torch.empty_like(xt)
+        gp = np.zeros(k)
+        gg = ws.vjp(torch.tensor(g, device="cuda"), gx, gparams=gp)
+        assert np.abs(gx.cpu().numpy() - gx_r).max() <= 1e-12
+        assert gg == pytest.approx(gg_r, rel=1e-12, abs=1e-12)
+        np.testing.assert_allclose(gp, gp_r[:k], rtol=1e-12, atol=1e-12)
+    finally:
+        refbind.Op.free_ws(rws)
 
 
 def test_aniso_kinks_match_the_reference_on_one_axis(cuda):
