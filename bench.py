@@ -529,7 +529,63 @@ def leg_uneven(args, n=256, seed=0):
            "kernel": kernel_name(bt), "step_kernel_ms": a, "step_kernel_busy_ms": bt.busy_ms(), "epilogue_ms": b_,
            "launches": c}
     log(f"uneven batch: {out}")
+    # The same batch as one captured hipGraph: on a capturing stream the solve
+    # enqueues the static schedule that max_iter allows (every step/epilogue
+    # pair, empty once an image has finished), so max_iter is set to twice
+    # the iterations the batch needed; the replay's images equal an eager
+    # solve's at that max_iter.
+    try:
+        out["graph"] = _uneven_graph(bt, xs, zs, ds, gam, ws, 2 * int(its.max()), max(1, min(args.steps, 3)), n)
+        log(f"uneven batch, one captured graph: {out['graph']}")
+    except Exception as e:  # reported, not fatal
+        out["graph"] = {"error": repr(e)}
     return out
+
+
+def _uneven_graph(bt, xs, zs, ds, gam, ws, max_iter, steps, n):
+    import ctypes as C
+
+    import torch
+    hip = C.CDLL("libamdhip64.so")
+    P = C.c_void_p
+    for name, argt in (("hipStreamBeginCapture", [P, C.c_int]), ("hipStreamEndCapture", [P, C.POINTER(P)]),
+                       ("hipGraphInstantiate", [C.POINTER(P), P, P, P, C.c_size_t]), ("hipGraphLaunch", [P, P]),
+                       ("hipStreamSynchronize", [P]), ("hipGraphExecDestroy", [P]), ("hipGraphDestroy", [P])):
+        getattr(hip, name).restype = C.c_int
+        getattr(hip, name).argtypes = argt
+    bt.set_timing(False)
+    st = torch.cuda.Stream()
+    s = C.c_void_p(st.cuda_stream)
+    ze = [torch.empty_like(z) for z in zs]
+    de = [torch.empty_like(d) for d in ds]
+    bt.solve(xs, ze, de, gam, weights=ws, tol=1e-6, max_iter=max_iter, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    g, ex = C.c_void_p(), C.c_void_p()
+    t0 = time.perf_counter()
+    assert hip.hipStreamBeginCapture(s, 2) == 0
+    try:
+        bt.solve(xs, zs, ds, gam, weights=ws, tol=1e-6, max_iter=max_iter, stream=st.cuda_stream)
+    finally:
+        assert hip.hipStreamEndCapture(s, C.byref(g)) == 0
+    assert hip.hipGraphInstantiate(C.byref(ex), g, None, None, 0) == 0
+    capture_s = time.perf_counter() - t0
+    try:
+        assert hip.hipGraphLaunch(ex, s) == 0 and hip.hipStreamSynchronize(s) == 0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            assert hip.hipGraphLaunch(ex, s) == 0
+        assert hip.hipStreamSynchronize(s) == 0
+        sec = (time.perf_counter() - t0) / steps
+        same = all(torch.equal(a, b) for a, b in zip(zs, ze))
+        gaps, its, zerrs, conv = bt.image_info()
+        launches, _ = bt.stats()
+    finally:
+        hip.hipGraphExecDestroy(ex)
+        hip.hipGraphDestroy(g)
+    return {"images_per_s": n / sec, "ms_per_batch": sec * 1e3, "max_iter": max_iter,
+            "kernels_in_graph": 2 * launches, "capture_and_instantiate_s": capture_s,
+            "equal_to_eager": bool(same), "converged": bool(conv.all())}
 
 
 def leg_f64(args, B=64, S=512):
@@ -578,24 +634,28 @@ def leg_aniso(args, B=64, S=512):
     from fastprox_amd.synth import digest, disks
     from fastprox_amd.torch import Batch
     x = disks(B, (S, S), seed=0, dtype=np.float64)
-    out = {"workload": f"anisotropic TV, {B} x {S}x{S} disks, w (1, 0.5), gamma 0.05, tol 1e-6 (device method)",
+    out = {"workload": f"anisotropic TV, {B} x {S}x{S} disks, w (1, 0.5), gamma 0.05, tol 1e-6; fista: dual FISTA "
+                       f"on the edge duals (the reference's device method), adbpg: the reference CPU library's "
+                       f"method (its iteration counts)",
            "input_digest": digest(x)}
-    for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
-        xs = torch.tensor(x, dtype=dt, device="cuda")
-        zs = torch.empty_like(xs)
-        ds = torch.empty((B, 2, S, S), dtype=dt, device="cuda")
-        bt = Batch(Op(False, 2, [1.0, 0.5]), dt, [(S, S)] * B)
-        xl, zl, dl = list(xs), list(zs), list(ds)
+    for method in ("fista", "adbpg"):
+        for dt, name in ((torch.float32, "f32"), (torch.float64, "f64")):
+            xs = torch.tensor(x, dtype=dt, device="cuda")
+            zs = torch.empty_like(xs)
+            ds = torch.empty((B, 2, S, S), dtype=dt, device="cuda")
+            bt = Batch(Op(False, 2, [1.0, 0.5]), dt, [(S, S)] * B, method=method)
+            xl, zl, dl = list(xs), list(zs), list(ds)
 
-        def run():
-            return bt.solve(xl, zl, dl, [0.05] * B, tol=1e-6, max_iter=args.max_iter)
-        run()
-        sec, st = _sync_time(run, 1)
-        gaps, its, zerrs, conv = bt.image_info()
-        out[name] = {"images_per_s": B / sec, "ms_per_batch": sec * 1e3, "iters_mean": float(its.mean()),
-                     "iters_max": int(its.max()), "converged": bool(conv.all())}
-        log(f"aniso {name}: {out[name]}")
-        del xs, zs, ds, bt
+            def run():
+                return bt.solve(xl, zl, dl, [0.05] * B, tol=1e-6, max_iter=args.max_iter)
+            run()
+            sec, st = _sync_time(run, 2)
+            gaps, its, zerrs, conv = bt.image_info()
+            key = name if method == "fista" else f"adbpg_{name}"
+            out[key] = {"images_per_s": B / sec, "ms_per_batch": sec * 1e3, "iters_mean": float(its.mean()),
+                        "iters_max": int(its.max()), "converged": bool(conv.all())}
+            log(f"aniso {key}: {out[key]}")
+            del xs, zs, ds, bt
     return out
 
 

@@ -175,9 +175,11 @@ class Workspace:
         _check(lib().fpr_workspace_stats(self.h, C.byref(a), C.byref(b)), "stats")
         return a.value, b.value
 
-    def prox(self, x, gamma, out, tol=1e-6, max_iter=1000, keep_structure=None):
+    def prox(self, x, gamma, out, tol=1e-6, max_iter=1000, keep_structure=None, with_info=True):
         """fp_prox (one gamma) or fp_prox_steps (an array of steps, the batch
-        shape). Returns the status (FP_OK or FP_ENOTCONV); raises on errors."""
+        shape). Returns the status (FP_OK or FP_ENOTCONV); raises on errors.
+        with_info=False passes no fp_info (the form a stream under hipGraph
+        capture takes: the results are read later by image_info/info)."""
         L = lib()
         opt = A.fp_options()
         L.fp_options_init(C.byref(opt))
@@ -186,7 +188,8 @@ class Workspace:
         xa, oa = as_array(x), as_array(out)
         if isinstance(gamma, (int, float, np.floating, np.integer)):
             info = A.fp_info()
-            s = L.fp_prox(self.op.h, float(gamma), xa.ref, oa.ref, C.byref(opt), self.h, C.byref(info))
+            s = L.fp_prox(self.op.h, float(gamma), xa.ref, oa.ref, C.byref(opt), self.h,
+                          C.byref(info) if with_info else None)
         else:
             ga = as_array(gamma)
             s = L.fp_prox_steps(self.op.h, ga.ref, xa.ref, oa.ref, C.byref(opt), self.h)

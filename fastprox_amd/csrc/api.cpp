@@ -711,6 +711,24 @@ fp_status record_kept(const fp_op *op, fp_workspace *ws, hipStream_t s) {
     return FP_OK;
 }
 
+// Whether stream s is being captured into a hipGraph.
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive;
+}
+
+// The results of a solve enqueued under capture, once it has run: waits for
+// the workspace's stream and records them.
+fp_status settle(fp_workspace *ws) {
+    if (!ws->pending) return FP_OK;
+    DevGuard g(ws->device);
+    if (hipStreamSynchronize((hipStream_t)ws->stream)) return FP_EDEVICE;
+    ws->eng->collect();
+    ws->pending = false;
+    fp_status st = FP_OK;
+    return record_results(ws->op, ws, nullptr, &st);
+}
+
 // One solve of ws's images from input d into z (device, contiguous), by the
 // workspace's method; ws->ptrs hold the engine's per-image pointers. keep:
 // the ADBPG solve also writes the per-edge dual the derivatives read.
@@ -744,6 +762,8 @@ fp_status run_solve(fp_workspace *ws, hipStream_t s, const void *d, void *z, boo
 fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *oa, const fp_options *opt,
                     fp_workspace *ws, fp_info *info, const fp_array *sa) {
     if (!op || !xa || !oa) return FP_EINVAL;
+    if (sa && ws && capturing((hipStream_t)ws->stream))
+        return fail(FP_EINVAL, "fp_prox_steps reads its steps on the host: not under capture");
     if (!sa && (!std::isfinite(step) || step < 0)) return FP_EPARAM;
     const fp_options o = opt ? *opt : defaults();
     if (o.tie != 0 && o.tie != 1) return FP_EINVAL;
@@ -831,12 +851,21 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     // Inputs: the caller's device array itself when contiguous and not
     // written by this call, else staged.
     const bool x_direct = x.device == FP_DEVICE_ROCM && x.contiguous() && x.data != out.data;
+    const bool z_direct = out.device == FP_DEVICE_ROCM && out.contiguous();
+    // On a capturing stream the solve is the static schedule (no host read
+    // or wait): device arrays only, no host results, no kept point.
+    ws->pending = false;
+    const bool cap = capturing(s);
+    if (cap) {
+        if (ws->ad) return fail(FP_EINVAL, "ADBPG solves read the host between steps: not under capture");
+        if (!x_direct || !z_direct) return fail(FP_EINVAL, "under capture x and out are contiguous device arrays");
+        if (keep || info || sa) return fail(FP_EINVAL, "under capture: no host results, kept points or steps");
+    }
     const void *d = x.data;
     if (!x_direct) {
         if (fp_status st2 = stage_in(*ws, x, ws->dstage, s)) return st2;
         d = ws->dstage;
     }
-    const bool z_direct = out.device == FP_DEVICE_ROCM && out.contiguous();
     void *z = z_direct ? (void *)out.data : ws->zstage;
     hipError_t err;
     const void *sw = op->device_weights(ws->device, (int)ws->tsize, err);
@@ -853,6 +882,11 @@ fp_status prox_impl(const fp_op *op, double step, const fp_array *xa, fp_array *
     if (fp_status r = run_solve(ws, s, d, z, warm, o.tol, o.max_iter, keep)) {
         ws->dual_valid = false;
         return r;
+    }
+    if (cap) {  // the results are read at the first query after the graph has run
+        ws->pending = true;
+        ws->dual_valid = true;
+        return FP_OK;
     }
     if (!z_direct)
         if (fp_status st2 = stage_out(*ws, z, out, s)) return st2;
@@ -1341,6 +1375,9 @@ struct fpr_batch {
     std::vector<double> gammas, gaps, zerrs;
     std::vector<int> iters, conv, exps;
     bool valid = false;
+    // a solve enqueued on a capturing stream: results read at the first query
+    bool pending = false;
+    hipStream_t pending_stream = nullptr;
     bool uniform() const {
         for (size_t i = 1; i < specs.size(); ++i)
             for (int a = 0; a < 3; ++a)
@@ -1382,6 +1419,44 @@ fp_status batch_build_ad(fpr_batch &b) {
     if (int r = fpr::ad_reserve(sp, &a)) return (fp_status)r;
     b.ad.reset(a);
     return FP_OK;
+}
+
+// The engine's per-image results of the last solve into b.
+fp_status batch_results(fpr_batch &b) {
+    const fp_op &op = *b.op;
+    const fpr::ImgState *S = b.eng->state();
+    bool bad = false, notconv = false;
+    for (size_t i = 0; i < b.specs.size(); ++i) {
+        const fpr::ImgState &t = S[i];
+        if (t.status) bad = true;
+        double g = t.gp;
+        if (op.iso && g == g) g = g < 0.0 ? 0.0 : g;
+        if (op.iso && t.e) g = std::ldexp(g, 2 * t.e);
+        b.gaps[i] = g;
+        b.iters[i] = t.it;
+        b.zerrs[i] = t.zerr;
+        b.conv[i] = t.converged;
+        b.exps[i] = t.e;
+        if (!t.converged) notconv = true;
+    }
+    if (bad) {
+        b.valid = false;
+        return FP_EINVAL;
+    }
+    b.valid = true;
+    return notconv ? FP_ENOTCONV : FP_OK;
+}
+
+// A captured solve's results, once it has run (waits for its stream).
+fp_status settle_batch(const fpr_batch *cb) {
+    fpr_batch *b = const_cast<fpr_batch *>(cb);
+    if (!b->pending) return FP_OK;
+    DevGuard g(b->device);
+    if (hipStreamSynchronize(b->pending_stream)) return FP_EDEVICE;
+    b->eng->collect();
+    b->pending = false;
+    const fp_status st = batch_results(*b);
+    return st == FP_ENOTCONV ? FP_OK : st;
 }
 
 fp_status batch_build(fpr_batch &b) {
@@ -1565,7 +1640,9 @@ FP_API fp_status fp_workspace_get_stream(const fp_workspace *ws, void **stream) 
 }
 
 FP_API fp_status fp_workspace_info(const fp_workspace *ws, fp_info *info) {
-    if (!ws || !info || !ws->info_valid) return FP_EINVAL;
+    if (!ws || !info) return FP_EINVAL;
+    if (fp_status st = settle(const_cast<fp_workspace *>(ws))) return st;
+    if (!ws->info_valid) return FP_EINVAL;
     *info = ws->info;
     return ws->info_status;
 }
@@ -1815,7 +1892,9 @@ FP_API fp_status fp_workspace_deriv_exact(const fp_workspace *ws, int *exact, do
 
 FP_API fp_status fp_workspace_image_info(const fp_workspace *ws, double *gaps, int *iters, double *zerrs,
                                          int64_t count) {
-    if (!ws || !ws->info_valid) return FP_EINVAL;
+    if (!ws) return FP_EINVAL;
+    if (fp_status st = settle(const_cast<fp_workspace *>(ws))) return st;
+    if (!ws->info_valid) return FP_EINVAL;
     int64_t images = 1;
     for (int d = 0; d < ws->ndim - ws->k; ++d) images *= ws->shape[d];
     if (count != images) return FP_ESHAPE;
@@ -2054,34 +2133,31 @@ FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp
             b->valid = true;
             return notconv ? FP_ENOTCONV : FP_OK;
         }
+        b->pending = false;
+        const bool cap = capturing((hipStream_t)stream);
+        if (cap && b->ad) {
+            if (cur != b->device) (void)hipSetDevice(cur);
+            return fail(FP_EINVAL, "ADBPG solves read the host between steps: not under capture");
+        }
         const int r = b->eng->solve((hipStream_t)stream, b->ptrs.data(), b->gammas.data(), warm != 0, o.tol,
                                     o.max_iter);
         if (cur != b->device) (void)hipSetDevice(cur);
         if (r) return hip_status(r);
-        const fpr::ImgState *S = b->eng->state();
-        bool bad = false, notconv = false;
-        for (int64_t i = 0; i < B; ++i) {
-            const fpr::ImgState &t = S[i];
-            if (t.status) bad = true;
-            double g = t.gp;
-            if (op.iso && g == g) g = g < 0.0 ? 0.0 : g;
-            if (op.iso && t.e) g = std::ldexp(g, 2 * t.e);
-            b->gaps[(size_t)i] = g;
-            b->iters[(size_t)i] = t.it;
-            b->zerrs[(size_t)i] = t.zerr;
-            b->conv[(size_t)i] = t.converged;
-            b->exps[(size_t)i] = t.e;
-            if (!t.converged) notconv = true;
+        if (cap) {  // the results are read at the first query after the graph has run
+            b->pending = true;
+            b->pending_stream = (hipStream_t)stream;
+            b->valid = true;
+            return FP_OK;
         }
-        if (bad) return FP_EINVAL;
-        b->valid = true;
-        return notconv ? FP_ENOTCONV : FP_OK;
+        return batch_results(*b);
     });
 }
 
 FP_API fp_status fpr_batch_image_info(const fpr_batch *b, double *gaps, int *iters, double *zerrs, int *converged,
                                       int64_t count) {
-    if (!b || !b->valid) return FP_EINVAL;
+    if (!b) return FP_EINVAL;
+    if (fp_status st = settle_batch(b)) return st;
+    if (!b->valid) return FP_EINVAL;
     if (count != (int64_t)b->specs.size()) return FP_ESHAPE;
     for (int64_t i = 0; i < count; ++i) {
         if (gaps) gaps[i] = b->gaps[(size_t)i];
@@ -2178,7 +2254,9 @@ FP_API fp_status fpr_workspace_set_tail(fp_workspace *ws, int tt, int ry, double
 }
 
 FP_API fp_status fpr_batch_scale_exps(const fpr_batch *b, int *exps, int64_t count) {
-    if (!b || !exps || !b->valid) return FP_EINVAL;
+    if (!b || !exps) return FP_EINVAL;
+    if (fp_status st = settle_batch(b)) return st;
+    if (!b->valid) return FP_EINVAL;
     if (count != (int64_t)b->specs.size()) return FP_ESHAPE;
     for (int64_t i = 0; i < count; ++i) exps[i] = b->exps[(size_t)i];
     return FP_OK;

@@ -292,6 +292,23 @@ static int run_solve(Engine &E, const Problem &P, const KernelChoice &kc, hipStr
     (void)0;
     hipError_t err = hipGetLastError();
     if (err) return err;
+    if (E.last_static()) {
+        // The whole schedule, with no host read: every pair after an image
+        // has finished returns at once for it (an empty list).
+        const int64_t chunks = (cap + steps_per_sync - 1) / steps_per_sync;
+        for (int64_t c = 0; c < chunks; ++c)
+            for (int i = 0; i < steps_per_sync; ++i) {
+                A.list_in = list[p];
+                A.count_in = counts + p;
+                A.count_out = counts + (p ^ 1);
+                launch_step<T>(cur_kc, P.iso != 0, P.sw != 0, A, grid, s, nullptr, nullptr);
+                launch_epilogue(cur_desc, state, part, list[p ^ 1], counts + (p ^ 1), TT, u_round, eta, B, s,
+                                nullptr);
+                p ^= 1;
+            }
+        E.stats.launches += (int)(chunks * steps_per_sync);
+        return hipGetLastError();
+    }
     // Optional timing: an event before and after every step and epilogue,
     // read back after each chunk of launches.
     const bool timing = E.timing;
@@ -457,6 +474,11 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
                   int max_iter) {
     spans_.clear();
     ref_ = nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipError_t e = hipStreamIsCapturing(s, &cs)) return e;
+    static_ = cs == hipStreamCaptureStatusActive;
+    for (Engine *h = half_.get(); h; h = h->half_.get()) h->static_ = static_;
+    if (static_ && timing) return hipErrorInvalidValue;  // timing reads events back: not under capture
     if (timing) {
         hipError_t e = ref_ev_ ? hipSuccess : hipEventCreate(&ref_ev_);
         if (!e) e = hipEventRecord(ref_ev_, s);
@@ -468,9 +490,32 @@ int Engine::solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bo
     return r;
 }
 
+void Engine::collect() {
+    if (!half_) return;
+    half_->collect();
+    std::memcpy(hstate_ + own(), half_->state(), sizeof(ImgState) * (size_t)half_->nimg());
+}
+
 int Engine::solve_parts(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol,
                         int max_iter) {
     if (!half_) return solve_own(s, ptrs, gammas, warm, tol, max_iter);
+    if (static_) {
+        // Under capture both parts are enqueued from this thread: the second
+        // part's stream joins the capture through an event and rejoins s at
+        // the end, so the graph runs the parts side by side as the eager
+        // solve does.
+        const int B0 = own();
+        half_->steps_per_sync = steps_per_sync;
+        hipError_t e = hipEventRecord(half_ev_[0], s);
+        if (!e) e = hipStreamWaitEvent(half_stream_, half_ev_[0], 0);
+        if (e) return e;
+        const int r0 = solve_own(s, ptrs, gammas, warm, tol, max_iter);
+        const int r1 = half_->solve_parts(half_stream_, ptrs + B0, gammas + B0, warm, tol, max_iter);
+        e = hipEventRecord(half_ev_[1], half_stream_);
+        if (!e) e = hipStreamWaitEvent(s, half_ev_[1], 0);
+        if (r0 || r1) return r0 ? r0 : r1;
+        return e;
+    }
     const int B0 = own();
     half_->spans_.clear();
     half_->ref_ = ref_;
@@ -553,7 +598,7 @@ int Engine::solve_own(hipStream_t s, const ImagePtrs *ptrs, const double *gammas
                               desc_tail_, total_tiles_tail_, tail_frac_);
     if (r) return r;
     if ((e = hipMemcpyAsync(hstate_, state_, sizeof(ImgState) * (size_t)B, hipMemcpyDeviceToHost, s))) return e;
-    return hipStreamSynchronize(s);
+    return static_ ? hipSuccess : hipStreamSynchronize(s);
 }
 
 int Engine::copy_dual_iterate(hipStream_t s, void *dst, int64_t img_stride) const {

@@ -56,8 +56,16 @@ public:
     int setup(int tsize, const Problem &P, const KernelChoice &kc, const std::vector<ImageSpec> &imgs);
 
     // Solves every image, ptrs[i] its data and gammas[i] its step (host
-    // arrays of nimg()). Blocks until done; results in state().
+    // arrays of nimg()). Blocks until done; results in state(). On a stream
+    // that is being captured into a hipGraph, it instead enqueues the whole
+    // static schedule (every step/epilogue pair that max_iter allows, each
+    // returning at once for the images already finished: the same iterates),
+    // with no host synchronisation and no host read: the results are in
+    // state() once the caller has synchronised and called collect().
     int solve(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
+    // After a static (captured) solve has run: gathers the parts' states.
+    void collect();
+    bool last_static() const { return static_; }
 
     // The tail kernel: once the live tiles of a solve drop below frac of the
     // batch's tiles, the solve continues with kernel kc on its own tiling
@@ -123,6 +131,7 @@ public:
     bool allow_split = true;
 
 private:
+    bool static_ = false;  // this solve enqueues the static schedule (its stream is capturing)
     int own() const { return (int)specs_.size(); }  // the images of this engine itself
     int solve_own(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);
     int solve_parts(hipStream_t s, const ImagePtrs *ptrs, const double *gammas, bool warm, double tol, int max_iter);

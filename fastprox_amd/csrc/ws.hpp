@@ -95,6 +95,9 @@ struct fp_workspace {
     std::vector<fpr::ImagePtrs> ptrs;
     // results of the last call
     bool dual_valid = false, warm = false, seeded = false;
+    // A solve enqueued on a capturing stream: its per-image results are read
+    // (after a wait for the stream) at the first query of them.
+    bool pending = false;
     bool info_valid = false;
     fp_status info_status = FP_OK;
     fp_info info{0.0, 0, 0};

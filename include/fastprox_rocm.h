@@ -122,7 +122,16 @@ FP_API fp_status fp_workspace_set_warm(fp_workspace *ws, int on);
 FP_API fp_status fp_workspace_set_deriv_tol(fp_workspace *ws, double tol, int max_iter);
 FP_API fp_status fp_workspace_deriv_info(const fp_workspace *ws, double *residual, int *iters);
 
-/* include/fastprox.h:599-601 */
+/* include/fastprox.h:599-601. On a workspace whose stream is being
+ * captured into a hipGraph (fp_workspace_set_stream, hipStreamIsCapturing),
+ * fp_prox enqueues the whole static schedule that max_iter allows, each
+ * step and epilogue returning at once for the images already finished (the
+ * same iterates as an eager call), and neither waits nor reads the host: x
+ * and out must be contiguous device arrays, info NULL, keep_structure off,
+ * and the method dual FISTA (FPR_METHOD_ADBPG reads the host between its
+ * steps); fp_prox_steps is refused. The per-image results are read from the
+ * graph's last run by the first fp_workspace_info or fp_workspace_image_info
+ * (which waits for the stream). docs/spec.md:890 of the reference. */
 FP_API fp_status fp_prox(const fp_op *op, double gamma, const fp_array *x, fp_array *out, const fp_options *opt,
                          fp_workspace *ws, fp_info *info);
 /* include/fastprox.h:613-614 -- one step per image. */
@@ -305,7 +314,11 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
 FP_API void      fpr_batch_free(fpr_batch *b);
 /* Solves every image on stream (NULL: the null stream) and waits for it.
  * FP_ENOTCONV when an image stopped at max_iter; FP_EINVAL for an infinite
- * entry; FP_EPARAM for a step that is negative, not finite or out of range. */
+ * entry; FP_EPARAM for a step that is negative, not finite or out of range.
+ * On a stream being captured into a hipGraph it enqueues the static schedule
+ * instead, as fp_prox does, and returns FP_OK at once: the one captured
+ * graph then solves the whole batch; fpr_batch_image_info and
+ * fpr_batch_scale_exps read the graph's last run (waiting for the stream). */
 FP_API fp_status fpr_batch_solve(fpr_batch *b, const fpr_image *images, const fp_options *opt, int warm,
                                  void *stream);
 /* Per image, the results of the last solve (any pointer may be NULL). */
