@@ -284,18 +284,19 @@ __device__ void walk(ImgState &S, const double *gaps, const double *dots, int TT
 // Epilogue: one wavefront per image
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(64) void epilogue_kernel(const ImgDesc *desc, ImgState *state, const double *part,
-                                                      Tile *list_out, int *count_out, int TT, double u_round,
-                                                      double eta, int nimg) {
-    const int img = blockIdx.x;
-    if (img >= nimg) return;
-    const int lane = threadIdx.x;
+// The epilogue of one image, run by one wavefront (lane 0..63): the per-level
+// sums of its tiles' partials (the same fixed tree wherever it runs), the
+// walk, and the image's tiles for the next launch. gaps and dots: kMaxTT + 1
+// doubles of LDS for this wave, read and written by lane 0 only, so no
+// barrier is needed.
+__device__ void epilogue_img(const ImgDesc *desc, ImgState *state, const double *part, Tile *list_out,
+                             int *count_out, int TT, double u_round, double eta, int img, int lane, double *gaps,
+                             double *dots) {
     ImgState &S = state[img];
     const int mode = S.mode;
     if (mode == kDone || mode == kTV1D) return;
     const ImgDesc &D = desc[img];
     const int64_t N = (int64_t)D.L[0] * D.L[1] * D.L[2];
-    __shared__ double gaps[kMaxTT + 1], dots[kMaxTT + 1];
     const int cnt = S.cnt;
     const uint32_t gm = S.gap_mask, tm = S.test_mask;
     if (mode == kIterate) {
@@ -316,16 +317,14 @@ __global__ __launch_bounds__(64) void epilogue_kernel(const ImgDesc *desc, ImgSt
             }
         }
     }
-    __syncthreads();
-    __shared__ int base;
+    int base = -1;
     if (lane == 0) {
         if ((mode == kIterate || mode == kOutput) && cnt > 0) ++S.passes;
         walk(S, gaps, dots, TT, N, u_round, eta);
         const int m = S.mode;
         base = (m == kIterate || m == kOutput || m == kOutputNaN) ? atomicAdd(count_out, D.ntiles) : -1;
     }
-    __syncthreads();
-    const int b = base;
+    const int b = __shfl(base, 0, 64);
     if (b < 0) return;
     const int nxy = D.ntx * D.nty;
     for (int i = lane; i < D.ntiles; i += 64) {
@@ -338,6 +337,15 @@ __global__ __launch_bounds__(64) void epilogue_kernel(const ImgDesc *desc, ImgSt
         t.pad_ = i;
         list_out[b + i] = t;
     }
+}
+
+__global__ __launch_bounds__(64) void epilogue_kernel(const ImgDesc *desc, ImgState *state, const double *part,
+                                                      Tile *list_out, int *count_out, int TT, double u_round,
+                                                      double eta, int nimg) {
+    const int img = blockIdx.x;
+    if (img >= nimg) return;
+    __shared__ double gaps[kMaxTT + 1], dots[kMaxTT + 1];
+    epilogue_img(desc, state, part, list_out, count_out, TT, u_round, eta, img, threadIdx.x, gaps, dots);
 }
 
 // ---------------------------------------------------------------------------
@@ -1451,15 +1459,16 @@ constexpr int stream2d_waves() {
     return TT <= (ISO ? 2 : 4) ? 4 : 3;
 }
 
+// One launch's tiles (the list A.list_in of *A.count_in tiles): groups of
+// four consecutive tiles per workgroup, one per wave, each XCD taking a
+// contiguous range of the groups.
 template <class T, bool ISO, bool SW, int TT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stream2d_waves<T, ISO, SW, TT>())))
-void stream2d_kernel(StepArgs<T> A, int SEG) {
-    __shared__ StreamAcc<TT> accs[4];
+__device__ __forceinline__ void stream2d_body(const StepArgs<T> &A, int SEG, StreamAcc<TT> *accs) {
     // the wave's index is the same on all its lanes: in a scalar register, so
     // that the tile loop compiles as uniform control flow
     const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (blockIdx.x == 0 && threadIdx.x == 0) *A.count_out = 0;
-    const int ntiles = *A.count_in;
+    const int ntiles = __hip_atomic_load(A.count_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int nchunks = (ntiles + 3) >> 2;
     const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, j8 = blockIdx.x >> 3;
     const int cpx = (nchunks + 7) >> 3;
@@ -1474,6 +1483,13 @@ void stream2d_kernel(StepArgs<T> A, int SEG) {
         if (y0 - TT - 1 >= 0 && y0 + SEG + TT < H) stream_tile<T, ISO, SW, TT, true>(A, tl, lane, SEG, accs[wv]);
         else stream_tile<T, ISO, SW, TT, false>(A, tl, lane, SEG, accs[wv]);
     }
+}
+
+template <class T, bool ISO, bool SW, int TT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stream2d_waves<T, ISO, SW, TT>())))
+void stream2d_kernel(StepArgs<T> A, int SEG) {
+    __shared__ StreamAcc<TT> accs[4];
+    stream2d_body<T, ISO, SW, TT>(A, SEG, accs);
 }
 
 // ---------------------------------------------------------------------------
