@@ -187,8 +187,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool-gib", type=float, default=24.0)
-    ap.add_argument("--serve-cpu-requests", type=int, default=None,
-                    help="requests in the served CPU baseline's sample (default: as many as 25 s allow)")
+    ap.add_argument("--serve-cpu-requests", type=int, default=512,
+                    help="requests in the served CPU baseline's sample (stopped at 120 s of CPU time)")
     ap.add_argument("--cpu-table", default=None,
                     help="only time the reference library on the CPU configurations at these thread counts "
                          "(comma-separated) and print that table as JSON; no GPU is used")
@@ -804,11 +804,11 @@ def leg_serve512(args, xs, plain, n=2048):
     return out
 
 
-def serve_cpu_baseline(trace, src2d, src3d, budget_s=25.0, requests=None, seed=1):
+def serve_cpu_baseline(trace, src2d, src3d, budget_s=120.0, requests=None, seed=1):
     """The reference CPU solver's latency per request (no queueing), at 16
     threads, on a random sample of the same trace stratified by class (the
     2D and 3D requests in the trace's proportion, each class in a random
-    order), until `requests` are done or, without a count, the budget ends."""
+    order), until `requests` are done or the budget ends."""
     ref = ROOT / "oracle" / "_ref" / "libfastprox_ref.so"
     if not ref.exists():
         return None
@@ -827,7 +827,7 @@ def serve_cpu_baseline(trace, src2d, src3d, budget_s=25.0, requests=None, seed=1
     while by[2] or by[3]:
         if requests is not None and done >= requests:
             break
-        if requests is None and time.perf_counter() - t_all > budget_s:
+        if time.perf_counter() - t_all > budget_s:
             break
         # the class whose share of the sample is furthest below its share of the trace
         k = 3 if by[3] and (not by[2] or len(lat[3]) < share3 * (done + 1)) else 2
