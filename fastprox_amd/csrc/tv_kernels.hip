@@ -1271,26 +1271,44 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
     auto row_g1 = [&](int r) { return INTERIOR || (r >= 0 && r < H); };
 
     const int ystart = y0 - TT, ylast = yend - 1 + TT;
-    // Row ys + 1's input, loaded one step ahead.
-    T qd = (T)0, qu0 = (T)0, qu1 = (T)0, qv0 = (T)0, qv1 = (T)0, qa = (T)0, qb = (T)0;
-    auto load_row = [&](int ys) {
+    // A row's input, loaded ahead of its step; two slots when rows are
+    // loaded two steps ahead.
+    struct RowIn {
+        T d, u0, u1, v0, v1, a, b;
+    };
+    RowIn q[2] = {};
+    auto load_row = [&](int ys, RowIn &Q) {
         const bool in = xin && (INTERIOR || (ys >= 0 && ys < H));
         const uint32_t o = in ? (uint32_t)(ys * W + x) : 0u;
-        qd = in ? ld32<T>(dp, o) : (T)0;
-        qu0 = qu1 = qv0 = qv1 = (T)0;
+        Q.d = in ? ld32<T>(dp, o) : (T)0;
+        Q.u0 = Q.u1 = Q.v0 = Q.v1 = (T)0;
         if (!fresh && in) {
-            qu0 = ld32<T>(U0, o);
-            qu1 = ld32<T>(U1, o);
-            qv0 = ld32<T>(V0, o);
-            qv1 = ld32<T>(V1, o);
+            Q.u0 = ld32<T>(U0, o);
+            Q.u1 = ld32<T>(U1, o);
+            Q.v0 = ld32<T>(V0, o);
+            Q.v1 = ld32<T>(V1, o);
         }
         if constexpr (SW) {
-            qa = in ? ld32<T>(sw, o) : (T)0;
-            if constexpr (!ISO) qb = in ? ld32<T>(swb, o) : (T)0;
+            Q.a = in ? ld32<T>(sw, o) : (T)0;
+            if constexpr (!ISO) Q.b = in ? ld32<T>(swb, o) : (T)0;
         }
     };
-    load_row(ystart);
-    for (int ys = ystart; ys <= ylast; ++ys) {
+    // The interior float32 tiles of a full launch (an even number of rows:
+    // SEG + 2 TT) load each row two steps ahead, through two fixed slots
+    // that alternate; the others one step ahead through slot 0.
+    const bool deep = F32 && INTERIOR && cnt == TT && (SEG & 1) == 0;
+    load_row(ystart, q[0]);
+    if (deep) load_row(ystart + 1, q[1]);
+    // One step of the row pipeline: row ys enters at level 0 and each level
+    // m produces row ys - m. STEADY: every level is past its skipped rows
+    // (ys >= ystart + 2 TT) and the launch runs all TT levels, so no level
+    // tests either (uniform tests, but each costs scalar instructions and a
+    // branch per level and row). SL: the slot row ys was loaded into, or -1
+    // (slot 0, the next row loaded one step ahead).
+    auto row_step = [&](const int ys, auto steady, auto slot) {
+        constexpr bool STEADY = decltype(steady)::value;
+        constexpr int SL = decltype(slot)::value;
+        RowIn &Q = q[SL < 0 ? 0 : SL];
 #pragma unroll
         for (int m = TT + 1; m >= 1; --m) {
             dw[m] = dw[m - 1];
@@ -1298,17 +1316,21 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             if constexpr (SW && !ISO) rb[m] = rb[m - 1];
         }
         // Level 0: the new row.
-        const T nv0 = qv0, nv1 = qv1, nu0 = qu0, nu1 = qu1;
-        dw[0] = scale_in(qd, e);
+        const T nv0 = Q.v0, nv1 = Q.v1, nu0 = Q.u0, nu1 = Q.u1;
+        dw[0] = scale_in(Q.d, e);
         if constexpr (SW) {
             if constexpr (ISO) {
-                ra[0] = (T)iso_radius(gamd, (double)qa, boundd);
+                ra[0] = (T)iso_radius(gamd, (double)Q.a, boundd);
             } else {
-                ra[0] = (T)(gamd * (double)qa);
-                rb[0] = (T)(gamd * (double)qb);
+                ra[0] = (T)(gamd * (double)Q.a);
+                rb[0] = (T)(gamd * (double)Q.b);
             }
         }
-        if (ys < ylast) load_row(ys + 1);
+        if constexpr (SL < 0) {
+            if (ys < ylast) load_row(ys + 1, Q);
+        } else {
+            if (ys + 2 <= ylast) load_row(ys + 2, Q);
+        }
         T zn = madd(w1, nv1 - from_left(nv1), madd(w0, nv0 - ov0[0], dw[0]));
         G zun = (G)0;
         // The gap of level 0 (the loaded state) on row ys - 1.
@@ -1335,11 +1357,11 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             // rows need level m on rows from y0 - (TT - m) on, reached at
             // ys = ystart + 2m; until then the level is skipped (its values
             // are never read by a level, a partial sum or a store).
-            const bool need = ys >= ystart + 2 * m;
+            const bool need = STEADY || ys >= ystart + 2 * m;
             const T zc = oz[m - 1];
-            const T zr = from_right(zc);
             T v0n, v1n, u0n, u1n;
-            if (m <= cnt && need) {
+            if ((STEADY || m <= cnt) && need) {
+                const T zr = from_right(zc);  // inside the branch: the shift folds into the subtraction (DPP)
                 // An inactive axis has a 0 weight and v = 0 (cold start, or
                 // zeroed by the warm start), so its c is 0 as the reference's
                 // select makes it.
@@ -1388,7 +1410,7 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
             oz[m - 1] = pz;
             ozu[m - 1] = pzu;
             if (m < TT) {
-                if (m < cnt && need) pz = madd(w1, v1n - from_left(v1n), madd(w0, v0n - ov0[m], dw[m]));
+                if ((STEADY || m < cnt) && need) pz = madd(w1, v1n - from_left(v1n), madd(w0, v0n - ov0[m], dw[m]));
                 if (((gm >> m) & 1u) && need) {
                     // The gap of level m on row r - 1.
                     pzu = zrow((G)dw[m], (G)u0n, (G)ou0[m], (G)u1n);
@@ -1434,6 +1456,21 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
                 uf0 = u0n;
             }
         }
+    };
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    int ys = ystart;
+    if (deep) {
+        for (; ys < ystart + 2 * TT; ys += 2) {
+            row_step(ys, std::false_type{}, S0{});
+            row_step(ys + 1, std::false_type{}, S1{});
+        }
+        for (; ys <= ylast; ys += 2) {
+            row_step(ys, std::true_type{}, S0{});
+            row_step(ys + 1, std::true_type{}, S1{});
+        }
+    } else {
+        for (int y = ystart; y <= ylast; ++y) row_step(y, std::false_type{}, std::integral_constant<int, -1>{});
     }
     // The tile's partial sums of the flagged levels.
 #pragma unroll
