@@ -864,7 +864,7 @@ def bt_kernel(bt):
 
 def kernel_name(bt):
     kind, tt, ry = bt_kernel(bt)
-    return {0: f"stream:TT={tt},SEG={ry}", 1: "generic", 2: f"block:TT={tt},RY={ry}", 3: f"s3:SLICES={ry}"}[kind]
+    return {0: f"stream:TT={tt},SEG={ry}", 1: "generic", 2: f"block:TT={tt},RY={ry}", 3: f"s3:TT={tt},SLICES={ry}"}[kind]
 
 
 if __name__ == "__main__":

@@ -8,7 +8,8 @@ about 16 Mpx in all: at 512^2 the 64 images of the headline batch, whose
 tiles run as two halves on two streams) is solved to a gap tolerance of 1e-6 with each candidate:
 the 2D streaming kernel with TT iterations per launch and segments of SEG rows,
 with or without a tail kernel; the 3D streaming kernel with slabs of SLICES
-slices. The fastest candidate (wall time of the whole solve) is kept. Results
+slices and one, two or three iterations per launch (several for float32
+volumes without pixel weights). The fastest candidate (wall time of the whole solve) is kept. Results
 do not depend on the choice (every kernel computes the same iterates).
 
 Small batches are tuned on their own (SMALL_BATCHES: one image, as a plain
@@ -107,12 +108,16 @@ def _tune_one(k, dtype, iso, sw, side, B, small, log, quick):
                 if best is None or sec < best[0]:
                     best = cand
     else:
-        for sl in ((16, 32) if quick else (SMALL_3D if small else SLICES_3D)):
-            bt.set_kernel(3, 1, sl)
-            sec = _time_solve(bt, xs, zs, ds, gam, ws, 2)
-            log(f"  k=3 {dtype} iso={iso} sw={sw} {side}^3 x{B}: s3 SLICES={sl}: {B / sec:.1f} images/s")
-            if best is None or sec < best[0]:
-                best = (sec, 3, 1, sl, 0, 0, 0.0)
+        # several iterations per launch: float32 volumes without pixel weights
+        tts = (1, 2, 3) if tsize == 4 and not sw else (1,)
+        for tt in tts:
+            for sl in ((16, 32) if quick else (SMALL_3D if small else SLICES_3D)):
+                bt.set_kernel(3, tt, sl)
+                sec = _time_solve(bt, xs, zs, ds, gam, ws, 2)
+                log(f"  k=3 {dtype} iso={iso} sw={sw} {side}^3 x{B}: s3 TT={tt} SLICES={sl}: "
+                    f"{B / sec:.1f} images/s")
+                if best is None or sec < best[0]:
+                    best = (sec, 3, tt, sl, 0, 0, 0.0)
     sec, kind, tt, ry, ttt, tseg, frac = best
     del bt, xs, zs, ds, ws
     return (k, tsize, int(iso), int(sw), n, kind, tt, ry, ttt, tseg, frac, B / sec, B if small else 0)

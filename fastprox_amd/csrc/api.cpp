@@ -486,7 +486,7 @@ fpr::KernelChoice generic_kernel() {
     return kc;
 }
 
-fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, fpr::KernelChoice &kc) {
+fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, bool sw, fpr::KernelChoice &kc) {
     kc = fpr::KernelChoice{};
     if (kind == 1) {
         kc.generic = true;
@@ -499,10 +499,12 @@ fp_status kernel_choice(int kind, int tt, int ry, int k, int tsize, fpr::KernelC
         kc.generic = false;
         kc.stream = false;
         kc.s3 = true;
-        kc.TT = 1;
+        kc.TT = tt < 1 ? 1 : tt;
         kc.RY = ry;
         if (k != 3) return fail(FP_EINVAL, "the 3D streaming kernel is 3D only");
-        if (!fpr::kernel_available_s3(ry)) return fail(FP_EINVAL, "slices per tile out of range");
+        if (!fpr::kernel_available_s3(ry, kc.TT, tsize, sw))
+            return fail(FP_EINVAL, "3D streaming kernel: slices per tile out of range, or several iterations per "
+                                   "launch asked of float64 or pixel-weighted volumes");
         return FP_OK;
     }
     if (kind != 0 && kind != 2)
@@ -1944,7 +1946,8 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
         g_last_error[0] = 0;
         if (!ws) return fail(FP_EINVAL, "null workspace");
         fpr::KernelChoice kc;
-        if (fp_status st = kernel_choice(generic, tt, ry, ws->k, (int)ws->tsize, kc)) return st;
+        if (fp_status st = kernel_choice(generic, tt, ry, ws->k, (int)ws->tsize, !ws->op->spatial.empty(), kc))
+            return st;
         if (kc.stream && !fpr::stream_fits((int)ws->tsize, ws->N))
             return fail(FP_EINVAL, "images too large for the streaming kernel");
         ws->kc = kc;
@@ -2172,7 +2175,7 @@ FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry)
     return guard([&] {
         if (!b) return FP_EINVAL;
         fpr::KernelChoice kc;
-        if (fp_status st = kernel_choice(generic, tt, ry, b->op->k, (int)b->tsize, kc)) return st;
+        if (fp_status st = kernel_choice(generic, tt, ry, b->op->k, (int)b->tsize, b->weighted != 0, kc)) return st;
         if (kc.stream && !fpr::stream_fits((int)b->tsize, b->maxN))
             return fail(FP_EINVAL, "images too large for the streaming kernel");
         b->kc = kc;

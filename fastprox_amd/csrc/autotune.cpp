@@ -99,9 +99,9 @@ bool entry_kernel(const Entry &e, int tsize, KernelChoice &kc) {
         case 3:
             kc.stream = false;
             kc.s3 = true;
-            kc.TT = 1;
+            kc.TT = e.TT < 1 ? 1 : e.TT;
             kc.RY = e.RY;
-            return e.k == 3 && kernel_available_s3(e.RY);
+            return e.k == 3 && kernel_available_s3(e.RY, kc.TT, tsize, e.sw);
     }
     return false;
 }
@@ -113,7 +113,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
     if (k != 2) {
         kc.stream = false;
         kc.TT = 1;
-        if (const char *e = std::getenv("FASTPROX_ROCM_KERNEL3D")) {  // "generic" or "s3:SLICES"
+        if (const char *e = std::getenv("FASTPROX_ROCM_KERNEL3D")) {  // "generic" or "s3:SLICES[,TT]"
             if (e[0] == 'g') {
                 kc.generic = true;
                 kc.RY = 4;
@@ -121,10 +121,17 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
             }
             const char *c = e;
             while (*c && *c != ':') ++c;
-            if (*c == ':' && kernel_available_s3(std::atoi(c + 1))) {
-                kc.s3 = true;
-                kc.RY = std::atoi(c + 1);
-                return kc;
+            if (*c == ':') {
+                const int ry = std::atoi(c + 1);
+                int tt = 1;
+                for (const char *q = c; *q; ++q)
+                    if (*q == ',') tt = std::atoi(q + 1);
+                if (kernel_available_s3(ry, tt, tsize, sw)) {
+                    kc.s3 = true;
+                    kc.RY = ry;
+                    kc.TT = tt;
+                    return kc;
+                }
             }
         }
         if (const Entry *t = lookup(k, tsize, img, iso, sw, B))

@@ -1964,6 +1964,320 @@ __global__ __launch_bounds__(64 * NW) void stream3d_kernel(StepArgs<T> A, int SE
 }
 
 // ---------------------------------------------------------------------------
+// The 3D temporally tiled step kernel (float32, no pixel weights): TT
+// iterations per launch, one HBM pass. A workgroup of kS3TWaves waves owns
+// a tile of (64 - 2 TT) x (kS3TWaves - 2 TT) voxels of every slice of a slab
+// of SEGZ slices, with TT halo columns and rows on each side: wave w is row
+// y0 - TT + w, lane l column x0 - TT + l. The slab is walked slice by slice;
+// the TT levels are pipelined two slices apart, so that within one step no
+// level waits for another wave: at step s level m updates slice s - 2m (its
+// gradient needs z of level m - 1 at that slice and the next, both already
+// computed, and at the next row, which the wave below published at step
+// s - 1) and forms z of its previous slice s - 2m - 1 (whose v_1 at the row
+// above the wave above published at step s - 1). Each step publishes, per
+// level, v_1 of the slice it updated and z of the slice it formed (and, for
+// a gap level, u_1 and z(u)) in a double-buffered LDS exchange, and ends
+// with the one barrier of the step. The halo shrinks by one column, row and
+// slice per level, so the tile's own voxels are exact after TT levels.
+// Per voxel the arithmetic is step1_kernel's (float32, fused where the
+// expression allows).
+// ---------------------------------------------------------------------------
+
+template <int TT>
+struct S3TLds {
+    float v1[2][kS3TWaves][TT][64];
+    float z[2][kS3TWaves][TT][64];
+    float u1[2][kS3TWaves][TT + 1][64];
+    float zu[2][kS3TWaves][TT][64];
+    double red[kS3TWaves][2 * (TT + 1)];
+};
+
+template <bool ISO, int TT>
+__global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float> A, int SEGZ) {
+    using T = float;
+    constexpr int NW = kS3TWaves;
+    constexpr int TXI = 64 - 2 * TT, TYI = NW - 2 * TT;
+    constexpr int ND = 2 * TT + 2;  // the d delay line: d at slices s .. s - 2 TT - 1
+    __shared__ S3TLds<TT> X;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (blockIdx.x == 0 && tid == 0) *A.count_out = 0;
+    const int ntiles = __hip_atomic_load(A.count_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, j8 = blockIdx.x >> 3;
+    const int cpx = (ntiles + 7) >> 3;
+    const int cend = min((xcd + 1) * cpx, ntiles);
+    for (int ti = xcd * cpx + j8; ti < cend; ti += per) {
+        const Tile tl = A.list_in[ti];
+        const int img = __builtin_amdgcn_readfirstlane(tl.img);
+        const ImgDesc &D = A.desc[img];
+        ImgState &S = A.state[img];
+        const int mode = S.mode;
+        const int L0 = D.L[0], H = D.L[1], W = D.L[2];
+        const int64_t SZ = (int64_t)H * W, N = SZ * L0;
+        const int x = __builtin_amdgcn_readfirstlane(tl.tx) * TXI - TT + lane;
+        const int y = __builtin_amdgcn_readfirstlane(tl.ty) * TYI - TT + wv;
+        const int z0 = __builtin_amdgcn_readfirstlane(tl.tz) * SEGZ, z1 = min(z0 + SEGZ, L0);
+        const bool xin = x >= 0 && x < W, yin = y >= 0 && y < H, inxy = xin && yin;
+        const bool own = inxy && lane >= TT && lane < 64 - TT && wv >= TT && wv < NW - TT;
+        const int64_t oxy = inxy ? (int64_t)y * W + x : 0;
+        double *part = A.part + (int64_t)(D.tile_base + __builtin_amdgcn_readfirstlane(tl.pad_)) * kSlots;
+        if (mode == kOutputNaN) {
+            if (own)
+                for (int t = z0; t < z1; ++t) {
+                    const int64_t o = t * SZ + oxy;
+                    gout<T>(D.z)[o] = (T)NAN;
+                    for (int a = 0; a < 3; ++a) gout<T>(D.q)[a * D.q_cstride + o] = (T)NAN;
+                }
+            continue;
+        }
+        const int e = S.e, cnt = S.cnt;
+        const uint32_t tm = S.test_mask, gm = S.gap_mask;
+        const bool out_mode = mode == kOutput;
+        const bool fresh = S.fresh;
+        const double wd0 = S.wa[0], wd1 = S.wa[1], wd2 = S.wa[2];
+        const T w0 = (T)wd0, w1 = (T)wd1, w2 = (T)wd2;
+        const T tau = (T)S.tau;
+        const double gamd = S.gamma;
+        const T gam = (T)gamd;
+        T beta[TT + 1];
+#pragma unroll
+        for (int m = 1; m <= TT; ++m) beta[m] = (T)S.beta[m];
+        // Per-lane weights of the in-plane forward differences (0 where the
+        // neighbour leaves the image or the axis is inactive); of tau g for
+        // the update (t*), of g for the gap (g*).
+        const T gw1 = (wd1 != 0.0 && inxy && y + 1 < H) ? w1 : (T)0;
+        const T gw2 = (wd2 != 0.0 && inxy && x + 1 < W) ? w2 : (T)0;
+        const T tw1 = tau * gw1, tw2 = tau * gw2;
+        const bool act0 = wd0 != 0.0;
+        const auto dp = gin<T>(D.d);
+        const T *U = A.arena + D.st_off + (int64_t)S.cur * 6 * N;
+        const T *V = S.vsrc_u ? U : U + 3 * N;
+        T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 6 * N;
+        // the axis-0 weight of a slice: 0 outside the image or at its last slice
+        auto gw0_at = [&](int t) -> T { return (act0 && inxy && t >= 0 && t + 1 < L0) ? w0 : (T)0; };
+        auto in_slice = [&](int t) { return inxy && t >= 0 && t < L0; };
+
+        // Level m's ring: A = slice t_m - 2, B = slice t_m - 1 (t_m = s - 2 m).
+        T av[TT + 1][3], au[TT + 1][3], bv[TT + 1][3], bu[TT + 1][3];
+        T za[TT], zb[TT];    // z of level m at slices t_m - 2 and t_m - 1
+        T zua[TT], zub[TT];  // z(u) of a gap level, the same slices
+        T dq[ND];            // scaled d at slices s - k
+        double gacc[TT], dacc[TT + 1];
+#pragma unroll
+        for (int m = 0; m <= TT; ++m) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) av[m][a] = au[m][a] = bv[m][a] = bu[m][a] = (T)0;
+            dacc[m] = 0.0;
+            if (m < TT) {
+                za[m] = zb[m] = zua[m] = zub[m] = (T)0;
+                gacc[m] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < ND; ++k) dq[k] = (T)0;
+
+        // Level 0's input, loaded one step ahead.
+        T nv[3], nu[3], nd;
+        const int s_last_load = z1 - 1 + TT;
+        auto load_slice = [&](int s) {
+            const bool ok = in_slice(s) && s <= s_last_load;
+            const int64_t o = ok ? s * SZ + oxy : 0;
+            nd = ok ? dp[o] : (T)0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                nv[a] = (ok && !fresh) ? V[a * N + o] : (T)0;
+                nu[a] = (ok && !fresh) ? U[a * N + o] : (T)0;
+            }
+        };
+        const int sbeg = z0 - TT;
+        const int send = z1 - 1 + 2 * TT + (out_mode ? 1 : 0);
+        load_slice(sbeg);
+        for (int s = sbeg; s <= send; ++s) {
+            const int p = s & 1, pp = p ^ 1;
+            // Level 0: slice s.
+            T cv[TT + 1][3], cu[TT + 1][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                cv[0][a] = nv[a];
+                cu[0][a] = nu[a];
+            }
+#pragma unroll
+            for (int k = ND - 1; k >= 1; --k) dq[k] = dq[k - 1];
+            dq[0] = scale_in(nd, e);
+            if (s < send) load_slice(s + 1);
+#pragma unroll
+            for (int m = 0; m <= TT; ++m) {
+                const int t = s - 2 * m;  // the slice level m updates
+                if (m >= 1) {
+                    // UPDATE(m) at slice t from level m - 1 (its ring's A is slice t).
+                    const bool upd = t >= z0 - (TT - m) && t <= z1 - 1 + (TT - m);
+                    if (upd && m <= cnt) {
+                        const T zc = za[m - 1];
+                        const T zyn = wv + 1 < NW ? X.z[pp][wv + 1][m - 1][lane] : (T)0;
+                        const T zxn = from_right(zc);
+                        // outside the image every c is 0, so v stays 0 there
+                        const bool tin = in_slice(t);
+                        const T g0 = gw0_at(t) * (zb[m - 1] - zc);
+                        const T c0 = act0 && tin ? madd(tau, g0, av[m - 1][0]) : (T)0;
+                        const T c1 = tin ? madd(tw1, zyn - zc, av[m - 1][1]) : (T)0;
+                        const T c2 = tin ? madd(tw2, zxn - zc, av[m - 1][2]) : (T)0;
+                        T n0, n1, n2;
+                        if constexpr (ISO) {
+                            const T sc = ball_scale(madd(c2, c2, madd(c1, c1, c0 * c0)), gam);
+                            n0 = c0 * sc;
+                            n1 = c1 * sc;
+                            n2 = c2 * sc;
+                        } else {
+                            n0 = c0 > gam ? gam : (c0 < -gam ? -gam : c0);
+                            n1 = c1 > gam ? gam : (c1 < -gam ? -gam : c1);
+                            n2 = c2 > gam ? gam : (c2 < -gam ? -gam : c2);
+                        }
+                        if ((tm >> m) & 1u) {
+                            const T dd = (av[m - 1][0] - n0) * (n0 - au[m - 1][0]) +
+                                         (av[m - 1][1] - n1) * (n1 - au[m - 1][1]) +
+                                         (av[m - 1][2] - n2) * (n2 - au[m - 1][2]);
+                            if (own && t >= z0 && t < z1) dacc[m] += (double)dd;
+                        }
+                        cv[m][0] = madd(beta[m], n0 - au[m - 1][0], n0);
+                        cv[m][1] = madd(beta[m], n1 - au[m - 1][1], n1);
+                        cv[m][2] = madd(beta[m], n2 - au[m - 1][2], n2);
+                        cu[m][0] = n0;
+                        cu[m][1] = n1;
+                        cu[m][2] = n2;
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            cv[m][a] = av[m - 1][a];
+                            cu[m][a] = au[m - 1][a];
+                        }
+                    }
+                }
+                if (m < TT) {
+                    // ZCOMP(m): z (and on a gap level z(u)) of slice t - 1 from
+                    // the ring's B (t - 1) and A (t - 2) and the row above.
+                    const int tz = t - 1;
+                    // needed by UPDATE(m + 1) on its slices and the next, and by the gap
+                    const bool zon = tz >= z0 - TT + m + 1 && tz <= z1 - 1 + (TT - m);
+                    if (zon) {
+                        const T v1u = wv > 0 ? X.v1[pp][wv - 1][m][lane] : (T)0;
+                        zb[m] = madd(w2, bv[m][2] - from_left(bv[m][2]),
+                                     madd(w1, bv[m][1] - v1u, madd(w0, bv[m][0] - av[m][0], dq[2 * m + 1])));
+                        if ((gm >> m) & 1u) {
+                            const T u1u = wv > 0 ? X.u1[pp][wv - 1][m][lane] : (T)0;
+                            zub[m] = madd(w2, bu[m][2] - from_left(bu[m][2]),
+                                          madd(w1, bu[m][1] - u1u, madd(w0, bu[m][0] - au[m][0], dq[2 * m + 1])));
+                        }
+                        // The gap of level m at slice t - 2 (its ring's A), a slab slice.
+                        const int tg = t - 2;
+                        if (((gm >> m) & 1u) && tg >= z0 && tg < z1) {
+                            const T zc = zua[m];
+                            const T zyn = wv + 1 < NW ? X.zu[pp][wv + 1][m][lane] : (T)0;
+                            const T zxn = from_right(zc);  // every lane, before the per-lane selects
+                            const bool tin = in_slice(tg);
+                            const T g0 = tin ? gw0_at(tg) * (zub[m] - zc) : (T)0;
+                            const T g1 = tin ? gw1 * (zyn - zc) : (T)0;
+                            const T g2 = tin ? gw2 * (zxn - zc) : (T)0;
+                            T gp;
+                            if constexpr (ISO) {
+                                gp = gam * sqrt32(g0 * g0 + g1 * g1 + g2 * g2) -
+                                     (g0 * au[m][0] + g1 * au[m][1] + g2 * au[m][2]);
+                            } else {
+                                gp = (gam * fabsf(g0) - g0 * au[m][0]) + (gam * fabsf(g1) - g1 * au[m][1]) +
+                                     (gam * fabsf(g2) - g2 * au[m][2]);
+                            }
+                            if (own) gacc[m] += (double)gp;
+                        }
+                    }
+                }
+            }
+            // The final level: the iterate of slice s - 2 TT; on an output
+            // launch, z and the dual of slice s - 2 TT - 1 (from the ring).
+            {
+                const int t = s - 2 * TT;
+                if (cnt > 0 && own && t >= z0 && t < z1) {
+                    const int64_t o = t * SZ + oxy;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        Uo[a * N + o] = cu[TT][a];
+                        Uo[(3 + a) * N + o] = cv[TT][a];
+                    }
+                }
+                if (out_mode) {
+                    const int to = t - 1;
+                    const T u1u = wv > 0 ? X.u1[pp][wv - 1][TT][lane] : (T)0;
+                    const double ul2 = from_left((double)bu[TT][2]);
+                    if (own && to >= z0 && to < z1) {
+                        const int64_t o = to * SZ + oxy;
+                        const double zz = (((double)dq[2 * TT + 1] + wd0 * ((double)bu[TT][0] - (double)au[TT][0])) +
+                                           wd1 * ((double)bu[TT][1] - (double)u1u)) +
+                                          wd2 * ((double)bu[TT][2] - ul2);
+                        gout<T>(D.z)[o] = (T)ldexp(zz, e);
+                        if constexpr (ISO) {
+#pragma unroll
+                            for (int a = 0; a < 3; ++a)
+                                gout<T>(D.q)[a * D.q_cstride + o] = (T)ldexp((double)bu[TT][a] / S.wmax, e);
+                        } else {
+                            gout<T>(D.q)[o] = (T)(wd0 * ((double)au[TT][0] - (double)bu[TT][0]));
+                            gout<T>(D.q)[D.q_cstride + o] = (T)(wd1 * ((double)u1u - (double)bu[TT][1]));
+                            gout<T>(D.q)[2 * D.q_cstride + o] = (T)(wd2 * (ul2 - (double)bu[TT][2]));
+                        }
+                    }
+                }
+            }
+            // Publish this step's values, rotate the rings, one barrier.
+#pragma unroll
+            for (int m = 0; m < TT; ++m) {
+                X.v1[p][wv][m][lane] = cv[m][1];
+                X.z[p][wv][m][lane] = zb[m];
+                if ((gm >> m) & 1u) {
+                    X.u1[p][wv][m][lane] = cu[m][1];
+                    X.zu[p][wv][m][lane] = zub[m];
+                }
+            }
+            if (out_mode) X.u1[p][wv][TT][lane] = cu[TT][1];
+#pragma unroll
+            for (int m = 0; m <= TT; ++m) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    av[m][a] = bv[m][a];
+                    au[m][a] = bu[m][a];
+                    bv[m][a] = cv[m][a];
+                    bu[m][a] = cu[m][a];
+                }
+                if (m < TT) {
+                    za[m] = zb[m];
+                    zua[m] = zub[m];
+                }
+            }
+            __syncthreads();
+        }
+        // The tile's partial sums: per wave, then over the waves in order.
+#pragma unroll
+        for (int m = 0; m <= TT; ++m) {
+            if (m < TT && ((gm >> m) & 1u)) {
+                const double v = wave_sum(gacc[m]);
+                if (lane == 0) X.red[wv][2 * m] = v;
+            }
+            if (m >= 1 && ((tm >> m) & 1u)) {
+                const double v = wave_sum(dacc[m]);
+                if (lane == 0) X.red[wv][2 * m + 1] = v;
+            }
+        }
+        __syncthreads();
+        if (tid < 2 * (TT + 1)) {
+            const int m = tid >> 1;
+            const bool on = (tid & 1) ? (m >= 1 && ((tm >> m) & 1u)) : (m < TT && ((gm >> m) & 1u));
+            if (on) {
+                double sum = 0.0;
+                for (int i = 0; i < NW; ++i) sum += X.red[i][tid];
+                part[tid] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Bookkeeping kernels
 // ---------------------------------------------------------------------------
 
@@ -2178,6 +2492,15 @@ template <class T, bool ISO, bool SW>
 static void step_dispatch(const KernelChoice &kc, const StepArgs<T> &A, int grid, hipStream_t s, hipEvent_t t0,
                           hipEvent_t t1) {
     if (kc.s3) {
+        if (kc.TT >= 2) {
+            if constexpr (sizeof(T) == 4 && !SW) {
+                if (kc.TT == 2)
+                    launch_timed(stream3t_kernel<ISO, 2>, dim3(grid), dim3(64 * kS3TWaves), s, t0, t1, A, kc.RY);
+                else if (kc.TT == 3)
+                    launch_timed(stream3t_kernel<ISO, 3>, dim3(grid), dim3(64 * kS3TWaves), s, t0, t1, A, kc.RY);
+            }
+            return;  // kernel_available_s3 admits no other combination
+        }
         launch_timed(stream3d_kernel<T, ISO, SW, kS3Waves>, dim3(grid), dim3(64 * kS3Waves), s, t0, t1, A, kc.RY);
         return;
     }
@@ -2224,7 +2547,10 @@ void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A
     }
 }
 
-bool kernel_available_s3(int RY) { return RY >= 1 && RY <= 4096; }
+bool kernel_available_s3(int RY, int TT, int dtype_bytes, bool sw) {
+    if (RY < 1 || RY > 4096) return false;
+    return TT == 1 || ((TT == 2 || TT == 3) && dtype_bytes == 4 && !sw);
+}
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY) {
     if (stream) return (TT == 2 || TT == 4 || TT == 6 || TT == 8) && RY >= 1 && RY <= 4096;

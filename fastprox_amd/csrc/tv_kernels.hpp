@@ -40,23 +40,31 @@ struct GatherEntry {
 // one-iteration kernel (2D and 3D).
 // stream: the streaming kernel (TT iterations, segments of RY rows); else
 // the block kernel (tiles of RY rows in registers, TT iterations).
-// s3: the 3D streaming kernel (one iteration per launch; tiles of 62 x
-// (kS3Waves - 1) pixels by RY slices).
+// s3: the 3D streaming kernels. TT 1: one iteration per launch, tiles of
+// 62 x (kS3Waves - 1) voxels by RY slices; TT >= 2 (float32, no pixel
+// weights): TT iterations per launch, tiles of (64 - 2 TT) x (kS3TWaves -
+// 2 TT) voxels by RY slices.
 constexpr int kS3Waves = 8;
+constexpr int kS3TWaves = 16;
 struct KernelChoice {
     bool generic = false;
     bool stream = true;
     bool s3 = false;
     int TT = 4;
     int RY = 64;
-    int tile_x() const { return s3 ? 62 : (generic ? 64 : 64 - 2 * TT); }
-    int tile_y() const { return s3 ? kS3Waves - 1 : (generic ? 4 : (stream ? RY : RY - 2 * TT)); }
+    bool s3t() const { return s3 && TT >= 2; }
+    int tile_x() const { return s3 ? (TT >= 2 ? 64 - 2 * TT : 62) : (generic ? 64 : 64 - 2 * TT); }
+    int tile_y() const {
+        return s3 ? (TT >= 2 ? kS3TWaves - 2 * TT : kS3Waves - 1) : (generic ? 4 : (stream ? RY : RY - 2 * TT));
+    }
     int64_t tiles_z(int64_t L0) const { return s3 ? (L0 + RY - 1) / RY : (generic ? L0 : 1); }
-    int iters() const { return (generic || s3) ? 1 : TT; }
+    int iters() const { return generic ? 1 : (s3 ? (TT >= 2 ? TT : 1) : TT); }
 };
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY);
-bool kernel_available_s3(int RY);
+// The 3D streaming kernel with TT iterations per launch (TT 1: any dtype and
+// weights; TT 2 or 3: float32 without pixel weights) and slabs of RY slices.
+bool kernel_available_s3(int RY, int TT = 1, int dtype_bytes = 8, bool sw = false);
 
 // The prologue's per-block partials: 4 doubles per block, prologue_blocks(maxN) blocks per image.
 int prologue_blocks(int64_t maxN);
