@@ -186,7 +186,9 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
 //   TT 4, 16 rows: 3,364 (3,624),                 p50 0.67 ms, p99 37 ms
 // Short launches help a lone request, long ones a full server. 3D: slabs of
 // 8 slices (the 3D requests set the trace's p99; the batch kernel's 16-slice
-// slabs lengthen it).
+// slabs lengthen it), two iterations per launch for float32 (the same trace
+// for 20 s: p99 37.3 -> 30.4 ms, the 3D requests' p99 54.2 -> 47.2 ms against
+// one iteration per launch; 16-slice slabs 30.8 / 47.3 ms).
 KernelChoice serve_kernel(int k, int tsize, bool iso) {
     const char *env = std::getenv(k == 2 ? "FASTPROX_ROCM_KERNEL" : "FASTPROX_ROCM_KERNEL3D");
     if (env || k == 2) {
@@ -194,7 +196,7 @@ KernelChoice serve_kernel(int k, int tsize, bool iso) {
         return default_kernel(k, tsize, img, iso, false, 64);
     }
     KernelChoice kc;
-    kc.TT = 1;
+    kc.TT = kernel_available_s3(8, 2, tsize, false) ? 2 : 1;
     kc.RY = 8;
     kc.stream = false;
     kc.s3 = true;

@@ -36,7 +36,9 @@ def test_every_line_names_a_built_kernel():
             if r["frac"] > 0:
                 assert r["tTT"] in (2, 4, 6, 8) and r["tTT"] >= r["TT"] and 0 < r["frac"] < 1
         else:
-            assert r["kind"] == 3 and r["TT"] == 1 and 1 <= r["RY"] <= 4096 and r["frac"] == 0
+            assert r["kind"] == 3 and 1 <= r["RY"] <= 4096 and r["frac"] == 0
+            # several iterations per launch: float32 volumes without pixel weights only
+            assert r["TT"] == 1 or (r["TT"] in (2, 3) and r["tsize"] == 4 and r["sw"] == 0)
 
 
 @pytest.mark.parametrize("cls", autotune.CLASSES, ids=lambda c: f"k{c[0]}-b{c[1]}-iso{int(c[2])}-sw{int(c[3])}")
