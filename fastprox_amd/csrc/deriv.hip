@@ -356,28 +356,10 @@ struct FPixD {  // a double pixel array
     __device__ double operator()(int p, int) const { return v[p]; }
 };
 
-// The apply's pixel values (tv_iso.cpp:881-887): pv_C - sum_a w_a (h_a[p] -
-// h_a[p - e_a]), zero for frozen images.
-struct FApply {
-    Geo G;
-    const double *pv, *h;
-    const DImg *di;
-    const CgState *cg;
-    __device__ double operator()(int gp, int C) const {
-        const int b = img_of(G, gp);
-        if (!cg[b].active) return 0.0;
-        const DImg &D = di[b];
-        int64_t c[3];
-        coords(G, gp - (int64_t)b * G.N, c);
-        double q = -pv[C];
-        for (int a = 0; a < G.k; ++a) {
-            if (D.wa[a] == 0.0) continue;
-            const int ax = 3 - G.k + a;
-            const double hp = h[a * G.BN + gp];
-            q += c[ax] > 0 ? D.wa[a] * (hp - h[a * G.BN + gp - G.S[ax]]) : D.wa[a] * hp;
-        }
-        return -q;
-    }
+// Values already in member order (the apply's, apply_kernel).
+struct FPos {
+    const double *v;
+    __device__ double operator()(int i, int) const { return v[i]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -406,6 +388,28 @@ __device__ __forceinline__ double curv(const Geo &G, const DImg &D, const double
     return radius(D, sw, sw_img, b, p) / dmaxd(nr, D.mu);
 }
 
+// h = H_q G (M pv) at pixel gq (local index q, coordinates c) of image b
+// (the apply's first half): 0 at a flat pixel.
+template <class T>
+__device__ __forceinline__ void h_at(const Geo &G, const DImg &D, const double *gz, const uint8_t *flat,
+                                     const int *label, const double *pv, const T *sw, int64_t sw_img, int b,
+                                     int64_t gq, int64_t q, const int64_t *c, double *y) {
+    if (flat[gq] & 1) {
+        for (int a = 0; a < G.k; ++a) y[a] = 0.0;
+        return;
+    }
+    const double x0 = pv[label[gq]];
+    double g[3];
+    for (int a = 0; a < G.k; ++a) {
+        const int ax = 3 - G.k + a;
+        y[a] = (D.wa[a] != 0.0 && c[ax] + 1 < G.L[ax]) ? D.wa[a] * (pv[label[gq + G.S[ax]]] - x0) : 0.0;
+        g[a] = gz[a * G.BN + gq];
+    }
+    double nr;
+    const double hs = curv(G, D, g, sw, sw_img, b, q, &nr);
+    hess_apply(G.k, g, hs, y);
+}
+
 // h = H G (M pv) at every pixel of a live image (the apply's first half).
 template <class T>
 __global__ __launch_bounds__(kThr) void hgv_kernel(Geo G, const DImg *di, const CgState *cg, const double *gz,
@@ -416,24 +420,150 @@ __global__ __launch_bounds__(kThr) void hgv_kernel(Geo G, const DImg *di, const 
     const int b = img_of(G, gp);
     if (!cg[b].active) return;
     const int64_t p = gp - (int64_t)b * G.N;
-    const DImg &D = di[b];
-    if (flat[gp] & 1) {
-        for (int a = 0; a < G.k; ++a) h[a * G.BN + gp] = 0.0;
-        return;
-    }
     int64_t c[3];
     coords(G, p, c);
-    const double x0 = pv[label[gp]];
-    double y[3], g[3];
-    for (int a = 0; a < G.k; ++a) {
-        const int ax = 3 - G.k + a;
-        y[a] = (D.wa[a] != 0.0 && c[ax] + 1 < G.L[ax]) ? D.wa[a] * (pv[label[gp + G.S[ax]]] - x0) : 0.0;
-        g[a] = gz[a * G.BN + gp];
-    }
-    double nr;
-    const double hs = curv(G, D, g, sw, sw_img, b, p, &nr);
-    hess_apply(G.k, g, hs, y);
+    double y[3];
+    h_at(G, di[b], gz, flat, label, pv, sw, sw_img, b, gp, p, c, y);
     for (int a = 0; a < G.k; ++a) h[a * G.BN + gp] = y[a];
+}
+
+// The apply's second half in pixel order (tv_iso.cpp:881-887): the value of
+// pixel p, pv_C - sum_a w_a (h_a[p] - h_a[p - e_a]) (0 for a frozen image),
+// stored at p's position in the member order, so that the region sums read
+// their members contiguously.
+__global__ __launch_bounds__(kThr) void div_kernel(Geo G, const DImg *di, const CgState *cg, const int *label,
+                                                   const int *pos, const double *pv, const double *h, double *vm) {
+    const int64_t gp = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (gp >= G.BN) return;
+    const int b = img_of(G, gp);
+    double v = 0.0;
+    if (cg[b].active) {
+        const DImg &D = di[b];
+        int64_t c[3];
+        coords(G, gp - (int64_t)b * G.N, c);
+        double q = -pv[label[gp]];
+        for (int a = 0; a < G.k; ++a) {
+            if (D.wa[a] == 0.0) continue;
+            const int ax = 3 - G.k + a;
+            const double hp = h[a * G.BN + gp];
+            q += c[ax] > 0 ? D.wa[a] * (hp - h[a * G.BN + gp - G.S[ax]]) : D.wa[a] * hp;
+        }
+        v = -q;
+    }
+    vm[pos[gp]] = v;
+}
+
+// 2D: both halves of the apply in one pass (h_at and div_kernel's value,
+// the same expressions in the same order). A wave covers 64 columns, x0 - 1
+// .. x0 + 62, of which lanes 1..62 are its own (lane 0 supplies h of the
+// column before them, lane 63 the region value of the column after them),
+// and walks rows y0 - 1 .. y0 + kApplyRows: at row y it loads the row's
+// region values (one gather per pixel: the right neighbour's comes from the
+// next lane, the lower neighbour's one row later), forms h of row y - 1,
+// and the value of row y - 1 from h there, h of the column before (the
+// previous lane) and h of the row above (the previous step).
+constexpr int kApplyRows = 8;
+constexpr int kApplyCols = 62;
+
+__device__ __forceinline__ double lane_from_left(double v) {  // lane i: lane i - 1's v (lane 0: 0)
+    return __shfl_up(v, 1, 64);
+}
+__device__ __forceinline__ double lane_from_right(double v) {  // lane i: lane i + 1's v (lane 63: its own)
+    return __shfl_down(v, 1, 64);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void apply2d_kernel(Geo G, const DImg *di, const CgState *cg, const double *gz,
+                                                      const uint8_t *flat, const int *label, const int *pos,
+                                                      const double *pv, const T *sw, int64_t sw_img, int ntx,
+                                                      int nty, int B, double *vm) {
+    const int lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t per = (int64_t)ntx * nty;
+    const int b = (int)(tile / per);
+    if (b >= B || !cg[b].active) return;
+    const int r = (int)(tile - (int64_t)b * per);
+    const int ty = r / ntx, tx = r - ty * ntx;
+    const DImg &D = di[b];
+    const int H = (int)G.L[1], W = (int)G.L[2];
+    const int x = tx * kApplyCols - 1 + lane;
+    const int y0 = ty * kApplyRows, yend = min(y0 + kApplyRows, H);
+    const bool xin = x >= 0 && x < W;
+    const bool own_col = xin && lane >= 1 && lane <= kApplyCols;
+    const int64_t base = (int64_t)b * G.N;
+    const double w0 = D.wa[0], w1 = D.wa[1];  // axis 0: rows, axis 1: columns
+    // Row yr's values at this lane (region value, flat, g) are loaded one
+    // step ahead and its label two steps ahead, so that neither the loads
+    // nor the label -> region-value gather wait inside a step.
+    auto row_in = [&](int yr) { return xin && yr >= 0 && yr < H; };
+    auto gpix = [&](int yr) { return row_in(yr) ? base + (int64_t)yr * W + x : (int64_t)0; };
+    int lab1 = row_in(y0) ? label[gpix(y0)] : 0;  // row y0's label
+    double xv, g0, g1;
+    bool fl;
+    {
+        const int yr = y0 - 1;
+        const bool in = row_in(yr);
+        const int64_t gp = gpix(yr);
+        xv = in ? pv[label[gp]] : 0.0;
+        fl = in ? (flat[gp] & 1) != 0 : true;
+        g0 = in ? gz[gp] : 0.0;
+        g1 = in ? gz[G.BN + gp] : 0.0;
+    }
+    // the previous row's (the row whose h is formed next)
+    double pxv = 0.0, pg0 = 0.0, pg1 = 0.0;
+    bool pfl = true;
+    double ph0 = 0.0;  // h_0 of the row before that (the row above the value's row)
+    for (int yr = y0 - 1; yr <= yend; ++yr) {
+        // the next row's values and the row after's label, in flight during this step
+        const bool in1 = row_in(yr + 1), in2 = row_in(yr + 2) && yr + 2 <= yend;
+        const int64_t gp1 = gpix(yr + 1);
+        const double xv1 = in1 ? pv[lab1] : 0.0;
+        const bool fl1 = in1 ? (flat[gp1] & 1) != 0 : true;
+        const double g01 = in1 ? gz[gp1] : 0.0, g11 = in1 ? gz[G.BN + gp1] : 0.0;
+        const int lab2 = in2 ? label[gpix(yr + 2)] : 0;
+        const double xr = lane_from_right(pxv);  // the right neighbour of row yr - 1
+        const int y = yr - 1;                    // the row whose h is formed now
+        if (y >= y0 - 1 && y >= 0) {
+            const bool pin = xin && y < H;
+            double h0 = 0.0, h1 = 0.0;
+            if (pin && !pfl) {
+                // h_at: y_a = w_a (pv[label[p + e_a]] - x0) where p + e_a is in the image
+                double yy[2];
+                yy[0] = (w0 != 0.0 && y + 1 < H) ? w0 * (xv - pxv) : 0.0;
+                yy[1] = (w1 != 0.0 && x + 1 < W) ? w1 * (xr - pxv) : 0.0;
+                double g[2] = {pg0, pg1};
+                const int64_t q = (int64_t)y * W + x;
+                double nr;
+                const double hs = curv(G, D, g, sw, sw_img, b, q, &nr);
+                hess_apply(2, g, hs, yy);
+                h0 = yy[0];
+                h1 = yy[1];
+            }
+            const double h1l = lane_from_left(h1);  // h_1 of the column before
+            if (own_col && y >= y0 && y < yend) {
+                double qv = -pxv;
+                if (w0 != 0.0) qv += y > 0 ? w0 * (h0 - ph0) : w0 * h0;
+                if (w1 != 0.0) qv += x > 0 ? w1 * (h1 - h1l) : w1 * h1;
+                vm[pos[base + (int64_t)y * W + x]] = -qv;
+            }
+            ph0 = h0;
+        }
+        pxv = xv;
+        pfl = fl;
+        pg0 = g0;
+        pg1 = g1;
+        xv = xv1;
+        fl = fl1;
+        g0 = g01;
+        g1 = g11;
+        lab1 = lab2;
+    }
+}
+
+// The member-order position of every pixel.
+__global__ __launch_bounds__(kThr) void pos_kernel(int64_t BN, const int *members, int *pos) {
+    const int64_t i = (int64_t)blockIdx.x * kThr + threadIdx.x;
+    if (i < BN) pos[members[i]] = (int)i;
 }
 
 // The Jacobi diagonal's terms moved onto pixels (tv_iso.cpp:833-870, in the
@@ -930,6 +1060,8 @@ struct Deriv {
     int8_t *nsg = nullptr;
     int *parent = nullptr, *isroot = nullptr, *rank = nullptr, *label = nullptr, *iota = nullptr;
     int *skey = nullptr, *members = nullptr, *rstart = nullptr, *rbase = nullptr, *cnt = nullptr;
+    int *pos = nullptr;     // iso: each pixel's position in the member order
+    double *vm = nullptr;   // iso: the apply's pixel values in the member order
     double *inv = nullptr, *dinv = nullptr, *cb = nullptr, *xr = nullptr, *rr = nullptr, *pv = nullptr,
            *qv = nullptr;
     double *head = nullptr, *tail = nullptr, *part = nullptr, *ppart = nullptr, *gsw_img = nullptr,
@@ -1010,6 +1142,8 @@ int reserve(Deriv &d) {
     if (sp.iso) {
         FPR_A(d.gz, 8 * (size_t)G.k * (size_t)BN);
         FPR_A(d.hbuf, 8 * (size_t)G.k * (size_t)BN);
+        FPR_A(d.vm, 8 * (size_t)BN);
+        FPR_A(d.pos, 4 * (size_t)BN);
         FPR_A(d.dinv, 8 * (size_t)BN);
         FPR_A(d.rr, 8 * (size_t)BN);
         FPR_A(d.pv, 8 * (size_t)BN);
@@ -1129,6 +1263,7 @@ int build(Deriv &d, hipStream_t s) {
     if (hipcub::DeviceRadixSort::SortPairs(d.cub, b2, d.label, d.skey, d.iota, d.members, (int)BN, 0, bits, s))
         return FP_EDEVICE;
     hipLaunchKernelGGL(rstart_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.skey, d.rstart);
+    if (sp.iso) hipLaunchKernelGGL(pos_kernel, dim3(gpx), dim3(kThr), 0, s, BN, d.members, d.pos);
     hipLaunchKernelGGL(inv_kernel, dim3(grid_of(d.nc)), dim3(kThr), 0, s, d.nc, d.rstart, d.inv);
     // region blocks per image
     std::vector<RBlock> &blocks = d.hblocks;
@@ -1186,14 +1321,29 @@ int run_cg(Deriv &d, hipStream_t s, double tol, int max_iter) {
     for (int done = 0; done < max_iter;) {
         for (int j = 0; j < kCheck && done < max_iter; ++j, ++done) {
             CgState *c0 = d.cg[d.cur], *c1 = d.cg[d.cur ^ 1];
-            if (d.sp.tsize == 4)
-                hipLaunchKernelGGL(hgv_kernel<float>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
-                                   d.label, d.pv, (const float *)d.sp.sw, d.sp.sw_img, d.hbuf);
-            else
-                hipLaunchKernelGGL(hgv_kernel<double>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
-                                   d.label, d.pv, (const double *)d.sp.sw, d.sp.sw_img, d.hbuf);
-            hipLaunchKernelGGL(rsum_kernel<FApply>, dim3(d.nch), dim3(kThr), 0, s, FApply{G, d.pv, d.hbuf, d.di, c0},
-                               d.skey, d.members, d.rstart, G.BN, d.qv, d.head, d.tail);
+            if (G.k == 2) {
+                const int ntx = (int)((G.L[2] + kApplyCols - 1) / kApplyCols);
+                const int nty = (int)((G.L[1] + kApplyRows - 1) / kApplyRows);
+                const int ablk = (int)(((int64_t)B * ntx * nty + 3) / 4);
+                if (d.sp.tsize == 4)
+                    hipLaunchKernelGGL(apply2d_kernel<float>, dim3(ablk), dim3(256), 0, s, G, d.di, c0, d.gz, d.flat,
+                                       d.label, d.pos, d.pv, (const float *)d.sp.sw, d.sp.sw_img, ntx, nty, B, d.vm);
+                else
+                    hipLaunchKernelGGL(apply2d_kernel<double>, dim3(ablk), dim3(256), 0, s, G, d.di, c0, d.gz,
+                                       d.flat, d.label, d.pos, d.pv, (const double *)d.sp.sw, d.sp.sw_img, ntx, nty,
+                                       B, d.vm);
+            } else {
+                if (d.sp.tsize == 4)
+                    hipLaunchKernelGGL(hgv_kernel<float>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
+                                       d.label, d.pv, (const float *)d.sp.sw, d.sp.sw_img, d.hbuf);
+                else
+                    hipLaunchKernelGGL(hgv_kernel<double>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
+                                       d.label, d.pv, (const double *)d.sp.sw, d.sp.sw_img, d.hbuf);
+                hipLaunchKernelGGL(div_kernel, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.label, d.pos, d.pv,
+                                   d.hbuf, d.vm);
+            }
+            hipLaunchKernelGGL((rsum_kernel<FPos, true>), dim3(d.nch), dim3(kThr), 0, s, FPos{d.vm}, d.skey,
+                               d.members, d.rstart, G.BN, d.qv, d.head, d.tail);
             hipLaunchKernelGGL(cg_pq_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, c0, d.rstart, d.qv, d.head, d.tail,
                                d.pv, d.part);
             hipLaunchKernelGGL(cg_update_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.di, c0, d.pv, d.qv, d.dinv,

@@ -54,7 +54,8 @@ __device__ void uf_unite(int *par, int a, int b) {
 // consecutive positions from LDS in order; rows of kRsPer + 1 entries keep
 // the walk's LDS accesses on distinct banks. The sums are the same, term for
 // term and in the same order, as a walk over global memory.
-template <class F>
+// BYPOS: f takes the member's position (values already in member order).
+template <class F, bool BYPOS = false>
 __global__ __launch_bounds__(kRsThr) void rsum_kernel(F f, const int *skey, const int *members, const int *rstart,
                                                     int64_t M, double *out, double *head, double *tail) {
     constexpr int kRow = kRsPer + 1;
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kRsThr) void rsum_kernel(F f, const int *skey, cons
         const int C = skey[base + i];
         const int j = (i / kRsPer) * kRow + i % kRsPer;
         lk[j] = C;
-        lv[j] = f(members[base + i], C);
+        lv[j] = BYPOS ? f((int)(base + i), C) : f(members[base + i], C);
     }
     __syncthreads();
     const int64_t m0 = base + (int64_t)t * kRsPer;
