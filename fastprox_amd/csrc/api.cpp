@@ -597,6 +597,7 @@ fp_status create_impl(const fp_op *op, const fp_array *like, const fp_options *o
     if (fp_status s = check_image(*op, x.shape + (x.ndim - op->k))) return s;
     w->kc = fpr::default_kernel(op->k, (int)w->tsize, w->img, op->iso, !op->spatial.empty(), B);
     if (w->kc.stream && !fpr::stream_fits((int)w->tsize, w->N)) w->kc = generic_kernel();
+    if (w->kc.s3t() && !fpr::stream_fits((int)w->tsize, w->N)) w->kc.TT = 1;  // the one-iteration 3D kernel
     w->method = default_method(*op, false);
     if (w->method == FPR_METHOD_ADBPG) {
         if (fp_status s = build_ad(*w)) return s;
@@ -1948,7 +1949,7 @@ FP_API fp_status fpr_workspace_set_kernel(fp_workspace *ws, int generic, int tt,
         fpr::KernelChoice kc;
         if (fp_status st = kernel_choice(generic, tt, ry, ws->k, (int)ws->tsize, !ws->op->spatial.empty(), kc))
             return st;
-        if (kc.stream && !fpr::stream_fits((int)ws->tsize, ws->N))
+        if ((kc.stream || kc.s3t()) && !fpr::stream_fits((int)ws->tsize, ws->N))
             return fail(FP_EINVAL, "images too large for the streaming kernel");
         ws->kc = kc;
         if (ws->numel == 0) return FP_OK;
@@ -2053,6 +2054,7 @@ FP_API fp_status fpr_batch_create(const fp_op *op, fp_dtype dtype, int64_t count
         }
         b->kc = fpr::default_kernel(k, (int)b->tsize, b->rep, op->iso, b->weighted != 0, count);
         if (b->kc.stream && !fpr::stream_fits((int)b->tsize, b->maxN)) b->kc = generic_kernel();
+        if (b->kc.s3t() && !fpr::stream_fits((int)b->tsize, b->maxN)) b->kc.TT = 1;
         b->ptrs.assign((size_t)count, fpr::ImagePtrs{});
         b->gammas.assign((size_t)count, 0.0);
         b->gaps.assign((size_t)count, 0.0);
@@ -2176,7 +2178,7 @@ FP_API fp_status fpr_batch_set_kernel(fpr_batch *b, int generic, int tt, int ry)
         if (!b) return FP_EINVAL;
         fpr::KernelChoice kc;
         if (fp_status st = kernel_choice(generic, tt, ry, b->op->k, (int)b->tsize, b->weighted != 0, kc)) return st;
-        if (kc.stream && !fpr::stream_fits((int)b->tsize, b->maxN))
+        if ((kc.stream || kc.s3t()) && !fpr::stream_fits((int)b->tsize, b->maxN))
             return fail(FP_EINVAL, "images too large for the streaming kernel");
         b->kc = kc;
         b->valid = false;

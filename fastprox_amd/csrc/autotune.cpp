@@ -81,7 +81,7 @@ const Entry *lookup(int k, int tsize, const int64_t *img, bool iso, bool sw, int
     return last;
 }
 
-bool entry_kernel(const Entry &e, int tsize, KernelChoice &kc) {
+bool entry_kernel(const Entry &e, int tsize, int64_t n, KernelChoice &kc) {
     kc = KernelChoice{};
     switch (e.kind) {
         case 0:
@@ -101,7 +101,7 @@ bool entry_kernel(const Entry &e, int tsize, KernelChoice &kc) {
             kc.s3 = true;
             kc.TT = e.TT < 1 ? 1 : e.TT;
             kc.RY = e.RY;
-            return e.k == 3 && kernel_available_s3(e.RY, kc.TT, tsize, e.sw);
+            return e.k == 3 && kernel_available_s3(e.RY, kc.TT, tsize, e.sw, n);
     }
     return false;
 }
@@ -126,7 +126,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
                 int tt = 1;
                 for (const char *q = c; *q; ++q)
                     if (*q == ',') tt = std::atoi(q + 1);
-                if (kernel_available_s3(ry, tt, tsize, sw)) {
+                if (kernel_available_s3(ry, tt, tsize, sw, img[0] * img[1] * img[2])) {
                     kc.s3 = true;
                     kc.RY = ry;
                     kc.TT = tt;
@@ -135,7 +135,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
             }
         }
         if (const Entry *t = lookup(k, tsize, img, iso, sw, B))
-            if (entry_kernel(*t, tsize, kc)) return kc;
+            if (entry_kernel(*t, tsize, img[0] * img[1] * img[2], kc)) return kc;
         kc = KernelChoice{};
         kc.stream = false;
         kc.s3 = true;
@@ -168,7 +168,7 @@ KernelChoice default_kernel(int k, int tsize, const int64_t *img, bool iso, bool
         }
     }
     if (const Entry *t = lookup(k, tsize, img, iso, sw, B))
-        if (entry_kernel(*t, tsize, kc)) return kc;
+        if (entry_kernel(*t, tsize, img[0] * img[1] * img[2], kc)) return kc;
     kc = KernelChoice{};
     kc.stream = true;
     kc.TT = 4;

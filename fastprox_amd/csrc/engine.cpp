@@ -185,7 +185,7 @@ int Engine::setup(int tsize, const Problem &P, const KernelChoice &kc, const std
     total_tiles_ = tiles;
     tiles_cap_ = tiles;
     // the streaming 2D kernel's 32-bit plane offsets
-    if (kc.stream && !kc.generic && !stream_fits(tsize, maxN_)) return hipErrorInvalidValue;
+    if (((kc.stream && !kc.generic) || kc.s3t()) && !stream_fits(tsize, maxN_)) return hipErrorInvalidValue;
     hipError_t e;
     if ((e = fpr::dev_alloc(&arena_, (size_t)std::max<int64_t>(arena, 1) * (size_t)tsize))) return e;
     if ((e = fpr::dev_alloc((void **)&desc_, sizeof(ImgDesc) * (size_t)std::max(B, 1)))) return e;

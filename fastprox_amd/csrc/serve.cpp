@@ -590,6 +590,11 @@ int Server::submit(ServeRequest r, int64_t *id) {
     if (!r.x) return FP_EINVAL;
     if (!(r.gamma >= 0.0) || !std::isfinite(r.gamma) || !(r.tol >= 0.0) || !std::isfinite(r.tol) || r.max_iter < 1)
         return FP_EPARAM;
+    {
+        // the streaming kernels address each plane with 32-bit byte offsets
+        const KernelChoice &kc = pools_[c]->kernel();
+        if ((kc.stream || kc.s3t()) && !stream_fits(pools_[c]->tsize(), r.L[0] * r.L[1] * r.L[2])) return FP_ESHAPE;
+    }
     if (!pools_[c]->fits_ever(r)) return FP_ENOMEM;
     // the admission's gather waits for the work on the caller's stream
     // before this point (the work that produces x)

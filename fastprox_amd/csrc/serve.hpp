@@ -102,6 +102,7 @@ public:
     // Frees the oldest kept dual other than keep's; false if there is none.
     bool evict_one_dual(int64_t keep = -1);
     const KernelChoice &kernel() const { return kc_; }
+    int tsize() const { return tsize_; }
     // Launch pairs per chunk: a finished request is seen when its chunk's
     // snapshot is read, so shorter chunks cut latency (4 against 8: served
     // p50 0.67 against 0.78 ms, the same p99; 2 lengthens the p99).

@@ -2019,14 +2019,19 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
         const int z0 = __builtin_amdgcn_readfirstlane(tl.tz) * SEGZ, z1 = min(z0 + SEGZ, L0);
         const bool xin = x >= 0 && x < W, yin = y >= 0 && y < H, inxy = xin && yin;
         const bool own = inxy && lane >= TT && lane < 64 - TT && wv >= TT && wv < NW - TT;
-        const int64_t oxy = inxy ? (int64_t)y * W + x : 0;
+        const uint32_t oxy = inxy ? (uint32_t)(y * W + x) : 0u;
         double *part = A.part + (int64_t)(D.tile_base + __builtin_amdgcn_readfirstlane(tl.pad_)) * kSlots;
+        // Plane bases (uniform, in scalar registers); every access is a base
+        // plus a 32-bit offset (N sizeof(T) < 2^32: kernel_available_s3).
+        const auto zo = uniform(gout<T>(D.z));
+        const gptr<T> qo[3] = {uniform(gout<T>(D.q)), uniform(gout<T>(D.q) + D.q_cstride),
+                               uniform(gout<T>(D.q) + 2 * D.q_cstride)};
         if (mode == kOutputNaN) {
             if (own)
                 for (int t = z0; t < z1; ++t) {
-                    const int64_t o = t * SZ + oxy;
-                    gout<T>(D.z)[o] = (T)NAN;
-                    for (int a = 0; a < 3; ++a) gout<T>(D.q)[a * D.q_cstride + o] = (T)NAN;
+                    const uint32_t o = (uint32_t)(t * SZ) + oxy;
+                    st32<T>(zo, o, (T)NAN);
+                    for (int a = 0; a < 3; ++a) st32<T>(qo[a], o, (T)NAN);
                 }
             continue;
         }
@@ -2049,10 +2054,14 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
         const T gw2 = (wd2 != 0.0 && inxy && x + 1 < W) ? w2 : (T)0;
         const T tw1 = tau * gw1, tw2 = tau * gw2;
         const bool act0 = wd0 != 0.0;
-        const auto dp = gin<T>(D.d);
-        const T *U = A.arena + D.st_off + (int64_t)S.cur * 6 * N;
-        const T *V = S.vsrc_u ? U : U + 3 * N;
-        T *Uo = A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 6 * N;
+        const auto dp = uniform(gin<T>(D.d));
+        const gptr<const T> U = uniform((gptr<const T>)(A.arena + D.st_off + (int64_t)S.cur * 6 * N));
+        const gptr<const T> Ub[3] = {U, uniform(U + N), uniform(U + 2 * N)};
+        const gptr<const T> V = S.vsrc_u ? U : uniform(U + 3 * N);
+        const gptr<const T> Vb[3] = {V, uniform(V + N), uniform(V + 2 * N)};
+        const gptr<T> Uo = uniform((gptr<T>)(A.arena + D.st_off + (int64_t)(S.cur ^ 1) * 6 * N));
+        const gptr<T> Ob[6] = {Uo, uniform(Uo + N), uniform(Uo + 2 * N), uniform(Uo + 3 * N), uniform(Uo + 4 * N),
+                               uniform(Uo + 5 * N)};
         // the axis-0 weight of a slice: 0 outside the image or at its last slice
         auto gw0_at = [&](int t) -> T { return (act0 && inxy && t >= 0 && t + 1 < L0) ? w0 : (T)0; };
         auto in_slice = [&](int t) { return inxy && t >= 0 && t < L0; };
@@ -2081,12 +2090,12 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
         const int s_last_load = z1 - 1 + TT;
         auto load_slice = [&](int s) {
             const bool ok = in_slice(s) && s <= s_last_load;
-            const int64_t o = ok ? s * SZ + oxy : 0;
-            nd = ok ? dp[o] : (T)0;
+            const uint32_t o = ok ? (uint32_t)(s * SZ) + oxy : 0u;
+            nd = ok ? ld32<T>(dp, o) : (T)0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                nv[a] = (ok && !fresh) ? V[a * N + o] : (T)0;
-                nu[a] = (ok && !fresh) ? U[a * N + o] : (T)0;
+                nv[a] = (ok && !fresh) ? ld32<T>(Vb[a], o) : (T)0;
+                nu[a] = (ok && !fresh) ? ld32<T>(Ub[a], o) : (T)0;
             }
         };
         const int sbeg = z0 - TT;
@@ -2195,11 +2204,11 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
             {
                 const int t = s - 2 * TT;
                 if (cnt > 0 && own && t >= z0 && t < z1) {
-                    const int64_t o = t * SZ + oxy;
+                    const uint32_t o = (uint32_t)(t * SZ) + oxy;
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        Uo[a * N + o] = cu[TT][a];
-                        Uo[(3 + a) * N + o] = cv[TT][a];
+                        st32<T>(Ob[a], o, cu[TT][a]);
+                        st32<T>(Ob[3 + a], o, cv[TT][a]);
                     }
                 }
                 if (out_mode) {
@@ -2207,19 +2216,18 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
                     const T u1u = wv > 0 ? X.u1[pp][wv - 1][TT][lane] : (T)0;
                     const double ul2 = from_left((double)bu[TT][2]);
                     if (own && to >= z0 && to < z1) {
-                        const int64_t o = to * SZ + oxy;
+                        const uint32_t o = (uint32_t)(to * SZ) + oxy;
                         const double zz = (((double)dq[2 * TT + 1] + wd0 * ((double)bu[TT][0] - (double)au[TT][0])) +
                                            wd1 * ((double)bu[TT][1] - (double)u1u)) +
                                           wd2 * ((double)bu[TT][2] - ul2);
-                        gout<T>(D.z)[o] = (T)ldexp(zz, e);
+                        st32<T>(zo, o, (T)ldexp(zz, e));
                         if constexpr (ISO) {
 #pragma unroll
-                            for (int a = 0; a < 3; ++a)
-                                gout<T>(D.q)[a * D.q_cstride + o] = (T)ldexp((double)bu[TT][a] / S.wmax, e);
+                            for (int a = 0; a < 3; ++a) st32<T>(qo[a], o, (T)ldexp((double)bu[TT][a] / S.wmax, e));
                         } else {
-                            gout<T>(D.q)[o] = (T)(wd0 * ((double)au[TT][0] - (double)bu[TT][0]));
-                            gout<T>(D.q)[D.q_cstride + o] = (T)(wd1 * ((double)u1u - (double)bu[TT][1]));
-                            gout<T>(D.q)[2 * D.q_cstride + o] = (T)(wd2 * (ul2 - (double)bu[TT][2]));
+                            st32<T>(qo[0], o, (T)(wd0 * ((double)au[TT][0] - (double)bu[TT][0])));
+                            st32<T>(qo[1], o, (T)(wd1 * ((double)u1u - (double)bu[TT][1])));
+                            st32<T>(qo[2], o, (T)(wd2 * (ul2 - (double)bu[TT][2])));
                         }
                     }
                 }
@@ -2547,9 +2555,9 @@ void launch_step(const KernelChoice &kc, bool iso, bool sw, const StepArgs<T> &A
     }
 }
 
-bool kernel_available_s3(int RY, int TT, int dtype_bytes, bool sw) {
+bool kernel_available_s3(int RY, int TT, int dtype_bytes, bool sw, int64_t maxN) {
     if (RY < 1 || RY > 4096) return false;
-    return TT == 1 || ((TT == 2 || TT == 3) && dtype_bytes == 4 && !sw);
+    return TT == 1 || ((TT == 2 || TT == 3) && dtype_bytes == 4 && !sw && stream_fits(dtype_bytes, maxN));
 }
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY) {

@@ -63,8 +63,9 @@ struct KernelChoice {
 
 bool kernel_available(int dtype_bytes, bool stream, int TT, int RY);
 // The 3D streaming kernel with TT iterations per launch (TT 1: any dtype and
-// weights; TT 2 or 3: float32 without pixel weights) and slabs of RY slices.
-bool kernel_available_s3(int RY, int TT = 1, int dtype_bytes = 8, bool sw = false);
+// weights; TT 2 or 3: float32 without pixel weights, volumes of at most
+// kStreamMaxPlaneBytes per plane) and slabs of RY slices.
+bool kernel_available_s3(int RY, int TT = 1, int dtype_bytes = 8, bool sw = false, int64_t maxN = 0);
 
 // The prologue's per-block partials: 4 doubles per block, prologue_blocks(maxN) blocks per image.
 int prologue_blocks(int64_t maxN);
