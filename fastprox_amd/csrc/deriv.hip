@@ -1141,7 +1141,7 @@ int reserve(Deriv &d) {
     FPR_A(d.nreg, sizeof(int64_t) * (size_t)B);
     if (sp.iso) {
         FPR_A(d.gz, 8 * (size_t)G.k * (size_t)BN);
-        FPR_A(d.hbuf, 8 * (size_t)G.k * (size_t)BN);
+        if (G.k == 3) FPR_A(d.hbuf, 8 * (size_t)G.k * (size_t)BN);  // 2D forms h inside apply2d_kernel
         FPR_A(d.vm, 8 * (size_t)BN);
         FPR_A(d.pos, 4 * (size_t)BN);
         FPR_A(d.dinv, 8 * (size_t)BN);
