@@ -476,7 +476,7 @@ template <class T>
 __global__ __launch_bounds__(256) void apply2d_kernel(Geo G, const DImg *di, const CgState *cg, const double *gz,
                                                       const uint8_t *flat, const int *label, const int *pos,
                                                       const double *pv, const T *sw, int64_t sw_img, int ntx,
-                                                      int nty, int B, double *vm) {
+                                                      int nty, int B, double *vm, double *apart) {
     const int lane = threadIdx.x & 63;
     const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int64_t per = (int64_t)ntx * nty;
@@ -513,6 +513,7 @@ __global__ __launch_bounds__(256) void apply2d_kernel(Geo G, const DImg *di, con
     double pxv = 0.0, pg0 = 0.0, pg1 = 0.0;
     bool pfl = true;
     double ph0 = 0.0;  // h_0 of the row before that (the row above the value's row)
+    double pq = 0.0;   // sum over the strip's pixels of pv_C(p) x value(p): its share of p . q
     for (int yr = y0 - 1; yr <= yend; ++yr) {
         // the next row's values and the row after's label, in flight during this step
         const bool in1 = row_in(yr + 1), in2 = row_in(yr + 2) && yr + 2 <= yend;
@@ -545,6 +546,7 @@ __global__ __launch_bounds__(256) void apply2d_kernel(Geo G, const DImg *di, con
                 if (w0 != 0.0) qv += y > 0 ? w0 * (h0 - ph0) : w0 * h0;
                 if (w1 != 0.0) qv += x > 0 ? w1 * (h1 - h1l) : w1 * h1;
                 vm[pos[base + (int64_t)y * W + x]] = -qv;
+                pq += pxv * -qv;
             }
             ph0 = h0;
         }
@@ -558,6 +560,9 @@ __global__ __launch_bounds__(256) void apply2d_kernel(Geo G, const DImg *di, con
         g1 = g11;
         lab1 = lab2;
     }
+    // the strip's share of p . q, for the image's fixed-order sum over its strips
+    for (int o = 32; o > 0; o >>= 1) pq += __shfl_xor(pq, o, 64);
+    if (lane == 0) apart[tile] = pq;
 }
 
 // The member-order position of every pixel.
@@ -714,22 +719,44 @@ __global__ __launch_bounds__(kThr) void cg_pq_kernel(const RBlock *rb, const CgS
     if (threadIdx.x == 0) part[3 * blockIdx.x] = t;
 }
 
-// alpha = rz / pq; x += alpha pv; r -= alpha qv; partials of r^2 dinv, r^2 inv.
+// p . q of image b: the pq pass's region-block partials, or (apt > 0) the
+// fused 2D apply's, summed per image by pqsum_kernel into pqv.
+__device__ __forceinline__ double image_pq(const double *part, const DImg &D, const double *pqv, int apt, int b) {
+    return apt == 0 ? image_total(part, D.bbase, D.bcount, 0) : pqv[b];
+}
+
+// The fused 2D apply's strip partials of p . q summed per image, one wave per
+// image: each lane its strips in order, then a fixed butterfly.
+__global__ __launch_bounds__(64) void pqsum_kernel(const CgState *cg, const double *apart, int apt, double *pqv) {
+    const int b = blockIdx.x, lane = threadIdx.x;
+    if (!cg[b].active) return;
+    double t = 0.0;
+    for (int j = lane; j < apt; j += 64) t += apart[(int64_t)b * apt + j];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if (lane == 0) pqv[b] = t;
+}
+
+// alpha = rz / pq; x += alpha pv; r -= alpha qv; partials of r^2 dinv, r^2
+// inv. With apt > 0 (no pq pass) the region sums of the apply are finalised
+// here.
 __global__ __launch_bounds__(kThr) void cg_update_kernel(const RBlock *rb, const DImg *di, const CgState *cg,
                                                          const double *pv, const double *qv, const double *dinv,
-                                                         const double *inv, double *x, double *r, double *part) {
+                                                         const double *inv, double *x, double *r, double *part,
+                                                         const double *pqv, int apt, const int *rstart,
+                                                         const double *head, const double *tail) {
     __shared__ double sh[kThr];
     const RBlock B = rb[blockIdx.x];
     const CgState S = cg[B.img];
     if (!S.active) return;
     const DImg &D = di[B.img];
-    const double pq = image_total(part, D.bbase, D.bcount, 0);
+    const double pq = image_pq(part, D, pqv, apt, B.img);
     if (!(pq > 0.0)) return;
     const double alpha = S.rz / pq;
     double t0 = 0.0, t1 = 0.0;
     for (int C = B.c0 + threadIdx.x; C < B.c1; C += kThr) {
+        const double q = apt ? region_value(C, qv, rstart, head, tail) : qv[C];
         x[C] += alpha * pv[C];
-        r[C] -= alpha * qv[C];
+        r[C] -= alpha * q;
         t0 += r[C] * r[C] * dinv[C];
         t1 += r[C] * r[C] * inv[C];
     }
@@ -745,7 +772,7 @@ __global__ __launch_bounds__(kThr) void cg_update_kernel(const RBlock *rb, const
 // next state (tv_iso.cpp:935-941; a non-positive pq stops the image).
 __global__ __launch_bounds__(kThr) void cg_dir_kernel(const RBlock *rb, const DImg *di, const CgState *cur,
                                                       CgState *nxt, const double *r, const double *dinv, double *pv,
-                                                      const double *part) {
+                                                      const double *part, const double *pqv, int apt) {
     const RBlock B = rb[blockIdx.x];
     const DImg &D = di[B.img];
     const bool lead = (int)blockIdx.x == D.bbase;
@@ -754,7 +781,7 @@ __global__ __launch_bounds__(kThr) void cg_dir_kernel(const RBlock *rb, const DI
         if (lead && threadIdx.x == 0) nxt[B.img] = S;
         return;
     }
-    const double pq = image_total(part, D.bbase, D.bcount, 0);
+    const double pq = image_pq(part, D, pqv, apt, B.img);
     if (!(pq > 0.0)) {
         if (lead && threadIdx.x == 0) {
             S.active = 0;
@@ -1062,6 +1089,9 @@ struct Deriv {
     int *skey = nullptr, *members = nullptr, *rstart = nullptr, *rbase = nullptr, *cnt = nullptr;
     int *pos = nullptr;     // iso: each pixel's position in the member order
     double *vm = nullptr;   // iso: the apply's pixel values in the member order
+    double *apart = nullptr;  // iso 2D: the fused apply's p . q partials (B x strips)
+    double *pqv = nullptr;    // iso 2D: p . q per image
+    int apt = 0;              // iso 2D: strips per image (0: the pq pass)
     double *inv = nullptr, *dinv = nullptr, *cb = nullptr, *xr = nullptr, *rr = nullptr, *pv = nullptr,
            *qv = nullptr;
     double *head = nullptr, *tail = nullptr, *part = nullptr, *ppart = nullptr, *gsw_img = nullptr,
@@ -1143,6 +1173,12 @@ int reserve(Deriv &d) {
         FPR_A(d.gz, 8 * (size_t)G.k * (size_t)BN);
         if (G.k == 3) FPR_A(d.hbuf, 8 * (size_t)G.k * (size_t)BN);  // 2D forms h inside apply2d_kernel
         FPR_A(d.vm, 8 * (size_t)BN);
+        if (G.k == 2) {
+            const int64_t ntx = (G.L[2] + kApplyCols - 1) / kApplyCols, nty = (G.L[1] + kApplyRows - 1) / kApplyRows;
+            d.apt = (int)(ntx * nty);
+            FPR_A(d.apart, 8 * (size_t)B * (size_t)d.apt);
+            FPR_A(d.pqv, 8 * (size_t)B);
+        }
         FPR_A(d.pos, 4 * (size_t)BN);
         FPR_A(d.dinv, 8 * (size_t)BN);
         FPR_A(d.rr, 8 * (size_t)BN);
@@ -1327,11 +1363,12 @@ int run_cg(Deriv &d, hipStream_t s, double tol, int max_iter) {
                 const int ablk = (int)(((int64_t)B * ntx * nty + 3) / 4);
                 if (d.sp.tsize == 4)
                     hipLaunchKernelGGL(apply2d_kernel<float>, dim3(ablk), dim3(256), 0, s, G, d.di, c0, d.gz, d.flat,
-                                       d.label, d.pos, d.pv, (const float *)d.sp.sw, d.sp.sw_img, ntx, nty, B, d.vm);
+                                       d.label, d.pos, d.pv, (const float *)d.sp.sw, d.sp.sw_img, ntx, nty, B, d.vm,
+                                       d.apart);
                 else
                     hipLaunchKernelGGL(apply2d_kernel<double>, dim3(ablk), dim3(256), 0, s, G, d.di, c0, d.gz,
                                        d.flat, d.label, d.pos, d.pv, (const double *)d.sp.sw, d.sp.sw_img, ntx, nty,
-                                       B, d.vm);
+                                       B, d.vm, d.apart);
             } else {
                 if (d.sp.tsize == 4)
                     hipLaunchKernelGGL(hgv_kernel<float>, dim3(gpx), dim3(kThr), 0, s, G, d.di, c0, d.gz, d.flat,
@@ -1344,12 +1381,16 @@ int run_cg(Deriv &d, hipStream_t s, double tol, int max_iter) {
             }
             hipLaunchKernelGGL((rsum_kernel<FPos, true>), dim3(d.nch), dim3(kThr), 0, s, FPos{d.vm}, d.skey,
                                d.members, d.rstart, G.BN, d.qv, d.head, d.tail);
-            hipLaunchKernelGGL(cg_pq_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, c0, d.rstart, d.qv, d.head, d.tail,
-                               d.pv, d.part);
+            // 2D: p . q came with the apply (no pq pass; the update finalises the region sums)
+            if (d.apt == 0)
+                hipLaunchKernelGGL(cg_pq_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, c0, d.rstart, d.qv, d.head,
+                                   d.tail, d.pv, d.part);
+            else
+                hipLaunchKernelGGL(pqsum_kernel, dim3(B), dim3(64), 0, s, c0, d.apart, d.apt, d.pqv);
             hipLaunchKernelGGL(cg_update_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.di, c0, d.pv, d.qv, d.dinv,
-                               d.inv, d.xr, d.rr, d.part);
+                               d.inv, d.xr, d.rr, d.part, d.pqv, d.apt, d.rstart, d.head, d.tail);
             hipLaunchKernelGGL(cg_dir_kernel, dim3(d.nrb), dim3(kThr), 0, s, d.rb, d.di, c0, c1, d.rr, d.dinv, d.pv,
-                               d.part);
+                               d.part, d.pqv, d.apt);
             d.cur ^= 1;
         }
         hipLaunchKernelGGL(count_active_kernel, dim3(1), dim3(256), 0, s, B, d.cg[d.cur], d.cnt);
