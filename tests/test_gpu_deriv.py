@@ -457,3 +457,25 @@ def test_c3_vjp_against_central_differences_and_the_reference(cuda):
         assert ggd == pytest.approx(rgg, rel=1e-7)
     finally:
         refbind.Op.free_ws(rws)
+
+
+@pytest.mark.parametrize("shape", [(1, 70), (70, 1), (2, 65), (63, 3), (5, 125), (130, 9)])
+def test_iso_vjp_thin_and_strip_edge_shapes(cuda, shape):
+    """The fused 2D apply walks 62-column strips down 8-row segments: images a
+    pixel wide or tall (one active axis), two or three pixels across, and
+    widths just past a strip (63, 65, 125, 130) give the oracle's VJP, with
+    per-image steps and a tight CG tolerance."""
+    rng = np.random.default_rng(sum(shape))
+    x = image((2,) + shape, 40 + shape[0], scale=0.3)
+    steps = np.array([0.12, 0.3])
+    w = [1.0, 0.7]
+    op, ws, xt, z, dual, gaps = _kept(x, 0.0, w, None, 1e-10, steps=steps)
+    ws.set_deriv_tol(1e-12, 100000)
+    g = torch.tensor(rng.normal(size=x.shape), device="cuda")
+    gx = torch.empty_like(g)
+    ws.vjp(g, gx)
+    zc, dc = z.cpu().numpy(), dual.cpu().numpy()
+    for b in range(2):  # values of order 1: no power-of-two rescaling (scale exponent 0)
+        r = oracle.iso_deriv(x[b], zc[b], dc[:, b], gaps[b], 0, steps[b], w, g=g[b].cpu().numpy(),
+                             deriv_tol=1e-12, deriv_max_iter=100000)
+        assert _rel(gx[b].cpu().numpy(), r.out) <= 1e-8, (shape, b)
