@@ -102,7 +102,6 @@ SlotPool::~SlotPool() {
     for (hipEvent_t e : stage_ev_)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
-    if (copy_ev_) (void)hipEventDestroy(copy_ev_);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -148,7 +147,6 @@ int SlotPool::setup(const Problem &P, const KernelChoice &kc, int tsize, int slo
     events_.resize(4);
     for (auto &ev : events_)
         if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming))) return e;
-    if ((e = hipEventCreateWithFlags(&copy_ev_, hipEventDisableTiming))) return e;
     return hipStreamSynchronize(stream);
 }
 
@@ -223,7 +221,9 @@ int SlotPool::reserve(const ServeRequest &r, int *slot, ImgDesc *Dout, int64_t *
     char *base = (char *)pool_;
     const size_t ts = (size_t)tsize_;
     D.d = base + (size_t)off_main * ts;
-    D.z = base + (size_t)(off_main + N) * ts;
+    // the output launch writes z straight into the caller's buffer; the
+    // pool's z block is the target only when the caller asked for none
+    D.z = r.z ? r.z : base + (size_t)(off_main + N) * ts;
     D.st_off = off_main + 2 * N;
     D.q = base + (size_t)off_dual * ts;
     D.sw = nullptr;
@@ -500,29 +500,17 @@ bool SlotPool::oldest_done() const {
     return !flight_.empty() && hipEventQuery(flight_.front().ev) != hipErrorNotReady;
 }
 
-int SlotPool::copy_out(const std::vector<int> &done, hipEvent_t chunk) {
-    // One pass per distinct caller stream: it waits for the chunk that wrote
-    // the outputs, copies them, and this pool's stream waits for the copies
-    // before any later work can reuse the memory they are read from.
+int SlotPool::order_out(const std::vector<int> &done, hipEvent_t chunk) {
+    // The outputs are in the callers' buffers once the chunk has run: each
+    // distinct caller stream waits for its event once.
     std::vector<hipStream_t> streams;
     for (int i : done) {
         const Slot &S = slots_[(size_t)i];
         if (S.req.z && std::find(streams.begin(), streams.end(), S.req.caller) == streams.end())
             streams.push_back(S.req.caller);
     }
-    for (hipStream_t cs : streams) {
+    for (hipStream_t cs : streams)
         if (hipError_t e = hipStreamWaitEvent(cs, chunk, 0)) return e;
-        for (int i : done) {
-            const Slot &S = slots_[(size_t)i];
-            if (!S.req.z || S.req.caller != cs) continue;
-            const char *zsrc = (const char *)pool_ + (size_t)(S.off_main + S.N) * (size_t)tsize_;
-            if (hipError_t e = hipMemcpyAsync(S.req.z, zsrc, (size_t)S.N * (size_t)tsize_, hipMemcpyDeviceToDevice,
-                                              cs))
-                return e;
-        }
-        if (hipError_t e = hipEventRecord(copy_ev_, cs)) return e;
-        if (hipError_t e = hipStreamWaitEvent(stream, copy_ev_, 0)) return e;
-    }
     return 0;
 }
 
@@ -542,7 +530,7 @@ int SlotPool::retire(std::vector<ServeResult> &out, double (*clock)(const void *
         if (snap[i].mode != kDone) continue;
         done.push_back(i);
     }
-    if (int e = copy_out(done, f.ev)) return e;
+    if (int e = order_out(done, f.ev)) return e;
     for (int i : done)
         if (int e = finish_slot(i, snap[i], out, now)) return e;
     return 0;

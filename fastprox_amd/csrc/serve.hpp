@@ -127,10 +127,9 @@ private:
         uint64_t stamp;
     };
     int finish_slot(int i, const ImgState &S, std::vector<ServeResult> &out, double now);
-    // Copies the outputs of the finished slots to their callers' buffers on
-    // the callers' streams (after the chunk's event), and orders this
-    // pool's later work after those copies.
-    int copy_out(const std::vector<int> &done, hipEvent_t chunk);
+    // Orders the callers' streams of the finished slots after the chunk
+    // whose output launch wrote their z.
+    int order_out(const std::vector<int> &done, hipEvent_t chunk);
     int reserve(const ServeRequest &r, int *slot, ImgDesc *D, int64_t *offs);  // kFits or what is short
     void occupy(int i, const ServeRequest &r, const ImgDesc &D, const int64_t *offs, double now);
     void release_reserved(const ImgDesc &D, const int64_t *offs);
@@ -177,7 +176,6 @@ private:
     int64_t seq_ = 0;
     std::deque<Flight> flight_;
     std::vector<hipEvent_t> events_;
-    hipEvent_t copy_ev_ = nullptr;  // recorded on a caller's stream after its copies
     int p_ = 0;       // the list the next step reads
     int live_ = 0;
     int top_ = 0;     // 1 + the highest slot ever used

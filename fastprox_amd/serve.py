@@ -86,8 +86,9 @@ class Server:
         """Queues the image x (a 2D or 3D cuda tensor of the server's dtype,
         any strides); out: a contiguous cuda tensor for z, or None. stream
         (default: torch's current stream): x is read after the work queued
-        on it so far, and out is written before the work queued on it after
-        the poll() that returns the request. Returns the request id."""
+        on it so far, and out is written (in place, at any time until then)
+        before the work queued on it after the poll() that returns the
+        request. Returns the request id."""
         if x.dtype != self.dtype or not x.is_cuda or x.dim() not in (2, 3):
             raise ValueError("x must be a 2D or 3D cuda tensor of the server's dtype")
         r = _Request()

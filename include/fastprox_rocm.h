@@ -428,7 +428,10 @@ typedef struct {
     int64_t shape[3];         /* the first ndim entries */
     const void *x;            /* device, the server's dtype; read at admission */
     const int64_t *strides;   /* element strides of x, or NULL for C order */
-    void *z;                  /* device output (contiguous), or NULL */
+    void *z;                  /* device output (contiguous), or NULL; the
+                                 server writes it in place at any time between
+                                 the submit and the poll that returns the
+                                 result, so the caller leaves it alone then */
     double gamma;
     double tol;               /* stop at gap <= tol |x|^2 / 2 */
     int max_iter;
