@@ -119,7 +119,10 @@ int SlotPool::setup(const Problem &P, const KernelChoice &kc, int tsize, int slo
     mem_.reset(pool_elems);
     tiles_.reset(tile_cap);
     hipError_t e;
-    if ((e = hipStreamCreate(&stream))) return e;
+    // non-blocking: a caller on the null stream is ordered by the events
+    // alone (a blocking pool stream would make every submit's event and every
+    // retirement's wait on that stream serialise with all chunks in flight)
+    if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking))) return e;
     if ((e = fpr::dev_alloc(&pool_, (size_t)pool_elems * (size_t)tsize))) return e;
     if ((e = fpr::dev_alloc((void **)&desc_, sizeof(ImgDesc) * (size_t)slots))) return e;
     if ((e = fpr::dev_alloc((void **)&state_, sizeof(ImgState) * (size_t)slots))) return e;
