@@ -300,21 +300,31 @@ __device__ void epilogue_img(const ImgDesc *desc, ImgState *state, const double 
     const int cnt = S.cnt;
     const uint32_t gm = S.gap_mask, tm = S.test_mask;
     if (mode == kIterate) {
-        for (int j = 0; j <= cnt; ++j) {
-            const bool g = (gm >> j) & 1u, d = (tm >> j) & 1u;
-            if (!g && !d) continue;
-            double sg = 0.0, sd = 0.0;
-            for (int i = lane; i < D.ntiles; i += 64) {
-                const double *p = part + (int64_t)(D.tile_base + i) * kSlots + 2 * j;
-                if (g) sg += p[0];
-                if (d) sd += p[1];
+        // Every level at once (a tile's kSlots partials are contiguous), so
+        // the loads and the levels' butterflies overlap; each level's sum is
+        // still its tiles in list order per lane, then the fixed butterfly.
+        // Levels the masks do not ask for are summed and dropped.
+        double s[kSlots];
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) s[j] = 0.0;
+        for (int i = lane; i < D.ntiles; i += 64) {
+            const double2 *p = (const double2 *)(part + (int64_t)(D.tile_base + i) * kSlots);
+#pragma unroll
+            for (int j = 0; j <= kMaxTT; ++j) {
+                const double2 v = p[j];
+                s[2 * j] += v.x;
+                s[2 * j + 1] += v.y;
             }
-            sg = wave_sum(sg);
-            sd = wave_sum(sd);
-            if (lane == 0) {
-                gaps[j] = sg;
-                dots[j] = sd;
-            }
+        }
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) s[j] = wave_sum(s[j]);
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j <= kMaxTT; ++j)
+                if (j <= cnt) {
+                    gaps[j] = ((gm >> j) & 1u) ? s[2 * j] : 0.0;
+                    dots[j] = ((tm >> j) & 1u) ? s[2 * j + 1] : 0.0;
+                }
         }
     }
     int base = -1;
