@@ -105,14 +105,20 @@ __device__ __forceinline__ gptr<T> gout(void *p) {
 // Element o of a plane at a uniform base, through a 32-bit byte offset
 // (o * sizeof(T) < 2^32, which kStreamMaxPlaneBytes guarantees for the streaming
 // 2D kernel): the access takes the scalar-base form, with no 64-bit address
-// arithmetic per lane.
+// arithmetic per lane. The byte offset goes through an empty asm that keeps it
+// a 32-bit register value: a caller's `in ? o : 0` would otherwise become a
+// select of two 64-bit offsets, which the scalar-base form cannot take.
+__device__ __forceinline__ uint32_t opaque32(uint32_t v) {
+    asm("" : "+v"(v));
+    return v;
+}
 template <class T>
 __device__ __forceinline__ T ld32(gptr<const T> p, uint32_t o) {
-    return *(gptr<const T>)((gptr<const char>)p + o * (uint32_t)sizeof(T));
+    return *(gptr<const T>)((gptr<const char>)p + opaque32(o * (uint32_t)sizeof(T)));
 }
 template <class T>
 __device__ __forceinline__ void st32(gptr<T> p, uint32_t o, T v) {
-    *(gptr<T>)((gptr<char>)p + o * (uint32_t)sizeof(T)) = v;
+    *(gptr<T>)((gptr<char>)p + opaque32(o * (uint32_t)sizeof(T))) = v;
 }
 
 // A pointer every lane holds the same value of, moved to scalar registers:
