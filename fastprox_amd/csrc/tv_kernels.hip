@@ -928,7 +928,8 @@ __device__ __forceinline__ void tile2d(const StepArgs<T> &A, const Tile tl, cons
 
     const int e = S.e;
     const int cnt = S.cnt;
-    const uint32_t tm = S.test_mask, gm = S.gap_mask;
+    const uint32_t tm0 = __builtin_amdgcn_readfirstlane(S.test_mask), gm0 = __builtin_amdgcn_readfirstlane(S.gap_mask);
+    const uint32_t tm = tm0, gm = gm0;
     const bool fresh = S.fresh;
     const T tau = (T)S.tau;
     const double gamd = S.gamma, boundd = S.bound;
@@ -1215,7 +1216,8 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
 
     const int e = S.e;
     const int cnt = S.cnt;
-    const uint32_t tm = S.test_mask, gm = S.gap_mask;
+    const uint32_t tm0 = __builtin_amdgcn_readfirstlane(S.test_mask), gm0 = __builtin_amdgcn_readfirstlane(S.gap_mask);
+    const uint32_t tm = tm0, gm = gm0;
     const bool fresh = S.fresh;
     const T tau = (T)S.tau;
     const double gamd = S.gamma, boundd = S.bound;
@@ -1325,6 +1327,13 @@ __device__ __forceinline__ void stream_tile(const StepArgs<T> &A, const Tile tl,
         constexpr bool STEADY = decltype(steady)::value;
         constexpr int SL = decltype(slot)::value;
         RowIn &Q = q[SL < 0 ? 0 : SL];
+        // The level masks, made opaque at each row (an empty asm on their
+        // scalar registers): each level's test is then a scalar bit test and
+        // branch in the row, where the compiler would otherwise hoist the
+        // 2 TT tests out of the row loop as per-lane masks (two scalar
+        // registers each, rebuilt through vector instructions at every use).
+        uint32_t gm = gm0, tm = tm0;
+        asm volatile("" : "+s"(gm), "+s"(tm));
 #pragma unroll
         for (int m = TT + 1; m >= 1; --m) {
             dw[m] = dw[m - 1];
