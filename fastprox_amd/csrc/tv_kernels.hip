@@ -1117,22 +1117,22 @@ __global__ __launch_bounds__(256) void step2d_kernel(StepArgs<T> A) {
 template <class T>
 __device__ __forceinline__ T from_left(T v) {  // lane i receives lane i - 1's value (lane 0: 0)
     if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
     } else {
         const long long b = __builtin_bit_cast(long long, v);
-        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xf, 0xf, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, true);
         return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
     }
 }
 template <class T>
 __device__ __forceinline__ T from_right(T v) {  // lane i receives lane i + 1's value (lane 63: 0)
     if constexpr (sizeof(T) == 4) {
-        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+        return __builtin_bit_cast(T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, true));
     } else {
         const long long b = __builtin_bit_cast(long long, v);
-        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xf, 0xf, false);
-        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, false);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xf, 0xf, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xf, 0xf, true);
         return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
     }
 }
@@ -2061,7 +2061,9 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
             continue;
         }
         const int e = S.e, cnt = S.cnt;
-        const uint32_t tm = S.test_mask, gm = S.gap_mask;
+        const uint32_t tm0 = __builtin_amdgcn_readfirstlane(S.test_mask),
+                       gm0 = __builtin_amdgcn_readfirstlane(S.gap_mask);
+        const uint32_t tm = tm0, gm = gm0;
         const bool out_mode = mode == kOutput;
         const bool fresh = S.fresh;
         const double wd0 = S.wa[0], wd1 = S.wa[1], wd2 = S.wa[2];
@@ -2128,6 +2130,10 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
         load_slice(sbeg);
         for (int s = sbeg; s <= send; ++s) {
             const int p = s & 1, pp = p ^ 1;
+            // the level masks, opaque at each slice: scalar bit tests here
+            // rather than per-lane masks hoisted out of the loop
+            uint32_t gm = gm0, tm = tm0;
+            asm volatile("" : "+s"(gm), "+s"(tm));
             // Level 0: slice s.
             T cv[TT + 1][3], cu[TT + 1][3];
 #pragma unroll
