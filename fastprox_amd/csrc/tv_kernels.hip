@@ -135,11 +135,13 @@ __device__ __forceinline__ P uniform(P p) {
 }
 
 // x 2^-e: exact unless the result leaves the type's range, so float64 and
-// float32 give the correctly rounded product directly (one ldexp).
+// float32 give the correctly rounded product directly (one ldexp). With e = 0
+// the ldexp returns v itself (denormals are kept in both precisions), so no
+// select on e is needed.
 template <class T>
 __device__ __forceinline__ T scale_in(T v, int e) {
-    if constexpr (sizeof(T) == 4) return e ? ldexpf(v, -e) : v;
-    else return e ? ldexp(v, -e) : v;
+    if constexpr (sizeof(T) == 4) return ldexpf(v, -e);
+    else return ldexp(v, -e);
 }
 
 // The radius of pixel p with pixel weights: tv_iso.cpp:180-185.
