@@ -2130,168 +2130,178 @@ __global__ __launch_bounds__(64 * kS3TWaves) void stream3t_kernel(StepArgs<float
         const int sbeg = z0 - TT;
         const int send = z1 - 1 + 2 * TT + (out_mode ? 1 : 0);
         load_slice(sbeg);
-        for (int s = sbeg; s <= send; ++s) {
-            const int p = s & 1, pp = p ^ 1;
-            // the level masks, opaque at each slice: scalar bit tests here
-            // rather than per-lane masks hoisted out of the loop
-            uint32_t gm = gm0, tm = tm0;
-            asm volatile("" : "+s"(gm), "+s"(tm));
-            // Level 0: slice s.
-            T cv[TT + 1][3], cu[TT + 1][3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                cv[0][a] = nv[a];
-                cu[0][a] = nu[a];
-            }
-#pragma unroll
-            for (int k = ND - 1; k >= 1; --k) dq[k] = dq[k - 1];
-            dq[0] = scale_in(nd, e);
-            if (s < send) load_slice(s + 1);
-#pragma unroll
-            for (int m = 0; m <= TT; ++m) {
-                const int t = s - 2 * m;  // the slice level m updates
-                if (m >= 1) {
-                    // UPDATE(m) at slice t from level m - 1 (its ring's A is slice t).
-                    const bool upd = t >= z0 - (TT - m) && t <= z1 - 1 + (TT - m);
-                    if (upd && m <= cnt) {
-                        const T zc = za[m - 1];
-                        const T zyn = wv + 1 < NW ? X.z[pp][wv + 1][m - 1][lane] : (T)0;
-                        const T zxn = from_right(zc);
-                        // outside the image every c is 0, so v stays 0 there
-                        const bool tin = in_slice(t);
-                        const T g0 = gw0_at(t) * (zb[m - 1] - zc);
-                        const T c0 = act0 && tin ? madd(tau, g0, av[m - 1][0]) : (T)0;
-                        const T c1 = tin ? madd(tw1, zyn - zc, av[m - 1][1]) : (T)0;
-                        const T c2 = tin ? madd(tw2, zxn - zc, av[m - 1][2]) : (T)0;
-                        T n0, n1, n2;
-                        if constexpr (ISO) {
-                            const T sc = ball_scale(madd(c2, c2, madd(c1, c1, c0 * c0)), gam);
-                            n0 = c0 * sc;
-                            n1 = c1 * sc;
-                            n2 = c2 * sc;
-                        } else {
-                            n0 = c0 > gam ? gam : (c0 < -gam ? -gam : c0);
-                            n1 = c1 > gam ? gam : (c1 < -gam ? -gam : c1);
-                            n2 = c2 > gam ? gam : (c2 < -gam ? -gam : c2);
-                        }
-                        if ((tm >> m) & 1u) {
-                            const T dd = (av[m - 1][0] - n0) * (n0 - au[m - 1][0]) +
-                                         (av[m - 1][1] - n1) * (n1 - au[m - 1][1]) +
-                                         (av[m - 1][2] - n2) * (n2 - au[m - 1][2]);
-                            if (own && t >= z0 && t < z1) dacc[m] += (double)dd;
-                        }
-                        cv[m][0] = madd(beta[m], n0 - au[m - 1][0], n0);
-                        cv[m][1] = madd(beta[m], n1 - au[m - 1][1], n1);
-                        cv[m][2] = madd(beta[m], n2 - au[m - 1][2], n2);
-                        cu[m][0] = n0;
-                        cu[m][1] = n1;
-                        cu[m][2] = n2;
-                    } else {
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            cv[m][a] = av[m - 1][a];
-                            cu[m][a] = au[m - 1][a];
-                        }
-                    }
-                }
-                if (m < TT) {
-                    // ZCOMP(m): z (and on a gap level z(u)) of slice t - 1 from
-                    // the ring's B (t - 1) and A (t - 2) and the row above.
-                    const int tz = t - 1;
-                    // needed by UPDATE(m + 1) on its slices and the next, and by the gap
-                    const bool zon = tz >= z0 - TT + m + 1 && tz <= z1 - 1 + (TT - m);
-                    if (zon) {
-                        const T v1u = wv > 0 ? X.v1[pp][wv - 1][m][lane] : (T)0;
-                        zb[m] = madd(w2, bv[m][2] - from_left(bv[m][2]),
-                                     madd(w1, bv[m][1] - v1u, madd(w0, bv[m][0] - av[m][0], dq[2 * m + 1])));
-                        if ((gm >> m) & 1u) {
-                            const T u1u = wv > 0 ? X.u1[pp][wv - 1][m][lane] : (T)0;
-                            zub[m] = madd(w2, bu[m][2] - from_left(bu[m][2]),
-                                          madd(w1, bu[m][1] - u1u, madd(w0, bu[m][0] - au[m][0], dq[2 * m + 1])));
-                        }
-                        // The gap of level m at slice t - 2 (its ring's A), a slab slice.
-                        const int tg = t - 2;
-                        if (((gm >> m) & 1u) && tg >= z0 && tg < z1) {
-                            const T zc = zua[m];
-                            const T zyn = wv + 1 < NW ? X.zu[pp][wv + 1][m][lane] : (T)0;
-                            const T zxn = from_right(zc);  // every lane, before the per-lane selects
-                            const bool tin = in_slice(tg);
-                            const T g0 = tin ? gw0_at(tg) * (zub[m] - zc) : (T)0;
-                            const T g1 = tin ? gw1 * (zyn - zc) : (T)0;
-                            const T g2 = tin ? gw2 * (zxn - zc) : (T)0;
-                            T gp;
-                            if constexpr (ISO) {
-                                gp = gam * sqrt32(g0 * g0 + g1 * g1 + g2 * g2) -
-                                     (g0 * au[m][0] + g1 * au[m][1] + g2 * au[m][2]);
-                            } else {
-                                gp = (gam * fabsf(g0) - g0 * au[m][0]) + (gam * fabsf(g1) - g1 * au[m][1]) +
-                                     (gam * fabsf(g2) - g2 * au[m][2]);
-                            }
-                            if (own) gacc[m] += (double)gp;
-                        }
-                    }
-                }
-            }
-            // The final level: the iterate of slice s - 2 TT; on an output
-            // launch, z and the dual of slice s - 2 TT - 1 (from the ring).
-            {
-                const int t = s - 2 * TT;
-                if (cnt > 0 && own && t >= z0 && t < z1) {
-                    const uint32_t o = (uint32_t)(t * SZ) + oxy;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        st32<T>(Ob[a], o, cu[TT][a]);
-                        st32<T>(Ob[3 + a], o, cv[TT][a]);
-                    }
-                }
-                if (out_mode) {
-                    const int to = t - 1;
-                    const T u1u = wv > 0 ? X.u1[pp][wv - 1][TT][lane] : (T)0;
-                    const double ul2 = from_left((double)bu[TT][2]);
-                    if (own && to >= z0 && to < z1) {
-                        const uint32_t o = (uint32_t)(to * SZ) + oxy;
-                        const double zz = (((double)dq[2 * TT + 1] + wd0 * ((double)bu[TT][0] - (double)au[TT][0])) +
-                                           wd1 * ((double)bu[TT][1] - (double)u1u)) +
-                                          wd2 * ((double)bu[TT][2] - ul2);
-                        st32<T>(zo, o, (T)ldexp(zz, e));
-                        if constexpr (ISO) {
-#pragma unroll
-                            for (int a = 0; a < 3; ++a) st32<T>(qo[a], o, (T)ldexp((double)bu[TT][a] / S.wmax, e));
-                        } else {
-                            st32<T>(qo[0], o, (T)(wd0 * ((double)au[TT][0] - (double)bu[TT][0])));
-                            st32<T>(qo[1], o, (T)(wd1 * ((double)u1u - (double)bu[TT][1])));
-                            st32<T>(qo[2], o, (T)(wd2 * (ul2 - (double)bu[TT][2])));
-                        }
-                    }
-                }
-            }
-            // Publish this step's values, rotate the rings, one barrier.
-#pragma unroll
-            for (int m = 0; m < TT; ++m) {
-                X.v1[p][wv][m][lane] = cv[m][1];
-                X.z[p][wv][m][lane] = zb[m];
-                if ((gm >> m) & 1u) {
-                    X.u1[p][wv][m][lane] = cu[m][1];
-                    X.zu[p][wv][m][lane] = zub[m];
-                }
-            }
-            if (out_mode) X.u1[p][wv][TT][lane] = cu[TT][1];
-#pragma unroll
-            for (int m = 0; m <= TT; ++m) {
-#pragma unroll
+        // The slice steps, two per trip at TT 2: the rings' and the delay
+        // line's values are renamed across the pair instead of copied at every
+        // slice (at TT 3 the pair would spill).
+        auto slice_step = [&](const int s) {
+                const int p = s & 1, pp = p ^ 1;
+                // the level masks, opaque at each slice: scalar bit tests here
+                // rather than per-lane masks hoisted out of the loop
+                uint32_t gm = gm0, tm = tm0;
+                asm volatile("" : "+s"(gm), "+s"(tm));
+                // Level 0: slice s.
+                T cv[TT + 1][3], cu[TT + 1][3];
+    #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    av[m][a] = bv[m][a];
-                    au[m][a] = bu[m][a];
-                    bv[m][a] = cv[m][a];
-                    bu[m][a] = cu[m][a];
+                    cv[0][a] = nv[a];
+                    cu[0][a] = nu[a];
                 }
-                if (m < TT) {
-                    za[m] = zb[m];
-                    zua[m] = zub[m];
+    #pragma unroll
+                for (int k = ND - 1; k >= 1; --k) dq[k] = dq[k - 1];
+                dq[0] = scale_in(nd, e);
+                if (s < send) load_slice(s + 1);
+    #pragma unroll
+                for (int m = 0; m <= TT; ++m) {
+                    const int t = s - 2 * m;  // the slice level m updates
+                    if (m >= 1) {
+                        // UPDATE(m) at slice t from level m - 1 (its ring's A is slice t).
+                        const bool upd = t >= z0 - (TT - m) && t <= z1 - 1 + (TT - m);
+                        if (upd && m <= cnt) {
+                            const T zc = za[m - 1];
+                            const T zyn = wv + 1 < NW ? X.z[pp][wv + 1][m - 1][lane] : (T)0;
+                            const T zxn = from_right(zc);
+                            // outside the image every c is 0, so v stays 0 there
+                            const bool tin = in_slice(t);
+                            const T g0 = gw0_at(t) * (zb[m - 1] - zc);
+                            const T c0 = act0 && tin ? madd(tau, g0, av[m - 1][0]) : (T)0;
+                            const T c1 = tin ? madd(tw1, zyn - zc, av[m - 1][1]) : (T)0;
+                            const T c2 = tin ? madd(tw2, zxn - zc, av[m - 1][2]) : (T)0;
+                            T n0, n1, n2;
+                            if constexpr (ISO) {
+                                const T sc = ball_scale(madd(c2, c2, madd(c1, c1, c0 * c0)), gam);
+                                n0 = c0 * sc;
+                                n1 = c1 * sc;
+                                n2 = c2 * sc;
+                            } else {
+                                n0 = c0 > gam ? gam : (c0 < -gam ? -gam : c0);
+                                n1 = c1 > gam ? gam : (c1 < -gam ? -gam : c1);
+                                n2 = c2 > gam ? gam : (c2 < -gam ? -gam : c2);
+                            }
+                            if ((tm >> m) & 1u) {
+                                const T dd = (av[m - 1][0] - n0) * (n0 - au[m - 1][0]) +
+                                             (av[m - 1][1] - n1) * (n1 - au[m - 1][1]) +
+                                             (av[m - 1][2] - n2) * (n2 - au[m - 1][2]);
+                                if (own && t >= z0 && t < z1) dacc[m] += (double)dd;
+                            }
+                            cv[m][0] = madd(beta[m], n0 - au[m - 1][0], n0);
+                            cv[m][1] = madd(beta[m], n1 - au[m - 1][1], n1);
+                            cv[m][2] = madd(beta[m], n2 - au[m - 1][2], n2);
+                            cu[m][0] = n0;
+                            cu[m][1] = n1;
+                            cu[m][2] = n2;
+                        } else {
+    #pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                cv[m][a] = av[m - 1][a];
+                                cu[m][a] = au[m - 1][a];
+                            }
+                        }
+                    }
+                    if (m < TT) {
+                        // ZCOMP(m): z (and on a gap level z(u)) of slice t - 1 from
+                        // the ring's B (t - 1) and A (t - 2) and the row above.
+                        const int tz = t - 1;
+                        // needed by UPDATE(m + 1) on its slices and the next, and by the gap
+                        const bool zon = tz >= z0 - TT + m + 1 && tz <= z1 - 1 + (TT - m);
+                        if (zon) {
+                            const T v1u = wv > 0 ? X.v1[pp][wv - 1][m][lane] : (T)0;
+                            zb[m] = madd(w2, bv[m][2] - from_left(bv[m][2]),
+                                         madd(w1, bv[m][1] - v1u, madd(w0, bv[m][0] - av[m][0], dq[2 * m + 1])));
+                            if ((gm >> m) & 1u) {
+                                const T u1u = wv > 0 ? X.u1[pp][wv - 1][m][lane] : (T)0;
+                                zub[m] = madd(w2, bu[m][2] - from_left(bu[m][2]),
+                                              madd(w1, bu[m][1] - u1u, madd(w0, bu[m][0] - au[m][0], dq[2 * m + 1])));
+                            }
+                            // The gap of level m at slice t - 2 (its ring's A), a slab slice.
+                            const int tg = t - 2;
+                            if (((gm >> m) & 1u) && tg >= z0 && tg < z1) {
+                                const T zc = zua[m];
+                                const T zyn = wv + 1 < NW ? X.zu[pp][wv + 1][m][lane] : (T)0;
+                                const T zxn = from_right(zc);  // every lane, before the per-lane selects
+                                const bool tin = in_slice(tg);
+                                const T g0 = tin ? gw0_at(tg) * (zub[m] - zc) : (T)0;
+                                const T g1 = tin ? gw1 * (zyn - zc) : (T)0;
+                                const T g2 = tin ? gw2 * (zxn - zc) : (T)0;
+                                T gp;
+                                if constexpr (ISO) {
+                                    gp = gam * sqrt32(g0 * g0 + g1 * g1 + g2 * g2) -
+                                         (g0 * au[m][0] + g1 * au[m][1] + g2 * au[m][2]);
+                                } else {
+                                    gp = (gam * fabsf(g0) - g0 * au[m][0]) + (gam * fabsf(g1) - g1 * au[m][1]) +
+                                         (gam * fabsf(g2) - g2 * au[m][2]);
+                                }
+                                if (own) gacc[m] += (double)gp;
+                            }
+                        }
+                    }
                 }
+                // The final level: the iterate of slice s - 2 TT; on an output
+                // launch, z and the dual of slice s - 2 TT - 1 (from the ring).
+                {
+                    const int t = s - 2 * TT;
+                    if (cnt > 0 && own && t >= z0 && t < z1) {
+                        const uint32_t o = (uint32_t)(t * SZ) + oxy;
+    #pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            st32<T>(Ob[a], o, cu[TT][a]);
+                            st32<T>(Ob[3 + a], o, cv[TT][a]);
+                        }
+                    }
+                    if (out_mode) {
+                        const int to = t - 1;
+                        const T u1u = wv > 0 ? X.u1[pp][wv - 1][TT][lane] : (T)0;
+                        const double ul2 = from_left((double)bu[TT][2]);
+                        if (own && to >= z0 && to < z1) {
+                            const uint32_t o = (uint32_t)(to * SZ) + oxy;
+                            const double zz = (((double)dq[2 * TT + 1] + wd0 * ((double)bu[TT][0] - (double)au[TT][0])) +
+                                               wd1 * ((double)bu[TT][1] - (double)u1u)) +
+                                              wd2 * ((double)bu[TT][2] - ul2);
+                            st32<T>(zo, o, (T)ldexp(zz, e));
+                            if constexpr (ISO) {
+    #pragma unroll
+                                for (int a = 0; a < 3; ++a) st32<T>(qo[a], o, (T)ldexp((double)bu[TT][a] / S.wmax, e));
+                            } else {
+                                st32<T>(qo[0], o, (T)(wd0 * ((double)au[TT][0] - (double)bu[TT][0])));
+                                st32<T>(qo[1], o, (T)(wd1 * ((double)u1u - (double)bu[TT][1])));
+                                st32<T>(qo[2], o, (T)(wd2 * (ul2 - (double)bu[TT][2])));
+                            }
+                        }
+                    }
+                }
+                // Publish this step's values, rotate the rings, one barrier.
+    #pragma unroll
+                for (int m = 0; m < TT; ++m) {
+                    X.v1[p][wv][m][lane] = cv[m][1];
+                    X.z[p][wv][m][lane] = zb[m];
+                    if ((gm >> m) & 1u) {
+                        X.u1[p][wv][m][lane] = cu[m][1];
+                        X.zu[p][wv][m][lane] = zub[m];
+                    }
+                }
+                if (out_mode) X.u1[p][wv][TT][lane] = cu[TT][1];
+    #pragma unroll
+                for (int m = 0; m <= TT; ++m) {
+    #pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        av[m][a] = bv[m][a];
+                        au[m][a] = bu[m][a];
+                        bv[m][a] = cv[m][a];
+                        bu[m][a] = cu[m][a];
+                    }
+                    if (m < TT) {
+                        za[m] = zb[m];
+                        zua[m] = zub[m];
+                    }
+                }
+                __syncthreads();
+        };
+        int s = sbeg;
+        if constexpr (TT == 2)
+            for (; s + 1 <= send; s += 2) {
+                slice_step(s);
+                slice_step(s + 1);
             }
-            __syncthreads();
-        }
+        for (; s <= send; ++s) slice_step(s);
         // The tile's partial sums: per wave, then over the waves in order.
 #pragma unroll
         for (int m = 0; m <= TT; ++m) {
